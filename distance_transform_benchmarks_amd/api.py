@@ -1,0 +1,230 @@
+"""Python mirror of the reference GGDT API over torch device tensors.
+
+Names and argument meaning follow dt/include/dt/generalised_distance_transform.hpp:7-27:
+
+  generalised_distance_transform(img, mask, lambda_, v=1e10)         raster schedule
+  generalised_distance_transform_blocks(img, mask, lambda_, v=1e10)  host-driven tile rounds
+  generalised_distance_transform_task(img, mask, lambda_, v=1e10)    persistent tile queue
+
+``img`` is a CUDA (HIP) tensor: uint8 (H, W) grey, uint8 (H, W, 3) RGB8 or
+float32 (H, W).  ``mask`` is uint8 (H, W) (mask != 0 marks a seed, D0 =
+v * (mask == 0), dt/src/utils.cu:12) or float32 (H, W) soft mask s in [0, 1]
+(D0 = v * (1 - s)).  The result is a float32 (H, W) tensor; all three
+schedules give the same bit-exact converged map.  Like the reference the
+functions validate ``0 <= lambda <= 1``; invalid arguments raise ValueError,
+device failures raise DtkError.
+
+Extension keywords (no reference counterpart): ``connectivity`` (4 or 8),
+``max_iterations`` (raster schedule, fixed iteration count), ``tile_rows``,
+``max_sweeps``, ``waves_per_cu`` (tile-solver tuning), ``out`` (write into
+an existing float32 tensor) and ``stats`` (return call statistics).
+
+Every call runs on torch's current stream; the task schedule does not
+synchronise with the host.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional, Sequence
+
+import torch
+
+from . import _lib
+from ._lib import DtkImage, DtkParams, DtkStats, check
+
+__all__ = [
+    "generalised_distance_transform", "generalised_distance_transform_blocks",
+    "generalised_distance_transform_task", "ggdt", "ggdt_batched", "BatchPlan", "predecessors",
+    "backward", "selftest", "tile_count", "round_per_blocks_buffer",
+]
+
+
+def _stream_handle(device) -> ctypes.c_void_p:
+    s = torch.cuda.current_stream(device)
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _as_image(t: torch.Tensor, kind: str) -> DtkImage:
+    """Describe a 2-D pitched device tensor (rows may be padded)."""
+    if not t.is_cuda:
+        raise ValueError(f"{kind} must be a device tensor")
+    if t.dtype == torch.uint8 and t.dim() == 3 and t.shape[2] == 3:
+        if t.stride(2) != 1 or t.stride(1) != 3:
+            raise ValueError(f"{kind}: RGB8 rows must be packed r,g,b")
+        return DtkImage(t.data_ptr(), t.shape[1], t.shape[0], t.stride(0), _lib.DTK_RGB8)
+    if t.dim() != 2:
+        raise ValueError(f"{kind} must be (H, W) (or (H, W, 3) uint8)")
+    if t.stride(1) != 1:
+        raise ValueError(f"{kind}: rows must be contiguous")
+    if t.dtype == torch.uint8:
+        return DtkImage(t.data_ptr(), t.shape[1], t.shape[0], t.stride(0), _lib.DTK_U8)
+    if t.dtype == torch.float32:
+        return DtkImage(t.data_ptr(), t.shape[1], t.shape[0], t.stride(0) * 4, _lib.DTK_F32)
+    raise ValueError(f"{kind}: unsupported dtype {t.dtype}")
+
+
+_SCHED = {"raster": _lib.SCHED_RASTER, "blocks": _lib.SCHED_BLOCKS, "task": _lib.SCHED_TASK}
+
+
+def _params(lambda_, v, connectivity, schedule, max_iterations, tile_rows, max_sweeps, waves_per_cu,
+            round_per_blocks=None) -> DtkParams:
+    p = _lib.default_params()
+    p.lambda_ = float(lambda_)
+    p.v = float(v)
+    if connectivity not in (4, 8):
+        raise ValueError("connectivity must be 4 or 8")
+    p.connectivity = connectivity
+    p.schedule = _SCHED[schedule]
+    p.max_iterations = int(max_iterations)
+    p.tile_rows = int(tile_rows)
+    p.max_sweeps = int(max_sweeps)
+    p.waves_per_cu = int(waves_per_cu)
+    if round_per_blocks is not None:
+        if round_per_blocks.dtype != torch.int32 or not round_per_blocks.is_cuda:
+            raise ValueError("round_per_blocks must be an int32 device tensor")
+        p.round_per_blocks = round_per_blocks.data_ptr()
+    return p
+
+
+def ggdt(img: torch.Tensor, mask: torch.Tensor, lambda_: float, v: float = 1e10, *, connectivity: int = 8,
+         schedule: str = "task", max_iterations: int = 0, tile_rows: int = 0, max_sweeps: int = 0,
+         waves_per_cu: int = 0, out: Optional[torch.Tensor] = None, stats: bool = False,
+         round_per_blocks: Optional[torch.Tensor] = None):
+    """GGDT of one image (any schedule).  Returns D, or (D, stats dict)."""
+    H, W = mask.shape[0], mask.shape[1]
+    if img.shape[0] != H or img.shape[1] != W:
+        raise ValueError("image and mask sizes differ")
+    if out is None:
+        out = torch.empty((H, W), dtype=torch.float32, device=img.device)
+    p = _params(lambda_, v, connectivity, schedule, max_iterations, tile_rows, max_sweeps, waves_per_cu,
+                round_per_blocks)
+    di, dm, dd = _as_image(img, "image"), _as_image(mask, "mask"), _as_image(out, "dist")
+    st = DtkStats()
+    with torch.cuda.device(img.device):
+        check(_lib.load().dtk_ggdt(ctypes.byref(di), ctypes.byref(dm), ctypes.byref(dd), ctypes.byref(p),
+                                   _stream_handle(img.device), ctypes.byref(st) if stats else None))
+    return (out, st.as_dict()) if stats else out
+
+
+def generalised_distance_transform(img, mask, lambda_, v=1e10, **kw):
+    """Reference raster schedule (dt/src/generalised_distance_transform.cu:101-148)."""
+    return ggdt(img, mask, lambda_, v, schedule="raster", **kw)
+
+
+def generalised_distance_transform_blocks(img, mask, lambda_, v=1e10, round_per_blocks=None, **kw):
+    """Reference blocks schedule (dt/src/generalised_distance_transform_blocks.cu:114-184)."""
+    return ggdt(img, mask, lambda_, v, schedule="blocks", round_per_blocks=round_per_blocks, **kw)
+
+
+def generalised_distance_transform_task(img, mask, lambda_, v=1e10, **kw):
+    """Reference task schedule (dt/src/generalised_distance_transform_task.cu:152-207)."""
+    return ggdt(img, mask, lambda_, v, schedule="task", **kw)
+
+
+def tile_count(width: int, height: int, tile_rows: int = 0) -> int:
+    return _lib.load().dtk_tile_count(width, height, tile_rows)
+
+
+def round_per_blocks_buffer(width: int, height: int, device="cuda") -> torch.Tensor:
+    """Zeroed int32 visit counters for the blocks schedule's instrumentation."""
+    return torch.zeros(tile_count(width, height, 0), dtype=torch.int32, device=device)
+
+
+def _batch_arrays(imgs, masks, outs):
+    n = len(imgs)
+    A = DtkImage * max(n, 1)
+    ai, am, ad = A(), A(), A()
+    for i in range(n):
+        ai[i] = _as_image(imgs[i], "image")
+        am[i] = _as_image(masks[i], "mask")
+        ad[i] = _as_image(outs[i], "dist")
+    return ai, am, ad
+
+
+def ggdt_batched(imgs: Sequence[torch.Tensor], masks: Sequence[torch.Tensor], lambda_: float, v: float = 1e10, *,
+                 connectivity: int = 8, tile_rows: int = 0, max_sweeps: int = 0, waves_per_cu: int = 0,
+                 outs: Optional[List[torch.Tensor]] = None, stats: bool = False):
+    """GGDT of many images of any sizes in one persistent launch."""
+    if len(imgs) != len(masks):
+        raise ValueError("imgs and masks differ in length")
+    if outs is None:
+        outs = [torch.empty(m.shape[:2], dtype=torch.float32, device=m.device) for m in masks]
+    p = _params(lambda_, v, connectivity, "task", 0, tile_rows, max_sweeps, waves_per_cu)
+    ai, am, ad = _batch_arrays(imgs, masks, outs)
+    st = DtkStats()
+    dev = imgs[0].device if imgs else torch.device("cuda")
+    with torch.cuda.device(dev):
+        check(_lib.load().dtk_ggdt_batched(len(imgs), ai, am, ad, ctypes.byref(p), _stream_handle(dev),
+                                           ctypes.byref(st) if stats else None))
+    return (outs, st.as_dict()) if stats else outs
+
+
+class BatchPlan:
+    """A batched GGDT captured as a HIP graph over fixed buffers.
+
+    The tensors given at creation are bound into the graph: refill them in
+    place and call ``launch()`` to recompute every ``outs[i]``.
+    """
+
+    def __init__(self, imgs, masks, lambda_, v=1e10, *, connectivity=8, tile_rows=0, max_sweeps=0,
+                 waves_per_cu=0, outs=None):
+        if outs is None:
+            outs = [torch.empty(m.shape[:2], dtype=torch.float32, device=m.device) for m in masks]
+        self.imgs, self.masks, self.outs = list(imgs), list(masks), list(outs)
+        self.device = self.imgs[0].device
+        p = _params(lambda_, v, connectivity, "task", 0, tile_rows, max_sweeps, waves_per_cu)
+        ai, am, ad = _batch_arrays(self.imgs, self.masks, self.outs)
+        self._plan = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            torch.cuda.synchronize(self.device)
+            check(_lib.load().dtk_batch_plan_create(len(self.imgs), ai, am, ad, ctypes.byref(p), None,
+                                                    ctypes.byref(self._plan)))
+
+    def launch(self):
+        with torch.cuda.device(self.device):
+            check(_lib.load().dtk_batch_plan_launch(self._plan, _stream_handle(self.device)))
+        return self.outs
+
+    def close(self):
+        if self._plan:
+            _lib.load().dtk_batch_plan_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def predecessors(img, mask, dist, lambda_, v=1e10, *, connectivity=8, out=None) -> torch.Tensor:
+    """Argmin predecessor codes (uint8: 0..7 in dt::c8 order, 8 = root)."""
+    if out is None:
+        out = torch.empty(dist.shape, dtype=torch.uint8, device=dist.device)
+    p = _params(lambda_, v, connectivity, "task", 0, 0, 0, 0)
+    di, dm, dd, dp = (_as_image(img, "image"), _as_image(mask, "mask"), _as_image(dist, "dist"),
+                      _as_image(out, "pred"))
+    with torch.cuda.device(img.device):
+        check(_lib.load().dtk_ggdt_pred(ctypes.byref(di), ctypes.byref(dm), ctypes.byref(dd), ctypes.byref(dp),
+                                        ctypes.byref(p), _stream_handle(img.device)))
+    return out
+
+
+def backward(img, mask, dist, pred, grad_dist, lambda_, v=1e10, *, connectivity=8):
+    """Adjoint of sum(grad_dist * D): returns (dL/dimg, dL/dmask)."""
+    gi = torch.empty_like(img, dtype=torch.float32)
+    gm = torch.empty(mask.shape, dtype=torch.float32, device=mask.device)
+    p = _params(lambda_, v, connectivity, "task", 0, 0, 0, 0)
+    gd = grad_dist.contiguous()
+    ims = [_as_image(t, n) for t, n in ((img, "image"), (mask, "mask"), (dist, "dist"), (pred, "pred"),
+                                        (gd, "grad_dist"), (gi, "grad_img"), (gm, "grad_mask"))]
+    with torch.cuda.device(img.device):
+        check(_lib.load().dtk_ggdt_backward(*[ctypes.byref(x) for x in ims], ctypes.byref(p),
+                                            _stream_handle(img.device)))
+    return gi, gm
+
+
+def selftest() -> dict:
+    out = (ctypes.c_int * 3)()
+    check(_lib.load().dtk_selftest(out))
+    return {"dpp_ok": bool(out[0]), "sqrt_mismatches": int(out[1]), "cu": int(out[2])}

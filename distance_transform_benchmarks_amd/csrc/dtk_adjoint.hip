@@ -1,0 +1,247 @@
+// Argmin predecessors and the adjoint of the GGDT (gfx950).  No reference
+// counterpart: the reference GGDT is forward-only.
+//
+// Forward (converged D): pred(p) = 8 when D(p) == D0(p) (a root), else the
+// first neighbour q in dt::c8 order (dt/include/dt/c8.hpp:19-28) whose
+// relaxation (D(q) + e) + g reproduces D(p) exactly and whose (D, index)
+// pair is lexicographically below p's, which keeps the forest acyclic.
+//
+// Backward of L = sum(W * D), rounding treated as exact:
+//   G(p) = W(p) + sum over children c of G(c)         (subtree sums)
+//   edge q -> p : dL/dI(p) += G(p) * l_grad * sign(I(p) - I(q)), dL/dI(q) -= same
+//   root r      : dL/ds(r) = -nu * G(r)               (D0 = nu * (1 - s))
+// Subtree sums without a sort: every pixel counts its children; one thread
+// starts at each leaf and climbs while it is the last child to arrive at
+// the parent (atomic counter), so each pixel is finished exactly once,
+// after all of its children.
+#include <hip/hip_runtime.h>
+
+#include "dtk_launch.hpp"
+#include "dtk_tile.hpp"
+
+namespace dtk
+{
+  __device__ constexpr int kOffX[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
+  __device__ constexpr int kOffY[8] = {0, -1, -1, -1, 0, 1, 1, 1};
+
+  __device__ __forceinline__ float rd_f(const unsigned char* base, long long pitch, int x, int y)
+  {
+    return reinterpret_cast<const float*>(base + (long long)y * pitch)[x];
+  }
+
+  template <int KIND>
+  __global__ void k_pred(ImgDesc d, KParams p, int c8)
+  {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= d.W || y >= d.H)
+      return;
+    const unsigned char* Db = reinterpret_cast<const unsigned char*>(d.D);
+    const float          Dp = rd_f(Db, d.D_pitch, x, y);
+    float                d0;
+    if (p.mask_kind == MASK_U8)
+      d0 = p.v * (float)(d.mask[(long long)y * d.mask_pitch + x] == 0);
+    else
+      d0 = p.v * (1.f - rd_f(d.mask, d.mask_pitch, x, y));
+    unsigned char code = 8;
+    if (!(Dp == d0))
+    {
+      const uint32_t    ip  = load_pix<KIND>(d.img + (long long)y * d.img_pitch, x);
+      const long long   pid = (long long)y * d.W + x;
+      for (int k = 0; k < 8; ++k)
+      {
+        const bool diag = kOffX[k] != 0 && kOffY[k] != 0;
+        if (diag && !c8)
+          continue;
+        const int nx = x + kOffX[k], ny = y + kOffY[k];
+        if (nx < 0 || ny < 0 || nx >= d.W || ny >= d.H)
+          continue;
+        const float     Dq  = rd_f(Db, d.D_pitch, nx, ny);
+        const long long qid = (long long)ny * d.W + nx;
+        if (!(Dq < Dp || (Dq == Dp && qid < pid)))
+          continue;
+        const uint32_t iq = load_pix<KIND>(d.img + (long long)ny * d.img_pitch, nx);
+        float          g  = p.l_grad * pix_dist<KIND>(ip, iq);
+        g                 = (g == g) ? g : DTK_INF;
+        float cur         = Dq + (diag ? p.e_diag : p.e_axis);
+        cur               = cur + g;
+        if (cur == Dp)
+        {
+          code = (unsigned char)k;
+          break;
+        }
+      }
+    }
+    d.pred[(long long)y * d.pred_pitch + x] = code;
+  }
+
+  // Children count of every pixel (gather, no atomics).
+  __global__ void k_child_count(ImgDesc d, uint32_t* counts, float* acc)
+  {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= d.W || y >= d.H)
+      return;
+    uint32_t n = 0;
+    for (int k = 0; k < 8; ++k)
+    {
+      const int cx = x - kOffX[k], cy = y - kOffY[k]; // child c with c + off[k] == p
+      if (cx < 0 || cy < 0 || cx >= d.W || cy >= d.H)
+        continue;
+      n += d.pred[(long long)cy * d.pred_pitch + cx] == k;
+    }
+    counts[(long long)y * d.W + x] = n;
+    acc[(long long)y * d.W + x]    = 0.f;
+  }
+
+  // Leaf-started climbs computing G.
+  __global__ void k_subtree(ImgDesc d, const unsigned char* gd, long long gd_pitch, uint32_t* counts, float* acc,
+                            float* G)
+  {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (x >= d.W || y >= d.H)
+      return;
+    long long id = (long long)y * d.W + x;
+    // Leaves are decided from the (immutable) predecessor map: the counters
+    // are being decremented by climbing threads while this kernel runs.
+    for (int k = 0; k < 8; ++k)
+    {
+      const int cx = x - kOffX[k], cy = y - kOffY[k];
+      if (cx < 0 || cy < 0 || cx >= d.W || cy >= d.H)
+        continue;
+      if (d.pred[(long long)cy * d.pred_pitch + cx] == k)
+        return;
+    }
+    for (;;)
+    {
+      const float a = __hip_atomic_load(&acc[id], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const float g = rd_f(gd, gd_pitch, x, y) + a;
+      G[id]         = g;
+      const int code = d.pred[(long long)y * d.pred_pitch + x];
+      if (code >= 8 || x + kOffX[code] < 0 || y + kOffY[code] < 0 || x + kOffX[code] >= d.W ||
+          y + kOffY[code] >= d.H)
+        return; // root (or an invalid code, treated as a root)
+      x += kOffX[code];
+      y += kOffY[code];
+      id = (long long)y * d.W + x;
+      __hip_atomic_fetch_add(&acc[id], g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // The last child to arrive continues with the parent; acq_rel orders
+      // every sibling's acc add before the parent's read of acc.
+      if (__hip_atomic_fetch_sub(&counts[id], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) != 1u)
+        return;
+    }
+  }
+
+  __device__ __forceinline__ float sgnf(float v) { return v > 0.f ? 1.f : (v < 0.f ? -1.f : 0.f); }
+
+  template <int KIND>
+  __device__ __forceinline__ float img_val(const ImgDesc& d, int x, int y)
+  {
+    return __uint_as_float(load_pix<IMG_F32>(d.img + (long long)y * d.img_pitch, x));
+  }
+
+  __global__ void k_grad(ImgDesc d, KParams p, const float* G, unsigned char* gi, long long gi_pitch,
+                         unsigned char* gm, long long gm_pitch)
+  {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= d.W || y >= d.H)
+      return;
+    const long long id   = (long long)y * d.W + x;
+    const float     ip   = img_val<IMG_F32>(d, x, y);
+    int             code = d.pred[(long long)y * d.pred_pitch + x];
+    if (code < 8 && (x + kOffX[code] < 0 || y + kOffY[code] < 0 || x + kOffX[code] >= d.W || y + kOffY[code] >= d.H))
+      code = 8; // invalid code: treated as a root
+    float acc = 0.f;
+    if (code < 8)
+    {
+      const float iq = img_val<IMG_F32>(d, x + kOffX[code], y + kOffY[code]);
+      acc += G[id] * p.l_grad * sgnf(ip - iq);
+    }
+    for (int k = 0; k < 8; ++k)
+    {
+      const int cx = x - kOffX[k], cy = y - kOffY[k];
+      if (cx < 0 || cy < 0 || cx >= d.W || cy >= d.H)
+        continue;
+      if (d.pred[(long long)cy * d.pred_pitch + cx] != k)
+        continue;
+      const float ic = img_val<IMG_F32>(d, cx, cy);
+      acc -= G[(long long)cy * d.W + cx] * p.l_grad * sgnf(ic - ip);
+    }
+    reinterpret_cast<float*>(gi + (long long)y * gi_pitch)[x] = acc;
+    if (gm)
+      reinterpret_cast<float*>(gm + (long long)y * gm_pitch)[x] = (code >= 8) ? -p.v * G[id] : 0.f;
+  }
+
+  hipError_t launch_pred(const ImgDesc& d, const KParams& p, bool c8, hipStream_t s)
+  {
+    if (d.W <= 0 || d.H <= 0)
+      return hipSuccess;
+    const dim3 grid((d.W + 255) / 256, d.H), blk(256);
+    switch (p.img_kind)
+    {
+    case IMG_U8:
+      hipLaunchKernelGGL(k_pred<IMG_U8>, grid, blk, 0, s, d, p, (int)c8);
+      break;
+    case IMG_RGB8:
+      hipLaunchKernelGGL(k_pred<IMG_RGB8>, grid, blk, 0, s, d, p, (int)c8);
+      break;
+    default:
+      hipLaunchKernelGGL(k_pred<IMG_F32>, grid, blk, 0, s, d, p, (int)c8);
+      break;
+    }
+    return hipGetLastError();
+  }
+
+  hipError_t launch_backward(const ImgDesc& d, const KParams& p, bool c8, const unsigned char* grad_dist,
+                             long long gd_pitch, unsigned char* grad_img, long long gi_pitch, unsigned char* grad_mask,
+                             long long gm_pitch, float* G, uint32_t* counts, float* acc, hipStream_t s)
+  {
+    (void)c8;
+    if (d.W <= 0 || d.H <= 0)
+      return hipSuccess;
+    const dim3 grid((d.W + 255) / 256, d.H), blk(256);
+    hipLaunchKernelGGL(k_child_count, grid, blk, 0, s, d, counts, acc);
+    hipLaunchKernelGGL(k_subtree, grid, blk, 0, s, d, grad_dist, gd_pitch, counts, acc, G);
+    hipLaunchKernelGGL(k_grad, grid, blk, 0, s, d, p, G, grad_img, gi_pitch, grad_mask, gm_pitch);
+    return hipGetLastError();
+  }
+
+  // ---- self-test ---------------------------------------------------------
+  __global__ void k_selftest_dpp(int* out)
+  {
+    const int   lane = threadIdx.x;
+    const float v    = (float)(lane + 1);
+    const float a    = shr_z(v);           // expect lane     (value of lane-1), 0 at lane 0
+    const float b    = shl_z(v);           // expect lane + 2 (value of lane+1), 0 at lane 63
+    const float c    = shr_keep(-1.f, v);  // lane 0 keeps -1
+    const bool  ok   = a == (lane == 0 ? 0.f : (float)lane) && b == (lane == 63 ? 0.f : (float)(lane + 2)) &&
+                    c == (lane == 0 ? -1.f : (float)lane);
+    // Integer shifts feeding an integer subtract (the uint8 cost pattern).
+    const uint32_t u  = (uint32_t)(lane * 37 % 256);
+    const uint32_t ul = shl_u(u), ur = shr_u(u);
+    const int      dl = (int)u - (int)ul, dr = (int)u - (int)ur;
+    const int      el = (int)u - (lane == 63 ? 0 : (int)((lane + 1) * 37 % 256));
+    const int      er = (int)u - (lane == 0 ? 0 : (int)((lane - 1) * 37 % 256));
+    const bool     ok2 = dl == el && dr == er && pix_dist<IMG_U8>(u, ul) == (float)(el < 0 ? -el : el);
+    const uint64_t all = __ballot(ok && ok2);
+    if (lane == 0)
+      out[0] = all == ~0ull ? 1 : 0;
+  }
+
+  __global__ void k_selftest_sqrt(float* out, int n)
+  {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+      out[i] = __builtin_sqrtf((float)i);
+  }
+
+  hipError_t launch_selftest(int* dpp_out, float* sqrt_out, hipStream_t s)
+  {
+    hipLaunchKernelGGL(k_selftest_dpp, dim3(1), dim3(64), 0, s, dpp_out);
+    const int n = 3 * 255 * 255 + 1;
+    hipLaunchKernelGGL(k_selftest_sqrt, dim3((n + 255) / 256), dim3(256), 0, s, sqrt_out, n);
+    return hipGetLastError();
+  }
+} // namespace dtk

@@ -1,0 +1,868 @@
+// Host side of libdtk: the C ABI declared in include/dtk.h.
+//
+// Error model: every entry point returns a dtk_status; a message for the
+// calling thread is kept for dtk_last_error().  Nothing throws across the
+// ABI (the C++ wrappers in include/dt/ turn codes into the reference's
+// exceptions, dt/src/generalised_distance_transform.cu:136-137).
+//
+// Memory: each device has one cached workspace (grown on demand with
+// hipMalloc, freed only when it must grow) for the queue, tile statistics,
+// batch tables and adjoint scratch.  Calls on one device share it, so calls
+// that overlap on one device must be ordered on one stream (the reference
+// is not re-entrant either: it rewrites a __constant__ symbol per call,
+// dt/src/generalised_distance_transform.cu:112).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "dtk.h"
+#include "dtk_launch.hpp"
+
+using namespace dtk;
+
+namespace
+{
+  thread_local std::string g_err;
+
+  dtk_status fail(dtk_status s, const std::string& msg)
+  {
+    g_err = msg;
+    return s;
+  }
+
+#define DTK_HIP(expr, what)                                                                                           \
+  do                                                                                                                  \
+  {                                                                                                                   \
+    hipError_t e_ = (expr);                                                                                           \
+    if (e_ != hipSuccess)                                                                                             \
+      return fail(e_ == hipErrorOutOfMemory ? DTK_ERR_OUT_OF_MEMORY : DTK_ERR_HIP,                                    \
+                  std::string(what) + ": " + hipGetErrorString(e_));                                                  \
+  } while (0)
+
+  constexpr int kMaxDevices = 64;
+
+  struct Workspace
+  {
+    std::mutex mu;
+    void*      dbuf = nullptr;
+    size_t     dcap = 0;
+    void*      hbuf = nullptr; // pinned host staging
+    size_t     hcap = 0;
+    int        cu   = 0;
+  };
+
+  Workspace g_ws[kMaxDevices];
+
+  int elem_size(int dtype)
+  {
+    switch (dtype)
+    {
+    case DTK_U8:
+      return 1;
+    case DTK_RGB8:
+      return 3;
+    case DTK_F32:
+      return 4;
+    default:
+      return 0;
+    }
+  }
+
+  size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+  uint32_t next_pow2(uint32_t v)
+  {
+    uint32_t p = 1;
+    while (p < v)
+      p <<= 1;
+    return p;
+  }
+
+  dtk_status current_ws(Workspace** out, int* dev_out)
+  {
+    int dev = 0;
+    DTK_HIP(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices)
+      return fail(DTK_ERR_UNSUPPORTED, "device index out of range");
+    Workspace& w = g_ws[dev];
+    if (w.cu == 0)
+    {
+      int cu = 0;
+      DTK_HIP(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+      w.cu = cu > 0 ? cu : 1;
+    }
+    *out = &w;
+    if (dev_out)
+      *dev_out = dev;
+    return DTK_OK;
+  }
+
+  dtk_status ws_reserve(Workspace& w, size_t dbytes, size_t hbytes)
+  {
+    if (dbytes > w.dcap)
+    {
+      if (w.dbuf)
+      {
+        DTK_HIP(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        DTK_HIP(hipFree(w.dbuf), "hipFree");
+        w.dbuf = nullptr;
+        w.dcap = 0;
+      }
+      const size_t cap = align_up(dbytes + dbytes / 4, 1 << 20);
+      DTK_HIP(hipMalloc(&w.dbuf, cap), "hipMalloc(workspace)");
+      w.dcap = cap;
+    }
+    if (hbytes > w.hcap)
+    {
+      if (w.hbuf)
+      {
+        DTK_HIP(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        DTK_HIP(hipHostFree(w.hbuf), "hipHostFree");
+        w.hbuf = nullptr;
+        w.hcap = 0;
+      }
+      const size_t cap = align_up(hbytes, 1 << 16);
+      DTK_HIP(hipHostMalloc(&w.hbuf, cap, hipHostMallocDefault), "hipHostMalloc(staging)");
+      w.hcap = cap;
+    }
+    return DTK_OK;
+  }
+
+  // Carves aligned pieces out of a base pointer.
+  struct Carver
+  {
+    char*  base;
+    size_t off = 0;
+    explicit Carver(void* b)
+      : base(static_cast<char*>(b))
+    {
+    }
+    template <typename T>
+    T* take(size_t count)
+    {
+      off      = align_up(off, 256);
+      T* p     = base ? reinterpret_cast<T*>(base + off) : nullptr;
+      off += count * sizeof(T);
+      return p;
+    }
+  };
+
+  dtk_status check_image(const dtk_image* im, const char* name, int need_dtype_mask)
+  {
+    if (!im)
+      return fail(DTK_ERR_INVALID_ARGUMENT, std::string(name) + " is NULL");
+    const int es = elem_size(im->dtype);
+    if (es == 0 || !((need_dtype_mask >> im->dtype) & 1))
+      return fail(DTK_ERR_INVALID_ARGUMENT, std::string(name) + ": unsupported dtype");
+    if (im->width < 0 || im->height < 0)
+      return fail(DTK_ERR_INVALID_ARGUMENT, std::string(name) + ": negative size");
+    if ((long long)im->width * im->height > 0)
+    {
+      if (!im->data)
+        return fail(DTK_ERR_INVALID_ARGUMENT, std::string(name) + ": NULL data");
+      if (im->pitch < (long long)im->width * es)
+        return fail(DTK_ERR_INVALID_ARGUMENT, std::string(name) + ": pitch smaller than a row");
+      if (im->dtype == DTK_F32 && ((uintptr_t)im->data % 4 != 0 || im->pitch % 4 != 0))
+        return fail(DTK_ERR_INVALID_ARGUMENT, std::string(name) + ": float rows must be 4-byte aligned");
+    }
+    return DTK_OK;
+  }
+
+  dtk_status check_params(const dtk_params* p)
+  {
+    if (!p)
+      return fail(DTK_ERR_INVALID_ARGUMENT, "params is NULL");
+    if (!(p->lambda >= 0.f && p->lambda <= 1.f))
+      return fail(DTK_ERR_INVALID_ARGUMENT, "lambda must be in [0, 1]");
+    if (!(p->v >= 0.f) || std::isinf(p->v))
+      return fail(DTK_ERR_INVALID_ARGUMENT, "v must be finite and >= 0");
+    if (p->connectivity != DTK_C4 && p->connectivity != DTK_C8)
+      return fail(DTK_ERR_INVALID_ARGUMENT, "connectivity must be DTK_C4 or DTK_C8");
+    if (p->schedule < DTK_SCHED_RASTER || p->schedule > DTK_SCHED_TASK)
+      return fail(DTK_ERR_INVALID_ARGUMENT, "unknown schedule");
+    if (p->tile_rows != 0 && p->tile_rows != 32 && p->tile_rows != 16)
+      return fail(DTK_ERR_INVALID_ARGUMENT, "tile_rows must be 0, 16 or 32");
+    if (p->max_iterations < 0 || p->max_sweeps < 0 || p->waves_per_cu < 0)
+      return fail(DTK_ERR_INVALID_ARGUMENT, "negative iteration / sweep / wave count");
+    if (p->max_iterations > 0 && p->schedule != DTK_SCHED_RASTER)
+      return fail(DTK_ERR_INVALID_ARGUMENT, "a fixed iteration count needs the raster schedule");
+    return DTK_OK;
+  }
+
+  KParams make_kparams(const dtk_params* p, int img_dtype, int mask_dtype, int rh)
+  {
+    KParams k{};
+    k.img_kind    = img_dtype == DTK_U8 ? IMG_U8 : (img_dtype == DTK_RGB8 ? IMG_RGB8 : IMG_F32);
+    k.mask_kind   = mask_dtype == DTK_U8 ? MASK_U8 : MASK_F32;
+    k.l_grad      = p->lambda;
+    const float le = 1.f - p->lambda; // dt/src/generalised_distance_transform.cu:108-111
+    k.e_axis      = le * 1.f;
+    k.e_diag      = le * std::sqrt(2.f);
+    k.v           = p->v;
+    k.zero_lambda = p->lambda == 0.f;
+    k.max_sweeps  = p->max_sweeps;
+    k.rh          = rh;
+    return k;
+  }
+
+  bool two_adds(const dtk_params* p) { return !(p->lambda == 0.f || p->lambda == 1.f); }
+
+  int region_rows(const dtk_params* p) { return p->tile_rows == 0 ? 16 : p->tile_rows; }
+
+  ImgDesc make_desc(const dtk_image* img, const dtk_image* mask, dtk_image* dist, int rh)
+  {
+    ImgDesc d{};
+    d.img        = static_cast<const unsigned char*>(img->data);
+    d.img_pitch  = img->pitch;
+    d.mask       = static_cast<const unsigned char*>(mask->data);
+    d.mask_pitch = mask->pitch;
+    d.D          = static_cast<float*>(dist->data);
+    d.D_pitch    = dist->pitch;
+    d.pred       = nullptr;
+    d.pred_pitch = 0;
+    d.W          = img->width;
+    d.H          = img->height;
+    d.tiles_x    = (d.W + kTileW - 1) / kTileW;
+    d.tiles_y    = (d.H + (rh - 2) - 1) / (rh - 2);
+    d.tile0      = 0;
+    return d;
+  }
+
+  dtk_status check_triplet(const dtk_image* img, const dtk_image* mask, const dtk_image* dist)
+  {
+    dtk_status s;
+    if ((s = check_image(img, "image", (1 << DTK_U8) | (1 << DTK_RGB8) | (1 << DTK_F32))) != DTK_OK)
+      return s;
+    if ((s = check_image(mask, "mask", (1 << DTK_U8) | (1 << DTK_F32))) != DTK_OK)
+      return s;
+    if ((s = check_image(dist, "dist", 1 << DTK_F32)) != DTK_OK)
+      return s;
+    if (img->width != mask->width || img->height != mask->height || img->width != dist->width ||
+        img->height != dist->height)
+      return fail(DTK_ERR_INVALID_ARGUMENT, "image, mask and dist sizes differ");
+    return DTK_OK;
+  }
+
+  int task_workers(const Workspace& w, const dtk_params* p, int S, bool c8, bool two, int ntiles)
+  {
+    int per_cu = task_occupancy(S, c8, two);
+    if (per_cu <= 0)
+      per_cu = 1;
+    if (p->waves_per_cu > 0 && p->waves_per_cu < per_cu)
+      per_cu = p->waves_per_cu;
+    long long n = (long long)per_cu * w.cu;
+    if (n > ntiles)
+      n = ntiles;
+    return (int)(n > 0 ? n : 1);
+  }
+
+  // Task-schedule buffers of one call, carved from a device block.
+  struct TaskBufs
+  {
+    Queue     q{};
+    ImgDesc*  table    = nullptr;
+    uint32_t* tile_img = nullptr;
+  };
+
+  size_t task_layout(void* base, int ntiles, uint32_t cap, int nimg, TaskBufs* out)
+  {
+    Carver c(base);
+    TaskBufs b;
+    b.q.ctrl     = c.take<uint32_t>(CTRL_WORDS);
+    b.q.tmin     = c.take<float>(ntiles);
+    b.q.tmax     = c.take<float>(ntiles);
+    b.q.state    = c.take<uint32_t>(ntiles);
+    b.q.slots    = c.take<unsigned long long>(cap);
+    b.q.cap_mask = cap - 1;
+    if (nimg > 1)
+    {
+      b.table    = c.take<ImgDesc>(nimg);
+      b.tile_img = c.take<uint32_t>(ntiles);
+    }
+    if (out)
+      *out = b;
+    return c.off + 256;
+  }
+
+  dtk_status fill_stats(dtk_stats* st, const uint32_t* ctrl_host, int rounds, int tiles, int rh, int workers)
+  {
+    if (!st)
+      return DTK_OK;
+    st->rounds    = rounds;
+    st->visits    = ctrl_host ? ctrl_host[CTRL_VISITS] : 0;
+    st->sweeps    = ctrl_host ? ctrl_host[CTRL_SWEEPS] : 0;
+    st->tiles     = tiles;
+    st->tile_rows = rh;
+    st->workers   = workers;
+    return DTK_OK;
+  }
+
+  // Builds the batch descriptors (host) for n images.
+  dtk_status build_batch(int n, const dtk_image* imgs, const dtk_image* masks, dtk_image* dists, const dtk_params* p,
+                         int rh, std::vector<ImgDesc>& descs, std::vector<uint32_t>& tile_img, int* ntiles)
+  {
+    descs.resize(n);
+    long long total = 0;
+    for (int i = 0; i < n; ++i)
+    {
+      dtk_status s = check_triplet(&imgs[i], &masks[i], &dists[i]);
+      if (s != DTK_OK)
+        return fail(s, "image " + std::to_string(i) + ": " + g_err);
+      if (imgs[i].dtype != imgs[0].dtype || masks[i].dtype != masks[0].dtype)
+        return fail(DTK_ERR_INVALID_ARGUMENT, "all images of a batch must share image and mask dtypes");
+      descs[i]       = make_desc(&imgs[i], &masks[i], &dists[i], rh);
+      descs[i].tile0 = (int)total;
+      if (descs[i].W > 0 && descs[i].H > 0)
+        total += (long long)descs[i].tiles_x * descs[i].tiles_y;
+      else
+        descs[i].tiles_x = descs[i].tiles_y = 0;
+      if (total > (1ll << 30))
+        return fail(DTK_ERR_UNSUPPORTED, "too many tiles in one batch");
+    }
+    tile_img.resize((size_t)total);
+    for (int i = 0; i < n; ++i)
+      for (int t = 0; t < descs[i].tiles_x * descs[i].tiles_y; ++t)
+        tile_img[(size_t)descs[i].tile0 + t] = (uint32_t)i;
+    *ntiles = (int)total;
+    (void)p;
+    return DTK_OK;
+  }
+
+  // Enqueue reset + init + seed + persistent solve for a prepared batch.
+  dtk_status enqueue_task(const Batch& b, const KParams& kp, const Queue& q, int S, bool c8, bool two, int workers,
+                          hipStream_t s)
+  {
+    DTK_HIP(launch_reset_ctrl(q.ctrl, s), "launch reset");
+    DTK_HIP(launch_init(b, kp, q, s), "launch init");
+    DTK_HIP(launch_seed(b, q, s), "launch seed");
+    DTK_HIP(launch_task(S, c8, two, b, kp, q, workers, s), "launch task");
+    return DTK_OK;
+  }
+
+  dtk_status read_ctrl(Workspace& w, const Queue& q, hipStream_t s, uint32_t** host)
+  {
+    uint32_t* h = static_cast<uint32_t*>(w.hbuf);
+    DTK_HIP(hipMemcpyAsync(h, q.ctrl, CTRL_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read counters");
+    DTK_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
+    *host = h;
+    if (h[CTRL_ERROR] & ERR_MASK_RANGE)
+      return fail(DTK_ERR_INVALID_ARGUMENT, "soft mask values must be in [0, 1]");
+    if (h[CTRL_ERROR] & ERR_WATCHDOG)
+      return fail(DTK_ERR_HIP, "device watchdog expired in the work queue");
+    return DTK_OK;
+  }
+} // namespace
+
+extern "C" {
+
+void dtk_params_default(dtk_params* p)
+{
+  if (!p)
+    return;
+  std::memset(p, 0, sizeof(*p));
+  p->lambda       = 0.f;
+  p->v            = 1e10f; // dt/include/dt/generalised_distance_transform.hpp:8
+  p->connectivity = DTK_C8;
+  p->schedule     = DTK_SCHED_TASK;
+}
+
+const char* dtk_last_error(void) { return g_err.c_str(); }
+
+const char* dtk_version(void) { return "dtk 0.1 (gfx950)"; }
+
+int dtk_tile_count(int width, int height, int tile_rows)
+{
+  const int rh = tile_rows == 0 ? 16 : tile_rows;
+  if (width <= 0 || height <= 0 || (rh != 32 && rh != 16))
+    return 0;
+  return ((width + kTileW - 1) / kTileW) * ((height + rh - 3) / (rh - 2));
+}
+
+int dtk_device_cu_count(void)
+{
+  Workspace* w = nullptr;
+  if (current_ws(&w, nullptr) != DTK_OK)
+    return 0;
+  return w->cu;
+}
+
+dtk_status dtk_ggdt(const dtk_image* img, const dtk_image* mask, dtk_image* dist, const dtk_params* p, void* stream,
+                    dtk_stats* stats)
+{
+  dtk_status st;
+  if ((st = check_params(p)) != DTK_OK || (st = check_triplet(img, mask, dist)) != DTK_OK)
+    return st;
+  const int   rh = region_rows(p);
+  hipStream_t s  = static_cast<hipStream_t>(stream);
+  if (img->width == 0 || img->height == 0)
+    return fill_stats(stats, nullptr, 0, 0, rh, 0);
+  Workspace* w = nullptr;
+  if ((st = current_ws(&w, nullptr)) != DTK_OK)
+    return st;
+  std::lock_guard<std::mutex> lock(w->mu);
+  const bool c8  = p->connectivity == DTK_C8;
+  const bool two = two_adds(p);
+  const int  S   = 64 / rh;
+  KParams    kp  = make_kparams(p, img->dtype, mask->dtype, rh);
+  ImgDesc    d   = make_desc(img, mask, dist, rh);
+
+  if (p->schedule == DTK_SCHED_RASTER)
+  {
+    const size_t words = raster_progress_words(d);
+    if ((long long)words > (long long)w->cu * 8)
+      return fail(DTK_ERR_UNSUPPORTED, "image too large for the resident raster pipeline");
+    if ((st = ws_reserve(*w, 4096 + words * 4, 4096)) != DTK_OK)
+      return st;
+    uint32_t* changed  = static_cast<uint32_t*>(w->dbuf);
+    uint32_t* progress = changed + 64;
+    uint32_t* h        = static_cast<uint32_t*>(w->hbuf);
+    DTK_HIP(launch_raster_init(d, kp, s), "launch raster init");
+    int iters = 0;
+    for (;;)
+    {
+      DTK_HIP(hipMemsetAsync(changed, 0, 2 * sizeof(uint32_t), s), "memset");
+      DTK_HIP(launch_raster_iteration(d, kp, c8, changed, progress, s), "launch raster pass");
+      DTK_HIP(hipMemcpyAsync(h, changed, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read flag");
+      DTK_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
+      ++iters;
+      if (h[1] & ERR_WATCHDOG)
+        return fail(DTK_ERR_HIP, "device watchdog expired in the raster pipeline");
+      if (!h[0] || (p->max_iterations > 0 && iters >= p->max_iterations))
+        break;
+    }
+    return fill_stats(stats, nullptr, iters, 0, 0, 0);
+  }
+
+  const int ntiles = d.tiles_x * d.tiles_y;
+  Batch     b{};
+  b.d0     = d;
+  b.nimg   = 1;
+  b.ntiles = ntiles;
+
+  if (p->schedule == DTK_SCHED_BLOCKS)
+  {
+    TaskBufs tb;
+    size_t   need = task_layout(nullptr, ntiles, 1, 1, nullptr) + 2 * align_up(ntiles, 256) + 512;
+    if ((st = ws_reserve(*w, need, 4096)) != DTK_OK)
+      return st;
+    task_layout(w->dbuf, ntiles, 1, 1, &tb);
+    Queue q     = tb.q;
+    q.slots     = nullptr; // blocks schedule: no queue
+    uint8_t* a0 = static_cast<uint8_t*>(w->dbuf) + align_up(task_layout(nullptr, ntiles, 1, 1, nullptr), 256);
+    uint8_t* a1 = a0 + align_up(ntiles, 256);
+    KParams  kb = kp;
+    kb.max_sweeps = 0;
+    DTK_HIP(hipMemsetAsync(a0, 0, 2 * align_up(ntiles, 256), s), "memset active");
+    DTK_HIP(launch_reset_ctrl(q.ctrl, s), "launch reset");
+    DTK_HIP(launch_init(b, kb, q, s), "launch init");
+    DTK_HIP(launch_blocks_seed(b, q, a0, s), "launch seed");
+    uint32_t* h      = static_cast<uint32_t*>(w->hbuf);
+    int       rounds = 0;
+    for (;;)
+    {
+      DTK_HIP(hipMemsetAsync(q.ctrl + CTRL_CHANGED, 0, sizeof(uint32_t), s), "memset");
+      DTK_HIP(launch_blocks(S, c8, two, b, kb, a0, a1, q.ctrl, p->round_per_blocks, s), "launch blocks");
+      DTK_HIP(hipMemcpyAsync(h, q.ctrl, CTRL_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read flag");
+      DTK_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
+      ++rounds;
+      if (h[CTRL_ERROR] & ERR_MASK_RANGE)
+        return fail(DTK_ERR_INVALID_ARGUMENT, "soft mask values must be in [0, 1]");
+      if (!h[CTRL_CHANGED])
+        break;
+      std::swap(a0, a1);
+    }
+    return fill_stats(stats, h, rounds, ntiles, rh, 0);
+  }
+
+  // Task schedule.
+  const int      workers = task_workers(*w, p, S, c8, two, ntiles);
+  const uint32_t cap     = next_pow2((uint32_t)(ntiles + workers + 1));
+  TaskBufs       tb;
+  if ((st = ws_reserve(*w, task_layout(nullptr, ntiles, cap, 1, nullptr), 4096)) != DTK_OK)
+    return st;
+  task_layout(w->dbuf, ntiles, cap, 1, &tb);
+  if ((st = enqueue_task(b, kp, tb.q, S, c8, two, workers, s)) != DTK_OK)
+    return st;
+  if (stats)
+  {
+    uint32_t* h = nullptr;
+    if ((st = read_ctrl(*w, tb.q, s, &h)) != DTK_OK)
+      return st;
+    return fill_stats(stats, h, 1, ntiles, rh, workers);
+  }
+  return DTK_OK;
+}
+
+dtk_status dtk_ggdt_batched(int n, const dtk_image* imgs, const dtk_image* masks, dtk_image* dists,
+                            const dtk_params* p, void* stream, dtk_stats* stats)
+{
+  dtk_status st;
+  if ((st = check_params(p)) != DTK_OK)
+    return st;
+  if (n < 0 || (n > 0 && (!imgs || !masks || !dists)))
+    return fail(DTK_ERR_INVALID_ARGUMENT, "bad batch arrays");
+  if (p->schedule != DTK_SCHED_TASK)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "batched calls use the task schedule");
+  if (n == 0)
+    return fill_stats(stats, nullptr, 0, 0, region_rows(p), 0);
+  const int             rh = region_rows(p);
+  std::vector<ImgDesc>  descs;
+  std::vector<uint32_t> tile_img;
+  int                   ntiles = 0;
+  if ((st = build_batch(n, imgs, masks, dists, p, rh, descs, tile_img, &ntiles)) != DTK_OK)
+    return st;
+  if (ntiles == 0)
+    return fill_stats(stats, nullptr, 0, 0, rh, 0);
+  Workspace* w = nullptr;
+  if ((st = current_ws(&w, nullptr)) != DTK_OK)
+    return st;
+  std::lock_guard<std::mutex> lock(w->mu);
+  hipStream_t    s   = static_cast<hipStream_t>(stream);
+  const bool     c8  = p->connectivity == DTK_C8;
+  const bool     two = two_adds(p);
+  const int      S   = 64 / rh;
+  const int      workers = task_workers(*w, p, S, c8, two, ntiles);
+  const uint32_t cap     = next_pow2((uint32_t)(ntiles + workers + 1));
+  const size_t   hbytes  = descs.size() * sizeof(ImgDesc) + tile_img.size() * 4 + 1024;
+  if ((st = ws_reserve(*w, task_layout(nullptr, ntiles, cap, n > 1 ? n : 2, nullptr), hbytes)) != DTK_OK)
+    return st;
+  TaskBufs tb;
+  task_layout(w->dbuf, ntiles, cap, n > 1 ? n : 2, &tb);
+  Batch b{};
+  b.d0     = descs[0];
+  b.nimg   = n;
+  b.ntiles = ntiles;
+  if (n > 1)
+  {
+    char* h = static_cast<char*>(w->hbuf);
+    std::memcpy(h, descs.data(), descs.size() * sizeof(ImgDesc));
+    std::memcpy(h + descs.size() * sizeof(ImgDesc), tile_img.data(), tile_img.size() * 4);
+    DTK_HIP(hipMemcpyAsync(tb.table, h, descs.size() * sizeof(ImgDesc), hipMemcpyHostToDevice, s), "upload table");
+    DTK_HIP(hipMemcpyAsync(tb.tile_img, h + descs.size() * sizeof(ImgDesc), tile_img.size() * 4,
+                           hipMemcpyHostToDevice, s),
+            "upload tile table");
+    // The staging buffer is reused by the next call: wait for the copies.
+    DTK_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
+    b.table    = tb.table;
+    b.tile_img = tb.tile_img;
+  }
+  KParams kp = make_kparams(p, imgs[0].dtype, masks[0].dtype, rh);
+  if ((st = enqueue_task(b, kp, tb.q, S, c8, two, workers, s)) != DTK_OK)
+    return st;
+  if (stats)
+  {
+    uint32_t* h = nullptr;
+    if ((st = read_ctrl(*w, tb.q, s, &h)) != DTK_OK)
+      return st;
+    return fill_stats(stats, h, 1, ntiles, rh, workers);
+  }
+  return DTK_OK;
+}
+
+struct dtk_batch_plan
+{
+  int             device  = 0;
+  void*           dbuf    = nullptr;
+  hipStream_t     cstream = nullptr;
+  hipGraph_t      graph   = nullptr;
+  hipGraphExec_t  exec    = nullptr;
+  int             ntiles  = 0;
+};
+
+dtk_status dtk_batch_plan_create(int n, const dtk_image* imgs, const dtk_image* masks, dtk_image* dists,
+                                 const dtk_params* p, void* stream, dtk_batch_plan** plan)
+{
+  dtk_status st;
+  if (!plan)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "plan is NULL");
+  *plan = nullptr;
+  if ((st = check_params(p)) != DTK_OK)
+    return st;
+  if (n <= 0 || !imgs || !masks || !dists)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "bad batch arrays");
+  if (p->schedule != DTK_SCHED_TASK)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "batched plans use the task schedule");
+  const int             rh = region_rows(p);
+  std::vector<ImgDesc>  descs;
+  std::vector<uint32_t> tile_img;
+  int                   ntiles = 0;
+  if ((st = build_batch(n, imgs, masks, dists, p, rh, descs, tile_img, &ntiles)) != DTK_OK)
+    return st;
+  if (ntiles == 0)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "batch has no pixels");
+  Workspace* w   = nullptr;
+  int        dev = 0;
+  if ((st = current_ws(&w, &dev)) != DTK_OK)
+    return st;
+  const bool     c8      = p->connectivity == DTK_C8;
+  const bool     two     = two_adds(p);
+  const int      S       = 64 / rh;
+  const int      workers = task_workers(*w, p, S, c8, two, ntiles);
+  const uint32_t cap     = next_pow2((uint32_t)(ntiles + workers + 1));
+  dtk_batch_plan* pl     = new dtk_batch_plan();
+  pl->device             = dev;
+  pl->ntiles             = ntiles;
+  const size_t bytes     = task_layout(nullptr, ntiles, cap, n > 1 ? n : 2, nullptr);
+  hipError_t   e         = hipMalloc(&pl->dbuf, bytes);
+  if (e != hipSuccess)
+  {
+    delete pl;
+    return fail(DTK_ERR_OUT_OF_MEMORY, std::string("hipMalloc(plan): ") + hipGetErrorString(e));
+  }
+  TaskBufs tb;
+  task_layout(pl->dbuf, ntiles, cap, n > 1 ? n : 2, &tb);
+  Batch b{};
+  b.d0     = descs[0];
+  b.nimg   = n;
+  b.ntiles = ntiles;
+  if (n > 1)
+  {
+    e = hipMemcpy(tb.table, descs.data(), descs.size() * sizeof(ImgDesc), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(tb.tile_img, tile_img.data(), tile_img.size() * 4, hipMemcpyHostToDevice);
+    b.table    = tb.table;
+    b.tile_img = tb.tile_img;
+  }
+  if (e == hipSuccess)
+    e = hipStreamCreateWithFlags(&pl->cstream, hipStreamNonBlocking);
+  if (e != hipSuccess)
+  {
+    dtk_batch_plan_destroy(pl);
+    return fail(DTK_ERR_HIP, std::string("plan setup: ") + hipGetErrorString(e));
+  }
+  KParams kp = make_kparams(p, imgs[0].dtype, masks[0].dtype, rh);
+  e          = hipStreamBeginCapture(pl->cstream, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess)
+  {
+    st            = enqueue_task(b, kp, tb.q, S, c8, two, workers, pl->cstream);
+    hipError_t e2 = hipStreamEndCapture(pl->cstream, &pl->graph);
+    if (st != DTK_OK)
+    {
+      dtk_batch_plan_destroy(pl);
+      return st;
+    }
+    e = e2;
+  }
+  if (e == hipSuccess)
+    e = hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess)
+  {
+    dtk_batch_plan_destroy(pl);
+    return fail(DTK_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  }
+  (void)stream;
+  *plan = pl;
+  return DTK_OK;
+}
+
+dtk_status dtk_batch_plan_launch(dtk_batch_plan* plan, void* stream)
+{
+  if (!plan || !plan->exec)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "invalid plan");
+  DTK_HIP(hipGraphLaunch(plan->exec, static_cast<hipStream_t>(stream)), "hipGraphLaunch");
+  return DTK_OK;
+}
+
+dtk_status dtk_batch_plan_destroy(dtk_batch_plan* plan)
+{
+  if (!plan)
+    return DTK_OK;
+  hipDeviceSynchronize();
+  if (plan->exec)
+    hipGraphExecDestroy(plan->exec);
+  if (plan->graph)
+    hipGraphDestroy(plan->graph);
+  if (plan->cstream)
+    hipStreamDestroy(plan->cstream);
+  if (plan->dbuf)
+    hipFree(plan->dbuf);
+  delete plan;
+  return DTK_OK;
+}
+
+dtk_status dtk_ggdt_pred(const dtk_image* img, const dtk_image* mask, const dtk_image* dist, dtk_image* pred,
+                         const dtk_params* p, void* stream)
+{
+  dtk_status st;
+  if ((st = check_params(p)) != DTK_OK)
+    return st;
+  dtk_image dcopy = dist ? *dist : dtk_image{};
+  if ((st = check_triplet(img, mask, dist ? &dcopy : nullptr)) != DTK_OK)
+    return st;
+  if ((st = check_image(pred, "pred", 1 << DTK_U8)) != DTK_OK)
+    return st;
+  if (pred->width != img->width || pred->height != img->height)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "pred size differs");
+  ImgDesc d    = make_desc(img, mask, &dcopy, 32);
+  d.pred       = static_cast<unsigned char*>(pred->data);
+  d.pred_pitch = pred->pitch;
+  KParams kp   = make_kparams(p, img->dtype, mask->dtype, 32);
+  DTK_HIP(launch_pred(d, kp, p->connectivity == DTK_C8, static_cast<hipStream_t>(stream)), "launch pred");
+  return DTK_OK;
+}
+
+dtk_status dtk_ggdt_backward(const dtk_image* img, const dtk_image* mask, const dtk_image* dist,
+                             const dtk_image* pred, const dtk_image* grad_dist, dtk_image* grad_img,
+                             dtk_image* grad_mask, const dtk_params* p, void* stream)
+{
+  dtk_status st;
+  if ((st = check_params(p)) != DTK_OK)
+    return st;
+  dtk_image dcopy = dist ? *dist : dtk_image{};
+  if ((st = check_triplet(img, mask, dist ? &dcopy : nullptr)) != DTK_OK)
+    return st;
+  if (img->dtype != DTK_F32)
+    return fail(DTK_ERR_UNSUPPORTED, "the adjoint needs a float32 image");
+  if ((st = check_image(pred, "pred", 1 << DTK_U8)) != DTK_OK ||
+      (st = check_image(grad_dist, "grad_dist", 1 << DTK_F32)) != DTK_OK ||
+      (st = check_image(grad_img, "grad_img", 1 << DTK_F32)) != DTK_OK)
+    return st;
+  if (grad_mask && (st = check_image(grad_mask, "grad_mask", 1 << DTK_F32)) != DTK_OK)
+    return st;
+  const int W = img->width, H = img->height;
+  if (pred->width != W || pred->height != H || grad_dist->width != W || grad_dist->height != H ||
+      grad_img->width != W || grad_img->height != H ||
+      (grad_mask && (grad_mask->width != W || grad_mask->height != H)))
+    return fail(DTK_ERR_INVALID_ARGUMENT, "gradient image sizes differ");
+  if (W == 0 || H == 0)
+    return DTK_OK;
+  Workspace* w = nullptr;
+  if ((st = current_ws(&w, nullptr)) != DTK_OK)
+    return st;
+  std::lock_guard<std::mutex> lock(w->mu);
+  const size_t npx = (size_t)W * H;
+  if ((st = ws_reserve(*w, 3 * align_up(npx * 4, 256) + 1024, 4096)) != DTK_OK)
+    return st;
+  Carver    c(w->dbuf);
+  float*    G      = c.take<float>(npx);
+  uint32_t* counts = c.take<uint32_t>(npx);
+  float*    acc    = c.take<float>(npx);
+  ImgDesc   d      = make_desc(img, mask, &dcopy, 32);
+  d.pred           = static_cast<unsigned char*>(pred->data);
+  d.pred_pitch     = pred->pitch;
+  KParams kp       = make_kparams(p, img->dtype, mask->dtype, 32);
+  const bool soft  = mask->dtype == DTK_F32;
+  DTK_HIP(launch_backward(d, kp, p->connectivity == DTK_C8, static_cast<const unsigned char*>(grad_dist->data),
+                          grad_dist->pitch, static_cast<unsigned char*>(grad_img->data), grad_img->pitch,
+                          soft && grad_mask ? static_cast<unsigned char*>(grad_mask->data) : nullptr,
+                          grad_mask ? grad_mask->pitch : 0, G, counts, acc, static_cast<hipStream_t>(stream)),
+          "launch backward");
+  if (grad_mask && !soft)
+    DTK_HIP(hipMemset2DAsync(grad_mask->data, grad_mask->pitch, 0, (size_t)W * 4, H, static_cast<hipStream_t>(stream)),
+            "zero grad_mask");
+  return DTK_OK;
+}
+
+dtk_status dtk_malloc_pitch(void** ptr, long long* pitch, long long width_bytes, int height)
+{
+  if (!ptr || !pitch || width_bytes < 0 || height < 0)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "bad dtk_malloc_pitch arguments");
+  size_t     pp  = 0;
+  void*      out = nullptr;
+  hipError_t e   = hipMallocPitch(&out, &pp, (size_t)(width_bytes > 0 ? width_bytes : 1), (size_t)(height > 0 ? height : 1));
+  if (e != hipSuccess)
+    return fail(DTK_ERR_OUT_OF_MEMORY, std::string("Unable to allocate GPU memory: ") + hipGetErrorString(e));
+  *ptr   = out;
+  *pitch = (long long)pp;
+  return DTK_OK;
+}
+
+dtk_status dtk_free(void* ptr)
+{
+  DTK_HIP(hipFree(ptr), "hipFree");
+  return DTK_OK;
+}
+
+static dtk_status copy2d(const dtk_image* src, dtk_image* dst, hipMemcpyKind kind, const char* what)
+{
+  if (!src || !dst)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "NULL image");
+  if (src->width != dst->width || src->height != dst->height || src->dtype != dst->dtype)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "transfer between different image shapes");
+  const int es = elem_size(src->dtype);
+  if (es == 0)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "unsupported dtype");
+  if (src->width == 0 || src->height == 0)
+    return DTK_OK;
+  hipError_t e = hipMemcpy2D(dst->data, (size_t)dst->pitch, src->data, (size_t)src->pitch,
+                             (size_t)src->width * es, (size_t)src->height, kind);
+  if (e != hipSuccess)
+    return fail(DTK_ERR_HIP, std::string(what) + hipGetErrorString(e));
+  return DTK_OK;
+}
+
+dtk_status dtk_host_to_device(const dtk_image* src, dtk_image* dst)
+{
+  return copy2d(src, dst, hipMemcpyHostToDevice, "Unable to transfert host to device: ");
+}
+
+dtk_status dtk_device_to_host(const dtk_image* src, dtk_image* dst)
+{
+  return copy2d(src, dst, hipMemcpyDeviceToHost, "Unable to transfert device to host: ");
+}
+
+dtk_status dtk_copy2d(void* dst, long long dst_pitch, const void* src, long long src_pitch, long long width_bytes,
+                      int height, int kind)
+{
+  if (width_bytes < 0 || height < 0 || kind < 1 || kind > 3)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "bad dtk_copy2d arguments");
+  if (width_bytes == 0 || height == 0)
+    return DTK_OK;
+  if (!dst || !src || dst_pitch < width_bytes || src_pitch < width_bytes)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "bad dtk_copy2d buffers");
+  const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : (kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+  const hipError_t    e = hipMemcpy2D(dst, (size_t)dst_pitch, src, (size_t)src_pitch, (size_t)width_bytes, (size_t)height, k);
+  if (e != hipSuccess)
+    return fail(DTK_ERR_HIP, std::string(kind == 1 ? "Unable to transfert host to device: "
+                                                    : (kind == 2 ? "Unable to transfert device to host: "
+                                                                 : "Unable to copy on device: ")) +
+                                 hipGetErrorString(e));
+  return DTK_OK;
+}
+
+dtk_status dtk_selftest(int* out3)
+{
+  if (!out3)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "out is NULL");
+  const int  n   = 3 * 255 * 255 + 1;
+  int*       dd  = nullptr;
+  float*     ds  = nullptr;
+  DTK_HIP(hipMalloc(&dd, sizeof(int) * 4), "hipMalloc");
+  hipError_t e = hipMalloc(&ds, sizeof(float) * n);
+  if (e != hipSuccess)
+  {
+    hipFree(dd);
+    return fail(DTK_ERR_OUT_OF_MEMORY, "hipMalloc");
+  }
+  std::vector<float> hs(n);
+  int                hd = 0;
+  e                     = launch_selftest(dd, ds, nullptr);
+  if (e == hipSuccess)
+    e = hipMemcpy(&hd, dd, sizeof(int), hipMemcpyDeviceToHost);
+  if (e == hipSuccess)
+    e = hipMemcpy(hs.data(), ds, sizeof(float) * n, hipMemcpyDeviceToHost);
+  hipFree(dd);
+  hipFree(ds);
+  if (e != hipSuccess)
+    return fail(DTK_ERR_HIP, std::string("selftest: ") + hipGetErrorString(e));
+  int bad = 0;
+  for (int i = 0; i < n; ++i)
+  {
+    const float ref = std::sqrt((float)i);
+    uint32_t    a, b;
+    std::memcpy(&a, &hs[i], 4);
+    std::memcpy(&b, &ref, 4);
+    bad += a != b;
+  }
+  out3[0] = hd;
+  out3[1] = bad;
+  out3[2] = dtk_device_cu_count();
+  return DTK_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,121 @@
+// Structures shared by the HIP kernels and the host side of libdtk.
+//
+// Tiling.  A tile owns a 62 x (RH-2) block of pixels and is solved by one
+// wave64 over a 64 x RH "region": the owned block plus a one-pixel halo on
+// every side.  Lane i of the wave holds region column i; with S row
+// segments RH = 64 / S (S = 2: 64 x 32 regions, 62 x 30 owned pixels).
+#pragma once
+
+#include <stdint.h>
+
+namespace dtk
+{
+  enum ImgKind : int
+  {
+    IMG_U8   = 0,
+    IMG_RGB8 = 1,
+    IMG_F32  = 2,
+  };
+
+  enum MaskKind : int
+  {
+    MASK_U8  = 0,
+    MASK_F32 = 1,
+  };
+
+  constexpr int kTileW = 62; // owned columns per tile (64 lanes minus the halo)
+
+  // One image of a call (or of a batch).  Pitches are in bytes.
+  struct ImgDesc
+  {
+    const unsigned char* img;
+    long long            img_pitch;
+    const unsigned char* mask;
+    long long            mask_pitch;
+    float*               D;
+    long long            D_pitch;
+    unsigned char*       pred;
+    long long            pred_pitch;
+    int                  W, H;
+    int                  tiles_x, tiles_y;
+    int                  tile0; // first global tile id of this image
+    int                  pad_;
+  };
+
+  // A call: one image (d0) or a batch described by a device table.
+  struct Batch
+  {
+    ImgDesc         d0;
+    const ImgDesc*  table;    // nimg entries (device), used when nimg > 1
+    const uint32_t* tile_img; // global tile id -> image index (device), nimg > 1
+    int             nimg;
+    int             ntiles;   // total tiles of the call
+  };
+
+  // Arithmetic of the relaxation.  cur = (D(q) + e) + g with e = l_eucl*ld
+  // and g = l_grad*|I(p)-I(q)| (dt/src/generalised_distance_transform.cu:45).
+  // When lambda is 0 or 1 one of the two addends is an exact zero and the
+  // kernels use a single add (fl(x + 0) == x for x >= +0): the cost arrays
+  // then hold e (lambda = 0) or g (lambda = 1).
+  struct KParams
+  {
+    int   img_kind;
+    int   mask_kind;
+    float l_grad;
+    float e_axis; // l_eucl * 1
+    float e_diag; // l_eucl * sqrtf(2)
+    float v;      // non-seed value (uint8 mask) or nu (float mask)
+    int   zero_lambda;
+    int   max_sweeps; // sweep cap per tile visit (0 = until stable)
+    int   rh;         // region rows (64 / S)
+  };
+
+  // Device work queue of dirty tiles (persistent kernel).
+  //   slots[CAP]: (seq << 32) | tile; seq = 2*ticket while free for that
+  //               ticket, 2*ticket + 1 once filled.
+  //   state[t]  : TILE_QUEUED | TILE_RUNNING | TILE_DIRTY bits of tile t.  A
+  //               tile is in the ring at most once and solved by at most one
+  //               wave at a time; a wake-up while it runs sets TILE_DIRTY and
+  //               the running wave re-queues it when it finishes.
+  //   ctrl      : counters, each on its own 64-byte line.
+  enum CtrlWord : int
+  {
+    CTRL_HEAD    = 0,
+    CTRL_TAIL    = 16,
+    CTRL_PENDING = 32,
+    CTRL_ERROR   = 48,
+    CTRL_VISITS  = 64,
+    CTRL_SWEEPS  = 80,
+    CTRL_CHANGED = 96,
+    CTRL_WORDS   = 128,
+  };
+
+  enum TileState : uint32_t
+  {
+    TILE_QUEUED  = 1u,
+    TILE_RUNNING = 2u,
+    TILE_DIRTY   = 4u,
+  };
+
+  struct Queue
+  {
+    unsigned long long* slots;
+    uint32_t*           state;
+    uint32_t*           ctrl;
+    float*              tmin; // per tile min / max of the initial map (seed selection)
+    float*              tmax;
+    uint32_t            cap_mask;
+    int                 pad_;
+  };
+
+  enum ErrorBits : uint32_t
+  {
+    ERR_MASK_RANGE = 1u, // soft mask value outside [0, 1] or NaN
+    ERR_WATCHDOG   = 2u, // a device-side wait exceeded kSpinLimit polls
+  };
+
+  // Upper bound on the polls of any device-side wait (queue slots, raster
+  // progress flags).  Legitimate waits end long before; an expired wait
+  // flags ERR_WATCHDOG and the kernel finishes instead of hanging.
+  constexpr uint32_t kSpinLimit = 1u << 26;
+} // namespace dtk

@@ -1,0 +1,36 @@
+// Host-side launchers of the GGDT kernels (defined in the .hip files).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "dtk_internal.hpp"
+
+namespace dtk
+{
+  // dtk_task.hip
+  hipError_t launch_reset_ctrl(uint32_t* ctrl, hipStream_t s);
+  hipError_t launch_init(const Batch& b, const KParams& p, const Queue& q, hipStream_t s);
+  hipError_t launch_seed(const Batch& b, const Queue& q, hipStream_t s);
+  hipError_t launch_blocks_seed(const Batch& b, const Queue& q, uint8_t* active, hipStream_t s);
+  hipError_t launch_task(int S, bool c8, bool two, const Batch& b, const KParams& p, const Queue& q, int workers,
+                         hipStream_t s);
+  hipError_t launch_blocks(int S, bool c8, bool two, const Batch& b, const KParams& p, uint8_t* active, uint8_t* next,
+                           uint32_t* ctrl, int* rpt, hipStream_t s);
+  int        task_occupancy(int S, bool c8, bool two);
+
+  // dtk_raster.hip: one full raster iteration (four passes) with the exact
+  // reference schedule; `changed` (device u32) is set when a pixel decreased.
+  hipError_t launch_raster_init(const ImgDesc& d, const KParams& p, hipStream_t s);
+  hipError_t launch_raster_iteration(const ImgDesc& d, const KParams& p, bool c8, uint32_t* changed,
+                                     uint32_t* progress, hipStream_t s);
+  size_t     raster_progress_words(const ImgDesc& d);
+
+  // dtk_adjoint.hip
+  hipError_t launch_pred(const ImgDesc& d, const KParams& p, bool c8, hipStream_t s);
+  hipError_t launch_backward(const ImgDesc& d, const KParams& p, bool c8, const unsigned char* grad_dist,
+                             long long gd_pitch, unsigned char* grad_img, long long gi_pitch, unsigned char* grad_mask,
+                             long long gm_pitch, float* G, uint32_t* counts, float* acc, hipStream_t s);
+
+  // dtk_selftest.hip
+  hipError_t launch_selftest(int* dpp_out, float* sqrt_out, hipStream_t s);
+} // namespace dtk

@@ -1,0 +1,279 @@
+// Raster GGDT passes with the exact reference schedule (gfx950).
+//
+// Reference: dt/src/generalised_distance_transform.cu:17-99 (passes) and
+// :101-134 (driver).  A row pass visits x = 1..W-1 (forward) or W-2..0
+// (backward); every pixel of column x reads column x-1 as it is after step
+// x-1 of the same pass.  The column passes do the same along y.  The CPU
+// oracle runs exactly this schedule, so after N iterations GPU and CPU
+// agree bit-for-bit (and at convergence for any N).
+//
+// MI355X mapping.  A pass is split into strips of R = 32 lines (rows for a
+// row pass, columns for a column pass).  One wave holds 64 lanes = its 32
+// lines plus G = 16 ghost lines on each side, and advances in chunks of
+// C = G steps: at step j of a chunk, lane l is exact when lanes l-1..l+1
+// were exact at step j-1, so after C steps the 32 owned lanes are exact even
+// though the outermost ghost lanes had no neighbours.  Before a chunk, a
+// strip waits until both neighbour strips have published the previous chunk
+// (a release/acquire progress flag), so the ghost lines it reloads are
+// final up to the chunk's first step.  Strips form a wavefront pipeline;
+// all strips of a pass are resident at once (checked by the host).
+//
+// Edge costs are formed on the fly from the image values of the current and
+// previous step (DPP for the neighbouring lines), with the same arithmetic
+// as the tile solver: cur = (D(q) + e) + g, one add when lambda is 0 or 1.
+#include <hip/hip_runtime.h>
+
+#include "dtk_launch.hpp"
+#include "dtk_tile.hpp"
+
+namespace dtk
+{
+  namespace
+  {
+    constexpr int kStripR = 32; // owned lines per strip
+    constexpr int kGhost  = 16; // ghost lines per side == steps per chunk
+  }
+
+  __global__ void k_raster_init(ImgDesc d, KParams p)
+  {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= d.W || y >= d.H)
+      return;
+    float d0;
+    if (p.mask_kind == MASK_U8)
+      d0 = p.v * (float)(d.mask[(long long)y * d.mask_pitch + x] == 0);
+    else
+      d0 = p.v * (1.f - reinterpret_cast<const float*>(d.mask + (long long)y * d.mask_pitch)[x]);
+    reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(d.D) + (long long)y * d.D_pitch)[x] = d0;
+  }
+
+  template <int KIND>
+  __device__ __forceinline__ uint32_t img_at(const ImgDesc& d, int x, int y)
+  {
+    return load_pix<KIND>(d.img + (long long)y * d.img_pitch, x);
+  }
+
+  __device__ __forceinline__ float& d_at(const ImgDesc& d, int x, int y)
+  {
+    return reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(d.D) + (long long)y * d.D_pitch)[x];
+  }
+
+  __device__ __forceinline__ void wait_progress(uint32_t* progress, int strip, uint32_t c, uint32_t* changed)
+  {
+    for (uint32_t spin = 0; __hip_atomic_load(&progress[strip], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < c; ++spin)
+    {
+      if (spin >= kSpinLimit)
+      {
+        // changed[1] is the error word of a raster call.
+        __hip_atomic_fetch_or(&changed[1], (uint32_t)ERR_WATCHDOG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+
+  // ROWS: row pass (steps along x, lanes are rows); else column pass.
+  template <int KIND, bool ROWS, bool FWD, bool C8, bool TWO>
+  __global__ void __launch_bounds__(64) k_raster_pass(ImgDesc d, KParams p, uint32_t* changed, uint32_t* progress)
+  {
+    const int lane   = threadIdx.x;
+    const int strip  = blockIdx.x;
+    const int nlines = ROWS ? d.H : d.W; // lines (lanes) dimension
+    const int nsteps = ROWS ? d.W : d.H; // step dimension
+    const int line   = strip * kStripR - kGhost + lane;
+    const bool lin   = line >= 0 && line < nlines;
+    const bool own   = lane >= kGhost && lane < kGhost + kStripR && lin;
+    const bool linL  = line - 1 >= 0 && line - 1 < nlines;
+    const bool linR  = line + 1 >= 0 && line + 1 < nlines;
+    const int  nstrips = gridDim.x;
+
+    const int len   = nsteps - 1;         // steps of the pass
+    const int start = FWD ? 1 : nsteps - 2;
+    const int dir   = FWD ? 1 : -1;
+    if (len <= 0)
+      return;
+    const int nchunks = (len + kGhost - 1) / kGhost;
+
+    const float ea = p.e_axis, ed = p.e_diag;
+    bool        any = false;
+
+    // Values of the previous step (the step before the chunk's first).
+    float    Dprev = DTK_INF;
+    uint32_t Iprev = 0;
+
+    for (int c = 0; c < nchunks; ++c)
+    {
+      // Wait for the neighbour strips to finish chunk c-1.
+      if (c > 0)
+      {
+        if (lane == 0)
+        {
+          if (strip > 0)
+            wait_progress(progress, strip - 1, (uint32_t)c, changed);
+          if (strip + 1 < nstrips)
+            wait_progress(progress, strip + 1, (uint32_t)c, changed);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __syncthreads();
+      }
+      const int k0 = c * kGhost;
+      const int kn = min(kGhost, len - k0);
+
+      // Previous step: owned lanes keep their registers, the others reload.
+      {
+        const int s = start + dir * (k0 - 1);
+        if (c == 0 || !own)
+        {
+          Dprev = DTK_INF;
+          Iprev = 0;
+          if (lin)
+          {
+            const int x = ROWS ? s : line, y = ROWS ? line : s;
+            Dprev      = d_at(d, x, y);
+            Iprev      = img_at<KIND>(d, x, y);
+          }
+        }
+      }
+      float    Dv[kGhost];
+      uint32_t Iv[kGhost];
+#pragma unroll
+      for (int j = 0; j < kGhost; ++j)
+      {
+        Dv[j] = DTK_INF;
+        Iv[j] = 0;
+        if (j < kn && lin)
+        {
+          const int s = start + dir * (k0 + j);
+          const int x = ROWS ? s : line, y = ROWS ? line : s;
+          Dv[j]      = d_at(d, x, y);
+          Iv[j]      = img_at<KIND>(d, x, y);
+        }
+      }
+
+#pragma unroll
+      for (int j = 0; j < kGhost; ++j)
+      {
+        if (j < kn)
+        {
+          const uint32_t ip = Iv[j];
+          float          nd = Dv[j];
+          {
+            float g = p.l_grad * pix_dist<KIND>(ip, Iprev);
+            g       = (g == g) ? g : DTK_INF;
+            const float cost = p.zero_lambda ? ea : g;
+            const float cur  = TWO ? (Dprev + ea) + cost : Dprev + cost;
+            nd               = fminf(nd, cur);
+          }
+          if constexpr (C8)
+          {
+            const float    dl = shr_z(Dprev), dr = shl_z(Dprev);
+            const uint32_t il = shr_u(Iprev), ir = shl_u(Iprev);
+            float          gl = p.l_grad * pix_dist<KIND>(ip, il);
+            float          gr = p.l_grad * pix_dist<KIND>(ip, ir);
+            gl                = (gl == gl) ? gl : DTK_INF;
+            gr                = (gr == gr) ? gr : DTK_INF;
+            const float cl    = p.zero_lambda ? ed : gl;
+            const float cr    = p.zero_lambda ? ed : gr;
+            float       curl  = TWO ? (dl + ed) + cl : dl + cl;
+            float       curr  = TWO ? (dr + ed) + cr : dr + cr;
+            curl              = (linL && lane > 0) ? curl : DTK_INF;
+            curr              = (linR && lane < 63) ? curr : DTK_INF;
+            nd                = min3f(nd, curl, curr);
+          }
+          if (lin && nd < Dv[j])
+          {
+            Dv[j] = nd;
+            any |= own;
+          }
+          Dprev = Dv[j];
+          Iprev = ip;
+        }
+      }
+
+      // Store the owned lines of the chunk and publish it.
+      if (own)
+      {
+#pragma unroll
+        for (int j = 0; j < kGhost; ++j)
+        {
+          if (j < kn)
+          {
+            const int s = start + dir * (k0 + j);
+            const int x = ROWS ? s : line, y = ROWS ? line : s;
+            d_at(d, x, y) = Dv[j];
+          }
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __syncthreads();
+      if (lane == 0)
+        __hip_atomic_store(&progress[strip], (uint32_t)(c + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (__ballot(any) != 0 && lane == 0)
+      __hip_atomic_store(changed, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+
+  template <int KIND, bool C8, bool TWO>
+  static hipError_t raster_iter_t(const ImgDesc& d, const KParams& p, uint32_t* changed, uint32_t* progress,
+                                  hipStream_t s)
+  {
+    const int rs = (d.H + kStripR - 1) / kStripR; // row-pass strips
+    const int cs = (d.W + kStripR - 1) / kStripR; // column-pass strips
+    hipError_t e;
+    if ((e = hipMemsetAsync(progress, 0, sizeof(uint32_t) * rs, s)) != hipSuccess)
+      return e;
+    hipLaunchKernelGGL((k_raster_pass<KIND, true, true, C8, TWO>), dim3(rs), dim3(64), 0, s, d, p, changed, progress);
+    if ((e = hipMemsetAsync(progress, 0, sizeof(uint32_t) * rs, s)) != hipSuccess)
+      return e;
+    hipLaunchKernelGGL((k_raster_pass<KIND, true, false, C8, TWO>), dim3(rs), dim3(64), 0, s, d, p, changed, progress);
+    if ((e = hipMemsetAsync(progress, 0, sizeof(uint32_t) * cs, s)) != hipSuccess)
+      return e;
+    hipLaunchKernelGGL((k_raster_pass<KIND, false, true, C8, TWO>), dim3(cs), dim3(64), 0, s, d, p, changed, progress);
+    if ((e = hipMemsetAsync(progress, 0, sizeof(uint32_t) * cs, s)) != hipSuccess)
+      return e;
+    hipLaunchKernelGGL((k_raster_pass<KIND, false, false, C8, TWO>), dim3(cs), dim3(64), 0, s, d, p, changed, progress);
+    return hipGetLastError();
+  }
+
+  template <int KIND>
+  static hipError_t raster_iter_k(const ImgDesc& d, const KParams& p, bool c8, uint32_t* changed, uint32_t* progress,
+                                  hipStream_t s)
+  {
+    const bool two = !(p.zero_lambda || p.l_grad == 1.f);
+    if (c8)
+      return two ? raster_iter_t<KIND, true, true>(d, p, changed, progress, s)
+                 : raster_iter_t<KIND, true, false>(d, p, changed, progress, s);
+    return two ? raster_iter_t<KIND, false, true>(d, p, changed, progress, s)
+               : raster_iter_t<KIND, false, false>(d, p, changed, progress, s);
+  }
+
+  hipError_t launch_raster_init(const ImgDesc& d, const KParams& p, hipStream_t s)
+  {
+    if (d.W <= 0 || d.H <= 0)
+      return hipSuccess;
+    hipLaunchKernelGGL(k_raster_init, dim3((d.W + 255) / 256, d.H), dim3(256), 0, s, d, p);
+    return hipGetLastError();
+  }
+
+  hipError_t launch_raster_iteration(const ImgDesc& d, const KParams& p, bool c8, uint32_t* changed,
+                                     uint32_t* progress, hipStream_t s)
+  {
+    switch (p.img_kind)
+    {
+    case IMG_U8:
+      return raster_iter_k<IMG_U8>(d, p, c8, changed, progress, s);
+    case IMG_RGB8:
+      return raster_iter_k<IMG_RGB8>(d, p, c8, changed, progress, s);
+    default:
+      return raster_iter_k<IMG_F32>(d, p, c8, changed, progress, s);
+    }
+  }
+
+  size_t raster_progress_words(const ImgDesc& d)
+  {
+    const int rs = (d.H + kStripR - 1) / kStripR;
+    const int cs = (d.W + kStripR - 1) / kStripR;
+    return (size_t)(rs > cs ? rs : cs);
+  }
+} // namespace dtk

@@ -1,0 +1,572 @@
+// Wave64 tile solver for the GGDT (device code, gfx950).
+//
+// Reference counterpart: the block solve of dt/src/generalised_distance_transform_blocks.cu:8-112
+// and dt/src/generalised_distance_transform_task.cu:13-127, with the per-tile
+// passes of dt/src/dt_block_passes.cuh:11-127.  The relaxation arithmetic is
+// the raster's (dt/src/generalised_distance_transform.cu:34-56).
+//
+// Region and layouts.  A tile owns 62 x (RH-2) pixels; one wave loads the
+// 64 x RH region around it (owned pixels plus a one-pixel halo).
+//
+//   column layout : lane i = region column i, register r = region row r.
+//                   Column sweeps step through the registers; the three
+//                   upper (or lower) neighbours come from lanes i-1, i, i+1
+//                   through DPP wave shifts.
+//   row layout    : the 64 columns are cut into S segments of SW = 64/S
+//                   columns; lane (h, r) = h*RH + r holds row r of segment h
+//                   in registers k = 1..SW (column SW*h + k - 1).  Registers
+//                   0 and SW+1 hold the neighbouring segments' edge columns
+//                   as read at the start of the row phase (or +inf at the
+//                   region border); they are read-only during the phase.
+//   Layout changes go through one LDS buffer of RH x 65 floats.
+//
+// Edge costs are precomputed once per visit, per undirected edge, as floats
+// (+inf for an edge that leaves the image or the region, which also makes
+// the DPP values that wrap between segments or off the wave harmless):
+//   column layout: N, NW, NE of every pixel (edges to the row above);
+//   row layout   : W, NW, SW of every pixel (edges to the column on the left).
+// A relaxation is pulled when the receiver owns the edge cost and pushed
+// (the sender adds its own cost, then the candidate is shifted) otherwise.
+//
+// Halo pixels are relaxed like owned pixels: every value the solver ever
+// holds is the float sum of a real path, so a smaller halo value is still a
+// valid upper bound.  Only owned pixels are written back.
+//
+// Convergence inside a visit: stop after four consecutive sweeps that changed
+// nothing, where a backward row sweep counts as quiet only if the forward row
+// sweep of the same phase was quiet too (otherwise its segment-edge copies
+// may be stale).  Such a window certifies every edge of the region.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "dtk_internal.hpp"
+
+namespace dtk
+{
+#define DTK_INF __builtin_huge_valf()
+
+  // ---- DPP wave shifts (gfx9 wave_shr:1 / wave_shl:1) --------------------
+  // shr: lane i receives lane i-1; shl: lane i receives lane i+1.
+  __device__ __forceinline__ float shr_z(float v)
+  {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, true));
+  }
+  __device__ __forceinline__ float shl_z(float v)
+  {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, true));
+  }
+  // Lanes without a source keep `old`.
+  __device__ __forceinline__ float shr_keep(float old, float v)
+  {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x138, 0xF, 0xF, false));
+  }
+  __device__ __forceinline__ float shl_keep(float old, float v)
+  {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x130, 0xF, 0xF, false));
+  }
+  // Integer wave shifts.  The result is copied through an opaque VALU move so
+  // the DPP is never folded into an integer VOP2 instruction: measured on
+  // gfx950, a folded v_subrev_u32_dpp with wave_shl:1 produced wrong values,
+  // while a plain v_mov_b32_dpp followed by the subtract is correct
+  // (dtk_selftest checks both the shifts and this subtract pattern).
+  __device__ __forceinline__ uint32_t opaque_u(uint32_t v)
+  {
+    uint32_t r;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+  }
+  __device__ __forceinline__ uint32_t shr_u(uint32_t v)
+  {
+    return opaque_u((uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, true));
+  }
+  __device__ __forceinline__ uint32_t shl_u(uint32_t v)
+  {
+    return opaque_u((uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xF, 0xF, true));
+  }
+
+  __device__ __forceinline__ float min3f(float a, float b, float c) { return fminf(fminf(a, b), c); }
+
+  __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+  // ---- pixels and edge costs -----------------------------------------------
+  template <int KIND>
+  __device__ __forceinline__ uint32_t load_pix(const unsigned char* row, int x)
+  {
+    if constexpr (KIND == IMG_U8)
+      return row[x];
+    else if constexpr (KIND == IMG_RGB8)
+    {
+      const unsigned char* q = row + 3 * x;
+      return (uint32_t)q[0] | ((uint32_t)q[1] << 8) | ((uint32_t)q[2] << 16);
+    }
+    else
+      return reinterpret_cast<const uint32_t*>(row)[x];
+  }
+
+  // |I(p) - I(q)|: uint8 abs difference (dt/src/utils.cuh:37-41); RGB8:
+  // correctly rounded sqrtf of the exact integer squared distance; float32:
+  // fabsf of the rounded difference.
+  template <int KIND>
+  __device__ __forceinline__ float pix_dist(uint32_t a, uint32_t b)
+  {
+    if constexpr (KIND == IMG_U8)
+    {
+      const int d = (int)a - (int)b;
+      return (float)(d < 0 ? -d : d);
+    }
+    else if constexpr (KIND == IMG_RGB8)
+    {
+      const int dr = (int)(a & 255u) - (int)(b & 255u);
+      const int dg = (int)((a >> 8) & 255u) - (int)((b >> 8) & 255u);
+      const int db = (int)((a >> 16) & 255u) - (int)((b >> 16) & 255u);
+      return __builtin_sqrtf((float)(dr * dr + dg * dg + db * db));
+    }
+    else
+      return fabsf(__uint_as_float(a) - __uint_as_float(b));
+  }
+
+  // Cost stored for an edge: e (lambda == 0, single add), else g = l_grad*dist.
+  // A NaN g is stored as +inf: a NaN candidate never wins a strict compare,
+  // and neither does +inf, so both leave D unchanged.
+  template <int KIND>
+  __device__ __forceinline__ float edge_cost(bool valid, uint32_t a, uint32_t b, float e_dir, const KParams& p)
+  {
+    float g = p.l_grad * pix_dist<KIND>(a, b);
+    g       = (g == g) ? g : DTK_INF;
+    g       = p.zero_lambda ? e_dir : g;
+    return valid ? g : DTK_INF;
+  }
+
+  struct VisitResult
+  {
+    uint32_t border;   // bit k: neighbour k must be revisited (see kNbrDX/kNbrDY)
+    int      sweeps;   // sweeps run
+    bool     converged;
+    bool     changed;  // any owned pixel changed
+  };
+
+  // Neighbour tiles in border-bit order: L, R, T, B, TL, TR, BL, BR.
+  __device__ constexpr int kNbrDX[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
+  __device__ constexpr int kNbrDY[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+
+  template <int RH>
+  __device__ __forceinline__ const float* drow(const ImgDesc& d, int y)
+  {
+    return reinterpret_cast<const float*>(reinterpret_cast<const unsigned char*>(d.D) + (long long)y * d.D_pitch);
+  }
+
+  // Column layout -> LDS -> row layout of one array.  k = 0..SW+1.
+  template <int S>
+  __device__ __forceinline__ void col_to_row(const float (&c)[64 / S], float (&rw)[64 / S + 2], float* T, int lane)
+  {
+    constexpr int RH = 64 / S, SW = 64 / S;
+#pragma unroll
+    for (int r = 0; r < RH; ++r)
+      T[r * 65 + lane] = c[r];
+    __syncthreads();
+    const int h = lane / RH, rr = lane % RH;
+#pragma unroll
+    for (int k = 0; k < SW + 2; ++k)
+    {
+      const int  col = SW * h + k - 1;
+      const bool ok  = (k >= 1 && k <= SW) || (k == 0 ? h > 0 : h < S - 1);
+      const float v  = T[rr * 65 + (ok ? col : 0)];
+      rw[k]          = ok ? v : DTK_INF;
+    }
+    __syncthreads();
+  }
+
+  template <int S>
+  __device__ __forceinline__ void row_to_col(const float (&rw)[64 / S + 2], float (&c)[64 / S], float* T, int lane)
+  {
+    constexpr int RH = 64 / S, SW = 64 / S;
+    const int     h = lane / RH, rr = lane % RH;
+#pragma unroll
+    for (int k = 1; k <= SW; ++k)
+      T[rr * 65 + SW * h + k - 1] = rw[k];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RH; ++r)
+      c[r] = T[r * 65 + lane];
+    __syncthreads();
+  }
+
+  // ---- sweeps --------------------------------------------------------------
+  // Column forward: row r pulls from row r-1 (dt/src/generalised_distance_transform.cu:59-99, Forward).
+  template <int RH, bool C8, bool TWO>
+  __device__ __forceinline__ uint64_t sweep_col_fwd(float (&D)[RH], const float (&cN)[RH], const float (&cNW)[RH],
+                                                    const float (&cNE)[RH], float ea, float ed)
+  {
+    uint64_t ch = 0;
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int r = 1; r < RH; ++r)
+    {
+      const float up = D[r - 1];
+      const float c  = (TWO ? up + ea : up) + cN[r];
+      float       m  = c;
+      if constexpr (C8)
+      {
+        const float t = TWO ? up + ed : up;
+        const float a = shr_z(t) + cNW[r];
+        const float b = shl_z(t) + cNE[r];
+        m             = min3f(a, b, c);
+      }
+      ch |= ballot(m < D[r]);
+      D[r] = fminf(m, D[r]);
+    }
+    return ch;
+  }
+
+  // Column backward: row r receives from row r+1.  The diagonal candidates
+  // are formed by the sender (its NW / NE cost) and shifted.
+  template <int RH, bool C8, bool TWO>
+  __device__ __forceinline__ uint64_t sweep_col_bwd(float (&D)[RH], const float (&cN)[RH], const float (&cNW)[RH],
+                                                    const float (&cNE)[RH], float ea, float ed)
+  {
+    uint64_t ch = 0;
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int r = RH - 2; r >= 0; --r)
+    {
+      const float dn = D[r + 1];
+      const float c  = (TWO ? dn + ea : dn) + cN[r + 1];
+      float       m  = c;
+      if constexpr (C8)
+      {
+        const float t   = TWO ? dn + ed : dn;
+        const float pNW = t + cNW[r + 1]; // (i, r+1) -> (i-1, r)
+        const float pNE = t + cNE[r + 1]; // (i, r+1) -> (i+1, r)
+        const float a   = shl_keep(D[r], pNW);
+        const float b   = shr_keep(D[r], pNE);
+        m               = min3f(a, b, c);
+      }
+      ch |= ballot(m < D[r]);
+      D[r] = fminf(m, D[r]);
+    }
+    return ch;
+  }
+
+  // Row forward: slot k pulls from slot k-1 (dt/src/generalised_distance_transform.cu:17-57, Forward).
+  template <int SW, bool C8, bool TWO>
+  __device__ __forceinline__ uint64_t sweep_row_fwd(float (&R)[SW + 2], const float (&rW)[SW + 2],
+                                                    const float (&rNW)[SW + 2], const float (&rSW)[SW + 2], float ea,
+                                                    float ed)
+  {
+    uint64_t ch = 0;
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int k = 1; k <= SW; ++k)
+    {
+      const float lf = R[k - 1];
+      const float c  = (TWO ? lf + ea : lf) + rW[k];
+      float       m  = c;
+      if constexpr (C8)
+      {
+        const float t = TWO ? lf + ed : lf;
+        const float a = shr_z(t) + rNW[k];
+        const float b = shl_z(t) + rSW[k];
+        m             = min3f(a, b, c);
+      }
+      ch |= ballot(m < R[k]);
+      R[k] = fminf(m, R[k]);
+    }
+    return ch;
+  }
+
+  // Row backward: slot k receives from slot k+1.
+  template <int SW, bool C8, bool TWO>
+  __device__ __forceinline__ uint64_t sweep_row_bwd(float (&R)[SW + 2], const float (&rW)[SW + 2],
+                                                    const float (&rNW)[SW + 2], const float (&rSW)[SW + 2], float ea,
+                                                    float ed)
+  {
+    uint64_t ch = 0;
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int k = SW; k >= 1; --k)
+    {
+      const float rt = R[k + 1];
+      const float c  = (TWO ? rt + ea : rt) + rW[k + 1];
+      float       m  = c;
+      if constexpr (C8)
+      {
+        const float t   = TWO ? rt + ed : rt;
+        const float pSW = t + rSW[k + 1]; // (c+1, r) -> (c, r+1)
+        const float pNW = t + rNW[k + 1]; // (c+1, r) -> (c, r-1)
+        const float a   = shr_keep(R[k], pSW);
+        const float b   = shl_keep(R[k], pNW);
+        m               = min3f(a, b, c);
+      }
+      ch |= ballot(m < R[k]);
+      R[k] = fminf(m, R[k]);
+    }
+    return ch;
+  }
+
+  // ---- cost precomputation --------------------------------------------------
+  // Image values of the region in the column layout (0 outside the image).
+  template <int KIND, int RH>
+  __device__ __forceinline__ void load_region_image(const ImgDesc& d, int X0, int Y0, int lane, uint32_t (&I)[RH])
+  {
+    const int  gx  = X0 + lane;
+    const bool cin = gx >= 0 && gx < d.W;
+#pragma unroll
+    for (int r = 0; r < RH; ++r)
+    {
+      const int gy = Y0 + r;
+      I[r]         = 0;
+      if (cin && gy >= 0 && gy < d.H)
+        I[r] = load_pix<KIND>(d.img + (long long)gy * d.img_pitch, gx);
+    }
+  }
+
+  // Which neighbour an edge array refers to (relative to pixel (i, r)).
+  enum EdgeDir : int
+  {
+    E_W  = 0, // (i-1, r)
+    E_N  = 1, // (i, r-1)
+    E_NW = 2, // (i-1, r-1)
+    E_NE = 3, // (i+1, r-1)
+    E_SW = 4, // (i-1, r+1)
+  };
+
+  // Cost of the edge from every pixel of the region to its DIR neighbour
+  // (column layout), +inf when either end is outside the image or region.
+  template <int KIND, int RH, int DIR>
+  __device__ __forceinline__ void edge_costs(const ImgDesc& d, const KParams& p, int X0, int Y0, int lane,
+                                             const uint32_t (&I)[RH], float (&out)[RH])
+  {
+    const int  gx  = X0 + lane;
+    const bool cin = gx >= 0 && gx < d.W;
+    const bool inL = lane >= 1 && gx - 1 >= 0 && gx - 1 < d.W;
+    const bool inR = lane <= 62 && gx + 1 >= 0 && gx + 1 < d.W;
+    const float e  = (DIR == E_W || DIR == E_N) ? p.e_axis : p.e_diag;
+#pragma unroll
+    for (int r = 0; r < RH; ++r)
+    {
+      const int  gy   = Y0 + r;
+      const bool here = cin && gy >= 0 && gy < d.H;
+      if constexpr (DIR == E_W)
+        out[r] = edge_cost<KIND>(here && inL, I[r], shr_u(I[r]), e, p);
+      else if constexpr (DIR == E_SW)
+      {
+        if (r <= RH - 2)
+        {
+          const bool rinD = gy + 1 >= 0 && gy + 1 < d.H;
+          out[r]          = edge_cost<KIND>(here && inL && rinD, I[r], shr_u(I[r + 1]), e, p);
+        }
+        else
+          out[r] = DTK_INF;
+      }
+      else
+      {
+        if (r >= 1)
+        {
+          const bool     rinU = gy - 1 >= 0 && gy - 1 < d.H;
+          const uint32_t u    = I[r - 1];
+          if constexpr (DIR == E_N)
+            out[r] = edge_cost<KIND>(here && rinU, I[r], u, e, p);
+          else if constexpr (DIR == E_NW)
+            out[r] = edge_cost<KIND>(here && rinU && inL, I[r], shr_u(u), e, p);
+          else
+            out[r] = edge_cost<KIND>(here && rinU && inR, I[r], shl_u(u), e, p);
+        }
+        else
+          out[r] = DTK_INF;
+      }
+    }
+  }
+
+  // All cost arrays of a visit: column layout N / NW / NE, row layout
+  // W / NW / SW.  Computed one array at a time to bound register pressure.
+  template <int KIND, int S, bool C8>
+  __device__ __forceinline__ void setup_costs(const ImgDesc& d, const KParams& p, int X0, int Y0, int lane, float* T,
+                                              float (&cN)[64 / S], float (&cNW)[64 / S], float (&cNE)[64 / S],
+                                              float (&rW)[64 / S + 2], float (&rNW)[64 / S + 2],
+                                              float (&rSW)[64 / S + 2])
+  {
+    constexpr int RH = 64 / S, SW = 64 / S;
+    uint32_t      I[RH];
+    load_region_image<KIND, RH>(d, X0, Y0, lane, I);
+    {
+      float tmp[RH];
+      edge_costs<KIND, RH, E_W>(d, p, X0, Y0, lane, I, tmp);
+      col_to_row<S>(tmp, rW, T, lane);
+    }
+    if constexpr (C8)
+    {
+      {
+        float tmp[RH];
+        edge_costs<KIND, RH, E_SW>(d, p, X0, Y0, lane, I, tmp);
+        col_to_row<S>(tmp, rSW, T, lane);
+      }
+      edge_costs<KIND, RH, E_NW>(d, p, X0, Y0, lane, I, cNW);
+      col_to_row<S>(cNW, rNW, T, lane);
+      edge_costs<KIND, RH, E_NE>(d, p, X0, Y0, lane, I, cNE);
+    }
+    else
+    {
+#pragma unroll
+      for (int r = 0; r < RH; ++r)
+        cNW[r] = cNE[r] = DTK_INF;
+#pragma unroll
+      for (int k = 0; k < SW + 2; ++k)
+        rNW[k] = rSW[k] = DTK_INF;
+    }
+    edge_costs<KIND, RH, E_N>(d, p, X0, Y0, lane, I, cN);
+  }
+
+  // ---- one visit of one tile --------------------------------------------------
+  // T: LDS RH*65 floats; Sv: LDS 2*RH floats.  Loads the region, iterates to
+  // its local fixed point (or max_sweeps), writes back the owned pixels and
+  // reports which neighbours saw a border pixel change.
+  template <int S, bool C8, bool TWO>
+  __device__ VisitResult visit_tile(const ImgDesc& d, const KParams& p, int tx, int ty, float* T, float* Sv)
+  {
+    constexpr int RH   = 64 / S;
+    constexpr int SW   = 64 / S;
+    const int     lane = threadIdx.x;
+    const int     X0   = tx * kTileW - 1;
+    const int     Y0   = ty * (RH - 2) - 1;
+    const int     gx   = X0 + lane;
+    const bool    cin  = gx >= 0 && gx < d.W;
+
+    float cN[RH], cNW[RH], cNE[RH];
+    float rW[SW + 2], rNW[SW + 2], rSW[SW + 2];
+    switch (p.img_kind)
+    {
+    case IMG_U8:
+      setup_costs<IMG_U8, S, C8>(d, p, X0, Y0, lane, T, cN, cNW, cNE, rW, rNW, rSW);
+      break;
+    case IMG_RGB8:
+      setup_costs<IMG_RGB8, S, C8>(d, p, X0, Y0, lane, T, cN, cNW, cNE, rW, rNW, rSW);
+      break;
+    default:
+      setup_costs<IMG_F32, S, C8>(d, p, X0, Y0, lane, T, cN, cNW, cNE, rW, rNW, rSW);
+      break;
+    }
+
+    float Dc[RH];
+#pragma unroll
+    for (int r = 0; r < RH; ++r)
+    {
+      const int gy = Y0 + r;
+      Dc[r]        = DTK_INF;
+      if (cin && gy >= 0 && gy < d.H)
+        Dc[r] = drow<RH>(d, gy)[gx];
+    }
+
+    // Initial values of the owned border (to decide which neighbours to wake).
+    const float init_top = Dc[1];
+    const float init_bot = Dc[RH - 2];
+    if (lane == 1 || lane == 62)
+    {
+      const int o = (lane == 62) ? RH : 0;
+#pragma unroll
+      for (int r = 0; r < RH; ++r)
+        Sv[o + r] = Dc[r];
+    }
+
+    const float ea = p.e_axis, ed = p.e_diag;
+    // A visit needs four quiet sweeps to certify its region, so a cap below
+    // four could never converge (the tile would re-queue itself forever).
+    const int   cap = p.max_sweeps > 0 ? (p.max_sweeps < 4 ? 4 : p.max_sweeps) : 0x7fffffff;
+    int         quiet = 0, sweeps = 0;
+    bool        conv = false, any = false;
+    float       R[SW + 2];
+    for (;;)
+    {
+      uint64_t ch = sweep_col_fwd<RH, C8, TWO>(Dc, cN, cNW, cNE, ea, ed);
+      ++sweeps;
+      any |= ch != 0;
+      quiet = ch ? 0 : quiet + 1;
+      if (quiet >= 4)
+      {
+        conv = true;
+        break;
+      }
+      if (sweeps >= cap)
+        break;
+      ch = sweep_col_bwd<RH, C8, TWO>(Dc, cN, cNW, cNE, ea, ed);
+      ++sweeps;
+      any |= ch != 0;
+      quiet = ch ? 0 : quiet + 1;
+      if (quiet >= 4)
+      {
+        conv = true;
+        break;
+      }
+      if (sweeps >= cap)
+        break;
+
+      col_to_row<S>(Dc, R, T, lane);
+      const uint64_t cf = sweep_row_fwd<SW, C8, TWO>(R, rW, rNW, rSW, ea, ed);
+      ++sweeps;
+      any |= cf != 0;
+      quiet = cf ? 0 : quiet + 1;
+      if (quiet >= 4 || sweeps >= cap)
+      {
+        conv = quiet >= 4;
+        row_to_col<S>(R, Dc, T, lane);
+        break;
+      }
+      const uint64_t cb = sweep_row_bwd<SW, C8, TWO>(R, rW, rNW, rSW, ea, ed);
+      ++sweeps;
+      any |= cb != 0;
+      quiet = (cb | cf) ? 0 : quiet + 1;
+      row_to_col<S>(R, Dc, T, lane);
+      if (quiet >= 4)
+      {
+        conv = true;
+        break;
+      }
+      if (sweeps >= cap)
+        break;
+    }
+
+    VisitResult res;
+    res.sweeps    = sweeps;
+    res.converged = conv;
+    res.changed   = any;
+    res.border    = 0;
+    if (any)
+    {
+      bool colc = false;
+      if (lane == 1 || lane == 62)
+      {
+        const int o = (lane == 62) ? RH : 0;
+#pragma unroll
+        for (int r = 1; r < RH - 1; ++r)
+          colc |= Dc[r] != Sv[o + r];
+      }
+      const uint64_t own  = 0x7FFFFFFFFFFFFFFEull; // lanes 1..62
+      const uint64_t top  = ballot(Dc[1] != init_top) & own;
+      const uint64_t bot  = ballot(Dc[RH - 2] != init_bot) & own;
+      const uint64_t cols = ballot(colc);
+      uint32_t       b    = 0;
+      b |= (cols >> 1) & 1u;                     // L
+      b |= ((cols >> 62) & 1u) << 1;             // R
+      b |= (top != 0 ? 1u : 0u) << 2;            // T
+      b |= (bot != 0 ? 1u : 0u) << 3;            // B
+      b |= (uint32_t)((top >> 1) & 1u) << 4;     // TL
+      b |= (uint32_t)((top >> 62) & 1u) << 5;    // TR
+      b |= (uint32_t)((bot >> 1) & 1u) << 6;     // BL
+      b |= (uint32_t)((bot >> 62) & 1u) << 7;    // BR
+      res.border = b;
+
+      // Write back the owned pixels.
+      if (lane >= 1 && lane <= 62 && cin)
+      {
+#pragma unroll
+        for (int r = 1; r < RH - 1; ++r)
+        {
+          const int gy = Y0 + r;
+          if (gy < d.H)
+            const_cast<float*>(drow<RH>(d, gy))[gx] = Dc[r];
+        }
+      }
+    }
+    return res;
+  }
+} // namespace dtk

@@ -1,0 +1,180 @@
+/*
+ * dtk.h -- C ABI of the MI355X-native generalised geodesic distance
+ * transform (GGDT) library (libdtk.so, HIP kernels for gfx950).
+ *
+ * This is the drop-in boundary for the reference's GGDT path
+ * (baptisteesteban/distance_transform_benchmarks).  The reference exposes a
+ * C++ API only; the C++ wrappers in include/dt/ (namespace dt) keep its
+ * signatures and call these entry points.  Each entry point below names the
+ * reference interface it replaces.
+ *
+ * Conventions
+ *   - Images are pitched 2-D buffers in device memory (pitch in BYTES), the
+ *     layout of dt::image2d_view (dt/include/dt/image2d_view.hpp:194-199):
+ *     address(x, y) = data + y * pitch + x * elem_size.
+ *   - Pixel types: DTK_U8 (grey), DTK_RGB8 (3 bytes r,g,b, the layout of
+ *     dt::rgb8, dt/include/dt/rgb.hpp:7-12), DTK_F32.
+ *   - Masks: DTK_U8 (mask != 0 marks a seed; D0 = v * (mask == 0),
+ *     dt/src/utils.cu:12) or DTK_F32 soft masks s in [0, 1] (D0 = v * (1 - s),
+ *     an extension of this library).
+ *   - Errors are returned as dtk_status codes, never thrown across the ABI;
+ *     dtk_last_error() returns a message for the calling thread.
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *     Calls are stream-ordered; the raster and blocks schedules read a
+ *     convergence flag back to the host once per round, the task schedule
+ *     does not synchronise.
+ */
+#ifndef DTK_H
+#define DTK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum
+{
+  DTK_OK                   = 0,
+  DTK_ERR_INVALID_ARGUMENT = 1,
+  DTK_ERR_OUT_OF_MEMORY    = 2,
+  DTK_ERR_HIP              = 3,
+  DTK_ERR_UNSUPPORTED      = 4,
+  DTK_ERR_NOT_CONVERGED    = 5
+} dtk_status;
+
+typedef enum
+{
+  DTK_U8   = 0,
+  DTK_RGB8 = 1,
+  DTK_F32  = 2
+} dtk_dtype;
+
+typedef enum
+{
+  DTK_C4 = 4,
+  DTK_C8 = 8
+} dtk_connectivity;
+
+typedef enum
+{
+  DTK_SCHED_RASTER = 0, /* full-image row/column passes (reference "raster")   */
+  DTK_SCHED_BLOCKS = 1, /* host-driven rounds over dirty tiles ("blocks")       */
+  DTK_SCHED_TASK   = 2  /* persistent kernel over a device work queue ("task")  */
+} dtk_schedule;
+
+/* A pitched 2-D image in device (or, for the transfer calls, host) memory. */
+typedef struct
+{
+  void*     data;
+  int       width;
+  int       height;
+  long long pitch; /* bytes between rows */
+  int       dtype; /* dtk_dtype */
+} dtk_image;
+
+/* Parameters of one call.  Initialise with dtk_params_default(). */
+typedef struct
+{
+  float lambda;         /* 0 <= lambda <= 1; l_grad = lambda, l_eucl = 1 - lambda */
+  float v;              /* non-seed value (uint8 mask) / nu (soft mask); finite, >= 0; default 1e10 */
+  int   connectivity;   /* DTK_C8 (reference) or DTK_C4 */
+  int   schedule;       /* dtk_schedule */
+  int   max_iterations; /* 0: run to convergence; N > 0: exactly N raster iterations (raster only) */
+  int   tile_rows;      /* tile solver region rows: 32 or 16 (0 = autotuned/default) */
+  int   max_sweeps;     /* sweep cap per tile visit (0 = until the tile is stable) */
+  int   waves_per_cu;   /* persistent workers per CU (0 = occupancy limit) */
+  int*  round_per_blocks; /* optional device int[dtk_tile_count()]: visits per tile (blocks schedule) */
+} dtk_params;
+
+/* Filled by calls that report statistics (may be NULL). */
+typedef struct
+{
+  int      rounds;     /* host rounds (raster iterations / block rounds); 1 for task */
+  uint32_t visits;     /* tile visits (blocks / task) */
+  uint32_t sweeps;     /* tile sweeps (blocks / task) */
+  int      tiles;      /* tiles of the call */
+  int      tile_rows;  /* region rows used */
+  int      workers;    /* persistent workers launched (task) */
+} dtk_stats;
+
+void dtk_params_default(dtk_params* p);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* dtk_last_error(void);
+
+/* Library version string. */
+const char* dtk_version(void);
+
+/* GGDT of one image, in place into `dist` (DTK_F32, same size).
+ * Replaces dt::generalised_distance_transform (schedule RASTER,
+ * dt/include/dt/generalised_distance_transform.hpp:7-8),
+ * dt::generalised_distance_transform_blocks (BLOCKS, :13-15) and
+ * dt::generalised_distance_transform_task (TASK, :21-23).  The converged
+ * result is identical for every schedule. */
+dtk_status dtk_ggdt(const dtk_image* img, const dtk_image* mask, dtk_image* dist, const dtk_params* p, void* stream,
+                    dtk_stats* stats);
+
+/* GGDT of n independent images of any sizes in ONE persistent launch that
+ * shares a single tile queue (schedule TASK).  No reference counterpart
+ * (the reference computes one image per call); each output equals the
+ * single-image call. */
+dtk_status dtk_ggdt_batched(int n, const dtk_image* imgs, const dtk_image* masks, dtk_image* dists,
+                            const dtk_params* p, void* stream, dtk_stats* stats);
+
+/* Batched plan captured as a HIP graph: buffers are bound at creation, each
+ * dtk_batch_plan_launch replays init + seed + persistent solve.  No
+ * reference counterpart. */
+typedef struct dtk_batch_plan dtk_batch_plan;
+dtk_status dtk_batch_plan_create(int n, const dtk_image* imgs, const dtk_image* masks, dtk_image* dists,
+                                 const dtk_params* p, void* stream, dtk_batch_plan** plan);
+dtk_status dtk_batch_plan_launch(dtk_batch_plan* plan, void* stream);
+dtk_status dtk_batch_plan_destroy(dtk_batch_plan* plan);
+
+/* Argmin predecessor codes of a converged map (uint8 image, same size as
+ * dist): 0..7 = neighbour offset index in the order of dt::c8
+ * (dt/include/dt/c8.hpp:19-28): (-1,0) (-1,-1) (0,-1) (1,-1) (1,0) (1,1)
+ * (0,1) (-1,1); 8 = root (D == D0).  No reference counterpart. */
+dtk_status dtk_ggdt_pred(const dtk_image* img, const dtk_image* mask, const dtk_image* dist, dtk_image* pred,
+                         const dtk_params* p, void* stream);
+
+/* Adjoint of L = sum(W * D) through the predecessor forest, for DTK_F32
+ * images and soft DTK_F32 masks: writes dL/dimage and dL/dmask (DTK_F32).
+ * `grad_dist` holds dL/dD (= W).  No reference counterpart. */
+dtk_status dtk_ggdt_backward(const dtk_image* img, const dtk_image* mask, const dtk_image* dist,
+                             const dtk_image* pred, const dtk_image* grad_dist, dtk_image* grad_img,
+                             dtk_image* grad_mask, const dtk_params* p, void* stream);
+
+/* Tiles a call on a width x height image uses (size of round_per_blocks). */
+int dtk_tile_count(int width, int height, int tile_rows);
+
+/* Pitched device allocation (replaces cudaMallocPitch in
+ * dt/include/dt/details/image2d_data_gpu.hpp:28). */
+dtk_status dtk_malloc_pitch(void** ptr, long long* pitch, long long width_bytes, int height);
+dtk_status dtk_free(void* ptr);
+
+/* Pitched copies (replace dt::host_to_device / dt::device_to_host,
+ * dt/src/transfert.cpp:10-28).  Synchronous. */
+dtk_status dtk_host_to_device(const dtk_image* src, dtk_image* dst);
+dtk_status dtk_device_to_host(const dtk_image* src, dtk_image* dst);
+
+/* Raw pitched 2-D copy: kind 1 = host to device, 2 = device to host,
+ * 3 = device to device.  Synchronous. */
+dtk_status dtk_copy2d(void* dst, long long dst_pitch, const void* src, long long src_pitch, long long width_bytes,
+                      int height, int kind);
+
+/* Device self-check of the assumptions the kernels rely on: DPP lane
+ * direction of wave shifts and correctly rounded sqrtf over every RGB8
+ * squared distance.  out[0] = DPP ok, out[1] = number of sqrtf mismatches
+ * (host reference), out[2] = device compute units. */
+dtk_status dtk_selftest(int* out3);
+
+/* Device properties used for launch shapes. */
+int dtk_device_cu_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DTK_H */

@@ -1,0 +1,169 @@
+"""ctypes wrapper of the CPU oracle (oracle/ggdt_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by the cpu_baseline leg of bench.py, always as the checker or the CPU
+baseline, never as the product path.  The product (the HIP library behind
+include/dtk.h) does not import or link anything here.
+
+Every function takes and returns contiguous numpy arrays of shape (H, W)
+(or (H, W, 3) for RGB8 images).  The restated reference functions are cited
+in ggdt_oracle.c.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libggdt_oracle.so")
+_lib = None
+
+IMG_U8, IMG_RGB8, IMG_F32 = 0, 1, 2
+MASK_U8, MASK_F32 = 0, 1
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        lib = ctypes.CDLL(_LIB_PATH)
+        vp, fp, i, f, dp = ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_double
+        lib.ora_init.argtypes = [vp, i, fp, i, i, f]
+        lib.ora_raster.argtypes = [vp, i, vp, i, fp, i, i, f, f, i, i]
+        lib.ora_raster.restype = i
+        lib.ora_raster_omp.argtypes = [vp, i, vp, i, fp, i, i, f, f, i, i, i]
+        lib.ora_raster_omp.restype = i
+        lib.ora_max_threads.restype = i
+        lib.ora_dijkstra.argtypes = [vp, i, vp, i, fp, i, i, f, f, i]
+        lib.ora_dijkstra.restype = i
+        lib.ora_pred.argtypes = [vp, i, vp, i, fp, vp, i, i, f, f, i]
+        lib.ora_backward.argtypes = [fp, fp, vp, fp, vp, vp, vp, i, i, f, f]
+        lib.ora_raster_f64.argtypes = [vp, vp, vp, i, i, dp, dp, i]
+        lib.ora_raster_f64.restype = i
+        lib.ora_fd_grad.argtypes = [fp, fp, fp, i, i, dp, dp, i, dp, vp, i, vp, vp]
+        _lib = lib
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _img_kind(img: np.ndarray) -> int:
+    if img.dtype == np.uint8 and img.ndim == 2:
+        return IMG_U8
+    if img.dtype == np.uint8 and img.ndim == 3 and img.shape[2] == 3:
+        return IMG_RGB8
+    if img.dtype == np.float32 and img.ndim == 2:
+        return IMG_F32
+    raise TypeError(f"unsupported image {img.dtype} {img.shape}")
+
+
+def _mask_kind(mask: np.ndarray) -> int:
+    if mask.dtype == np.uint8:
+        return MASK_U8
+    if mask.dtype == np.float32:
+        return MASK_F32
+    raise TypeError(f"unsupported mask {mask.dtype}")
+
+
+def _prep(img, mask):
+    img = np.ascontiguousarray(img)
+    mask = np.ascontiguousarray(mask)
+    if mask.shape != img.shape[:2]:
+        raise ValueError("image and mask shapes differ")
+    return img, mask
+
+
+def init(mask: np.ndarray, v: float) -> np.ndarray:
+    mask = np.ascontiguousarray(mask)
+    H, W = mask.shape
+    D = np.empty((H, W), np.float32)
+    _load().ora_init(_ptr(mask), _mask_kind(mask), _ptr(D), W, H, v)
+    return D
+
+
+def raster(img, mask, lam: float, v: float = 1e10, conn8: bool = True, max_iters: int = 0, threads: int = 1):
+    """Reference raster GGDT; converged when max_iters <= 0.  Returns (D, iterations)."""
+    img, mask = _prep(img, mask)
+    H, W = mask.shape
+    D = np.empty((H, W), np.float32)
+    lib = _load()
+    if threads > 1:
+        it = lib.ora_raster_omp(_ptr(img), _img_kind(img), _ptr(mask), _mask_kind(mask), _ptr(D), W, H, lam, v,
+                                int(conn8), max_iters, threads)
+    else:
+        it = lib.ora_raster(_ptr(img), _img_kind(img), _ptr(mask), _mask_kind(mask), _ptr(D), W, H, lam, v,
+                            int(conn8), max_iters)
+    return D, it
+
+
+def max_threads() -> int:
+    return _load().ora_max_threads()
+
+
+def dijkstra(img, mask, lam: float, v: float = 1e10, conn8: bool = True) -> np.ndarray:
+    img, mask = _prep(img, mask)
+    H, W = mask.shape
+    D = np.empty((H, W), np.float32)
+    if _load().ora_dijkstra(_ptr(img), _img_kind(img), _ptr(mask), _mask_kind(mask), _ptr(D), W, H, lam, v,
+                            int(conn8)) != 0:
+        raise MemoryError("oracle dijkstra allocation failed")
+    return D
+
+
+def pred(img, mask, D, lam: float, v: float, conn8: bool = True) -> np.ndarray:
+    img, mask = _prep(img, mask)
+    D = np.ascontiguousarray(D, np.float32)
+    H, W = mask.shape
+    P = np.empty((H, W), np.uint8)
+    _load().ora_pred(_ptr(img), _img_kind(img), _ptr(mask), _mask_kind(mask), _ptr(D), _ptr(P), W, H, lam, v,
+                     int(conn8))
+    return P
+
+
+def backward(img, D, P, Wt, lam: float, nu: float):
+    """Reference gradient of sum(Wt*D): returns (dimg, dmask, G) as float64."""
+    img = np.ascontiguousarray(img, np.float32)
+    D = np.ascontiguousarray(D, np.float32)
+    P = np.ascontiguousarray(P, np.uint8)
+    Wt = np.ascontiguousarray(Wt, np.float32)
+    H, W = D.shape
+    dimg = np.empty((H, W), np.float64)
+    dmask = np.empty((H, W), np.float64)
+    G = np.empty((H, W), np.float64)
+    _load().ora_backward(_ptr(img), _ptr(D), _ptr(P), _ptr(Wt), _ptr(dimg), _ptr(dmask), _ptr(G), W, H, lam, nu)
+    return dimg, dmask, G
+
+
+def raster_f64(img, smask, lam: float, nu: float, conn8: bool = True) -> np.ndarray:
+    img = np.ascontiguousarray(img, np.float64)
+    smask = np.ascontiguousarray(smask, np.float64)
+    H, W = img.shape
+    D = np.empty((H, W), np.float64)
+    _load().ora_raster_f64(_ptr(img), _ptr(smask), _ptr(D), W, H, lam, nu, int(conn8))
+    return D
+
+
+def fd_grad(img, smask, Wt, lam: float, nu: float, idx, eps: float = 1e-4, conn8: bool = True):
+    """FP64 central differences of sum(Wt*D) at flat pixel indices idx."""
+    img = np.ascontiguousarray(img, np.float32)
+    smask = np.ascontiguousarray(smask, np.float32)
+    Wt = np.ascontiguousarray(Wt, np.float32)
+    idx = np.ascontiguousarray(idx, np.int64)
+    H, W = img.shape
+    gi = np.empty(len(idx), np.float64)
+    gm = np.empty(len(idx), np.float64)
+    _load().ora_fd_grad(_ptr(img), _ptr(smask), _ptr(Wt), W, H, lam, nu, int(conn8), eps, _ptr(idx), len(idx),
+                        _ptr(gi), _ptr(gm))
+    return gi, gm
