@@ -1,0 +1,233 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X GGDT on its headline workload.
+
+Metric: converged Mpix/s on one 4096x4096 RGB8 image (three seeded blob
+fields), soft float32 seed mask of 64 discs (nu = 1e3), C8, lambda swept over
+{0.0, 0.5, 1.0}; the schedule is the persistent tile queue ("task").  One
+step = the three converged transforms of the sweep; inputs and outputs stay
+resident in HBM; the timed region is bracketed by device synchronisation.
+
+The JSON line also carries:
+  roofline      the persistent solve kernel (k_task): algorithmic bytes per
+                launch (11 B/px: RGB8 3 + D read 4 + D write 4) over its
+                average duration, from HIP events recorded on its stream
+                during the timed steps; `traffic` comes from the rocprofv3
+                FETCH_SIZE / WRITE_SIZE summary committed under profiles/
+                (null when absent);
+  cpu_baseline  the CPU oracle (a restatement of the reference's raster GGDT)
+                on the same 4096x4096 image at lambda=0.5, OpenMP over the
+                box's cores, also checked bit-exact against the GPU result;
+  batched       256 uneven uint8 images (C8, lambda=0.8): one graph-captured
+                persistent launch vs 256 per-image calls, images/s.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "converged Mpix/s @4096² RGB8 C8 + % HBM roofline; batched vs per-image imgs/s"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_PX = 11  # RGB8 3 + D read 4 + D write 4 (k_task); mask 4 B/px is read by k_init
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def load_traffic():
+    """HBM bytes per k_task launch measured by rocprofv3 (profiles/traffic_k_task_rgb4096.json)."""
+    p = os.path.join(ROOT, "profiles", "traffic_k_task_rgb4096.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_cores() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--tile-rows", type=int, default=0)
+    ap.add_argument("--max-sweeps", type=int, default=0)
+    ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched comparison")
+    ap.add_argument("--batch-count", type=int, default=256)
+    a = ap.parse_args(argv)
+
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on exactly one GPU (no multi-GPU path); use --gpus 1"}))
+        return 2
+
+    import numpy as np
+    import torch
+
+    import distance_transform_benchmarks_amd as dt
+    from distance_transform_benchmarks_amd import synth
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    lambdas = (0.0, 0.5, 1.0)
+    kw = dict(connectivity=8, tile_rows=a.tile_rows, max_sweeps=a.max_sweeps, waves_per_cu=a.waves_per_cu)
+
+    t0 = time.time()
+    img_np, mask_np = synth.rgb4096(a.size)
+    log(f"inputs {a.size}x{a.size} RGB8 generated in {time.time() - t0:.1f}s")
+    img = torch.from_numpy(img_np).to(dev)
+    mask = torch.from_numpy(mask_np).to(dev)
+    outs = {lam: torch.empty((a.size, a.size), dtype=torch.float32, device=dev) for lam in lambdas}
+    npx = a.size * a.size
+
+    def step():
+        for lam in lambdas:
+            dt.ggdt(img, mask, lam, 1e3, out=outs[lam], **kw)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # Per-lambda statistics (outside the timed region; these calls synchronise).
+    per_lambda = {}
+    for lam in lambdas:
+        _, st = dt.ggdt(img, mask, lam, 1e3, out=outs[lam], stats=True, **kw)
+        s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s0.record()
+        for _ in range(3):
+            dt.ggdt(img, mask, lam, 1e3, out=outs[lam], **kw)
+        s1.record()
+        torch.cuda.synchronize()
+        ms = s0.elapsed_time(s1) / 3
+        per_lambda[str(lam)] = {"ms": round(ms, 4), "mpix_s": round(npx / ms / 1e3, 1), "visits": st["visits"],
+                                "sweeps": st["sweeps"], "tiles": st["tiles"], "tile_rows": st["tile_rows"],
+                                "workers": st["workers"]}
+        log(f"lambda={lam}: {ms:.3f} ms/call, {per_lambda[str(lam)]['mpix_s']} Mpix/s, stats {st}")
+
+    # Timed region.
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with dt.api.KernelTimer() as kt:
+        e0.record()
+        for _ in range(a.steps):
+            step()
+        e1.record()
+        torch.cuda.synchronize()
+    elapsed_ms = e0.elapsed_time(e1)
+    ms_per_step = elapsed_ms / a.steps
+    value = a.steps * len(lambdas) * npx / (elapsed_ms * 1e3)
+    solve_avg_ms = kt.solve_ms / max(1, kt.calls)
+    setup_avg_ms = kt.setup_ms / max(1, kt.calls)
+    achieved = BYTES_PER_PX * npx / (solve_avg_ms * 1e-3) / 1e9
+    traffic = load_traffic()
+    roofline = {
+        "bound": "hbm", "kernel": "k_task (persistent tile solve)", "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+        "traffic": traffic.get("bytes_per_launch") if traffic else None,
+        "algorithmic_bytes_per_launch": BYTES_PER_PX * npx, "kernel_avg_ms": round(solve_avg_ms, 4),
+        "setup_avg_ms": round(setup_avg_ms, 4), "kernel_launches": kt.calls,
+    }
+    log(f"timed: {ms_per_step:.3f} ms/step, {value:.1f} Mpix/s; k_task avg {solve_avg_ms:.3f} ms "
+        f"-> {achieved:.1f} GB/s algorithmic")
+
+    result = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mpix/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded SplitMix64 blob fields and discs)",
+        "config": {"workload": f"rgb4096: {a.size}x{a.size} RGB8, soft f32 mask (64 discs, nu=1e3), C8, "
+                               f"lambda sweep {{0, 0.5, 1}}, converged, task schedule",
+                   "pixels_per_step": len(lambdas) * npx, "tile_rows": per_lambda["0.0"]["tile_rows"]},
+        "per_lambda": per_lambda, "roofline": roofline,
+    }
+
+    if not a.no_batched:
+        result["batched"] = bench_batched(dt, synth, torch, dev, a.batch_count)
+
+    if not a.no_cpu:
+        from oracle import oracle as O
+
+        cores = cpu_cores()
+        lam = 0.5
+        t = time.time()
+        want, iters = O.raster(img_np, mask_np, lam, 1e3, conn8=True, threads=cores)
+        cpu_s = time.time() - t
+        got = outs[lam].cpu().numpy()
+        exact = bool(np.array_equal(got.view(np.uint32), want.view(np.uint32)))
+        result["cpu_baseline"] = {
+            "value": round(npx / cpu_s / 1e6, 4), "unit": "Mpix/s", "cores": cores, "kind": "port",
+            "sample": f"CPU oracle raster GGDT, full {a.size}x{a.size} RGB8 image at lambda=0.5 ({iters} raster "
+                      f"iterations, {cpu_s:.1f}s, OpenMP {cores} threads)",
+            "gpu_bit_exact": exact,
+        }
+        log(f"cpu baseline: {cpu_s:.1f}s ({iters} iterations, {cores} threads); GPU bit-exact: {exact}")
+
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def bench_batched(dt, synth, torch, dev, count):
+    """256 uneven uint8 images, C8, lambda=0.8: graph-captured batched launch vs per-image calls."""
+    t = time.time()
+    data = synth.batch256(count)
+    imgs = [torch.from_numpy(i).to(dev) for i, _ in data]
+    masks = [torch.from_numpy(m).to(dev) for _, m in data]
+    outs_a = [torch.empty(m.shape, dtype=torch.float32, device=dev) for m in masks]
+    outs_b = [torch.empty(m.shape, dtype=torch.float32, device=dev) for m in masks]
+    px = sum(int(m.numel()) for m in masks)
+    log(f"batched inputs: {count} images, {px / 1e6:.1f} Mpx, generated in {time.time() - t:.1f}s")
+    lam = 0.8
+
+    def per_image():
+        for i in range(count):
+            dt.ggdt(imgs[i], masks[i], lam, 1e10, out=outs_a[i])
+
+    plan = dt.BatchPlan(imgs, masks, lam, 1e10, outs=outs_b)
+    for _ in range(2):
+        per_image()
+        plan.launch()
+    torch.cuda.synchronize()
+    reps = 3
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    e[0].record()
+    for _ in range(reps):
+        per_image()
+    e[1].record()
+    e[2].record()
+    for _ in range(reps):
+        plan.launch()
+    e[3].record()
+    torch.cuda.synchronize()
+    per_ms = e[0].elapsed_time(e[1]) / reps
+    bat_ms = e[2].elapsed_time(e[3]) / reps
+    same = all(torch.equal(x.view(torch.int32), y.view(torch.int32)) for x, y in zip(outs_a, outs_b))
+    plan.close()
+    r = {"workload": f"batch256: {count} uint8 images, sides log-uniform [64, 2048], aspect [0.5, 2], C8, "
+                     f"lambda=0.8", "pixels": px,
+         "per_image_ms": round(per_ms, 3), "per_image_imgs_s": round(count / per_ms * 1e3, 1),
+         "batched_graph_ms": round(bat_ms, 3), "batched_imgs_s": round(count / bat_ms * 1e3, 1),
+         "speedup": round(per_ms / bat_ms, 3), "outputs_identical": bool(same)}
+    log(f"batched: per-image {per_ms:.2f} ms, graph {bat_ms:.2f} ms, x{per_ms / bat_ms:.2f}, identical {same}")
+    return r
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -27,7 +27,8 @@ EXPORTS = (
     "dtk_params_default", "dtk_last_error", "dtk_version", "dtk_ggdt", "dtk_ggdt_batched",
     "dtk_batch_plan_create", "dtk_batch_plan_launch", "dtk_batch_plan_destroy", "dtk_ggdt_pred",
     "dtk_ggdt_backward", "dtk_tile_count", "dtk_malloc_pitch", "dtk_free", "dtk_host_to_device",
-    "dtk_device_to_host", "dtk_copy2d", "dtk_selftest", "dtk_device_cu_count",
+    "dtk_device_to_host", "dtk_copy2d", "dtk_selftest", "dtk_device_cu_count", "dtk_profile_begin",
+    "dtk_profile_end",
 )
 
 
@@ -94,9 +95,12 @@ def load() -> ctypes.CDLL:
                                ctypes.c_int]
     lib.dtk_selftest.argtypes = [P(ctypes.c_int)]
     lib.dtk_device_cu_count.restype = ctypes.c_int
+    lib.dtk_profile_begin.argtypes = []
+    lib.dtk_profile_end.argtypes = [P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_int)]
     for name in ("dtk_ggdt", "dtk_ggdt_batched", "dtk_batch_plan_create", "dtk_batch_plan_launch",
                  "dtk_batch_plan_destroy", "dtk_ggdt_pred", "dtk_ggdt_backward", "dtk_malloc_pitch", "dtk_free",
-                 "dtk_host_to_device", "dtk_device_to_host", "dtk_copy2d", "dtk_selftest"):
+                 "dtk_host_to_device", "dtk_device_to_host", "dtk_copy2d", "dtk_selftest", "dtk_profile_begin",
+                 "dtk_profile_end"):
         getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib

@@ -224,6 +224,21 @@ def backward(img, mask, dist, pred, grad_dist, lambda_, v=1e10, *, connectivity=
     return gi, gm
 
 
+class KernelTimer:
+    """Sums the device time of the task-schedule kernels of every call made
+    inside the ``with`` block (HIP events recorded on each call's stream)."""
+
+    def __enter__(self):
+        check(_lib.load().dtk_profile_begin())
+        return self
+
+    def __exit__(self, *exc):
+        a, b, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        check(_lib.load().dtk_profile_end(ctypes.byref(a), ctypes.byref(b), ctypes.byref(n)))
+        self.setup_ms, self.solve_ms, self.calls = a.value, b.value, n.value
+        return False
+
+
 def selftest() -> dict:
     out = (ctypes.c_int * 3)()
     check(_lib.load().dtk_selftest(out))

@@ -58,6 +58,34 @@ namespace
 
   Workspace g_ws[kMaxDevices];
 
+  // Kernel timing for benchmarks: while active, every task-schedule call
+  // records three HIP events on its stream (before the reset/init/seed
+  // kernels, before the persistent solve, after it).  No synchronisation
+  // happens until dtk_profile_end.  One profiler per process.
+  struct Profiler
+  {
+    std::mutex             mu;
+    bool                   on = false;
+    std::vector<hipEvent_t> ev;
+    size_t                 used = 0;
+  };
+  Profiler g_prof;
+
+  void prof_mark(hipStream_t s)
+  {
+    std::lock_guard<std::mutex> lock(g_prof.mu);
+    if (!g_prof.on)
+      return;
+    if (g_prof.used == g_prof.ev.size())
+    {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess)
+        return;
+      g_prof.ev.push_back(e);
+    }
+    hipEventRecord(g_prof.ev[g_prof.used++], s);
+  }
+
   int elem_size(int dtype)
   {
     switch (dtype)
@@ -335,12 +363,18 @@ namespace
 
   // Enqueue reset + init + seed + persistent solve for a prepared batch.
   dtk_status enqueue_task(const Batch& b, const KParams& kp, const Queue& q, int S, bool c8, bool two, int workers,
-                          hipStream_t s)
+                          hipStream_t s, bool profile = true)
   {
+    if (profile)
+      prof_mark(s);
     DTK_HIP(launch_reset_ctrl(q.ctrl, s), "launch reset");
     DTK_HIP(launch_init(b, kp, q, s), "launch init");
     DTK_HIP(launch_seed(b, q, s), "launch seed");
+    if (profile)
+      prof_mark(s);
     DTK_HIP(launch_task(S, c8, two, b, kp, q, workers, s), "launch task");
+    if (profile)
+      prof_mark(s);
     return DTK_OK;
   }
 
@@ -639,7 +673,7 @@ dtk_status dtk_batch_plan_create(int n, const dtk_image* imgs, const dtk_image* 
   e          = hipStreamBeginCapture(pl->cstream, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess)
   {
-    st            = enqueue_task(b, kp, tb.q, S, c8, two, workers, pl->cstream);
+    st            = enqueue_task(b, kp, tb.q, S, c8, two, workers, pl->cstream, false);
     hipError_t e2 = hipStreamEndCapture(pl->cstream, &pl->graph);
     if (st != DTK_OK)
     {
@@ -804,6 +838,40 @@ dtk_status dtk_host_to_device(const dtk_image* src, dtk_image* dst)
 dtk_status dtk_device_to_host(const dtk_image* src, dtk_image* dst)
 {
   return copy2d(src, dst, hipMemcpyDeviceToHost, "Unable to transfert device to host: ");
+}
+
+dtk_status dtk_profile_begin(void)
+{
+  std::lock_guard<std::mutex> lock(g_prof.mu);
+  g_prof.on   = true;
+  g_prof.used = 0;
+  return DTK_OK;
+}
+
+dtk_status dtk_profile_end(double* setup_ms, double* solve_ms, int* calls)
+{
+  std::lock_guard<std::mutex> lock(g_prof.mu);
+  g_prof.on  = false;
+  double a = 0, b = 0;
+  int    n = 0;
+  for (size_t i = 0; i + 3 <= g_prof.used; i += 3)
+  {
+    float t0 = 0, t1 = 0;
+    DTK_HIP(hipEventSynchronize(g_prof.ev[i + 2]), "hipEventSynchronize");
+    DTK_HIP(hipEventElapsedTime(&t0, g_prof.ev[i], g_prof.ev[i + 1]), "hipEventElapsedTime");
+    DTK_HIP(hipEventElapsedTime(&t1, g_prof.ev[i + 1], g_prof.ev[i + 2]), "hipEventElapsedTime");
+    a += t0;
+    b += t1;
+    ++n;
+  }
+  if (setup_ms)
+    *setup_ms = a;
+  if (solve_ms)
+    *solve_ms = b;
+  if (calls)
+    *calls = n;
+  g_prof.used = 0;
+  return DTK_OK;
 }
 
 dtk_status dtk_copy2d(void* dst, long long dst_pitch, const void* src, long long src_pitch, long long width_bytes,

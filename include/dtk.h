@@ -164,6 +164,15 @@ dtk_status dtk_device_to_host(const dtk_image* src, dtk_image* dst);
 dtk_status dtk_copy2d(void* dst, long long dst_pitch, const void* src, long long src_pitch, long long width_bytes,
                       int height, int kind);
 
+/* Kernel timing for benchmarks (no reference counterpart; the reference
+ * times whole calls with CUDA events, benchmarks/distance_transform.cpp:78-89).
+ * Between begin and end, every task-schedule call records HIP events on its
+ * own stream around its setup kernels (reset, init, seed) and around the
+ * persistent solve; end waits for them and returns the summed durations and
+ * the number of calls.  One profiler per process. */
+dtk_status dtk_profile_begin(void);
+dtk_status dtk_profile_end(double* setup_ms, double* solve_ms, int* calls);
+
 /* Device self-check of the assumptions the kernels rely on: DPP lane
  * direction of wave shifts and correctly rounded sqrtf over every RGB8
  * squared distance.  out[0] = DPP ok, out[1] = number of sqrtf mismatches
