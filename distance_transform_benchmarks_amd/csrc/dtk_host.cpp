@@ -240,7 +240,7 @@ namespace
 
   bool two_adds(const dtk_params* p) { return !(p->lambda == 0.f || p->lambda == 1.f); }
 
-  int region_rows(const dtk_params* p) { return p->tile_rows == 0 ? 16 : p->tile_rows; }
+  int region_rows(const dtk_params* p) { return p->tile_rows == 0 ? 32 : p->tile_rows; }
 
   ImgDesc make_desc(const dtk_image* img, const dtk_image* mask, dtk_image* dist, int rh)
   {
@@ -327,6 +327,12 @@ namespace
     st->tiles     = tiles;
     st->tile_rows = rh;
     st->workers   = workers;
+    st->cycles_wait  = ctrl_host ? ctrl_host[CTRL_T_WAIT] : 0;
+    st->cycles_visit = ctrl_host ? ctrl_host[CTRL_T_VISIT] : 0;
+    st->cycles_pub   = ctrl_host ? ctrl_host[CTRL_T_PUB] : 0;
+    st->cycles_setup = ctrl_host ? ctrl_host[CTRL_T_SETUP] : 0;
+    st->cycles_sweep = ctrl_host ? ctrl_host[CTRL_T_SWEEP] : 0;
+    st->cycles_store = ctrl_host ? ctrl_host[CTRL_T_STORE] : 0;
     return DTK_OK;
   }
 
@@ -411,7 +417,7 @@ const char* dtk_version(void) { return "dtk 0.1 (gfx950)"; }
 
 int dtk_tile_count(int width, int height, int tile_rows)
 {
-  const int rh = tile_rows == 0 ? 16 : tile_rows;
+  const int rh = tile_rows == 0 ? 32 : tile_rows;
   if (width <= 0 || height <= 0 || (rh != 32 && rh != 16))
     return 0;
   return ((width + kTileW - 1) / kTileW) * ((height + rh - 3) / (rh - 2));
@@ -455,15 +461,18 @@ dtk_status dtk_ggdt(const dtk_image* img, const dtk_image* mask, dtk_image* dist
     uint32_t* changed  = static_cast<uint32_t*>(w->dbuf);
     uint32_t* progress = changed + 64;
     uint32_t* h        = static_cast<uint32_t*>(w->hbuf);
-    DTK_HIP(launch_raster_init(d, kp, s), "launch raster init");
+    DTK_HIP(hipMemsetAsync(changed, 0, 2 * sizeof(uint32_t), s), "memset");
+    DTK_HIP(launch_raster_init(d, kp, changed + 1, s), "launch raster init");
     int iters = 0;
     for (;;)
     {
-      DTK_HIP(hipMemsetAsync(changed, 0, 2 * sizeof(uint32_t), s), "memset");
+      DTK_HIP(hipMemsetAsync(changed, 0, sizeof(uint32_t), s), "memset");
       DTK_HIP(launch_raster_iteration(d, kp, c8, changed, progress, s), "launch raster pass");
       DTK_HIP(hipMemcpyAsync(h, changed, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read flag");
       DTK_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
       ++iters;
+      if (h[1] & ERR_MASK_RANGE)
+        return fail(DTK_ERR_INVALID_ARGUMENT, "soft mask values must be in [0, 1]");
       if (h[1] & ERR_WATCHDOG)
         return fail(DTK_ERR_HIP, "device watchdog expired in the raster pipeline");
       if (!h[0] || (p->max_iterations > 0 && iters >= p->max_iterations))

@@ -81,12 +81,18 @@ namespace dtk
   enum CtrlWord : int
   {
     CTRL_HEAD    = 0,
-    CTRL_TAIL    = 16,
-    CTRL_PENDING = 32,
+    CTRL_TP      = 16, // 64-bit: low 32 bits = tail (tickets issued to pushers),
+                       //         high 32 bits = pending (tiles queued or running)
     CTRL_ERROR   = 48,
     CTRL_VISITS  = 64,
     CTRL_SWEEPS  = 80,
     CTRL_CHANGED = 96,
+    CTRL_T_WAIT  = 100, // summed s_memtime cycles / 1024 of the task workers:
+    CTRL_T_VISIT = 101, //   waiting for a tile, solving tiles,
+    CTRL_T_PUB   = 102, //   publishing (release, wake-ups, re-queue)
+    CTRL_T_SETUP = 103, //   inside visits: loads + edge costs,
+    CTRL_T_SWEEP = 104, //   sweeps,
+    CTRL_T_STORE = 105, //   border check + write-back
     CTRL_WORDS   = 128,
   };
 

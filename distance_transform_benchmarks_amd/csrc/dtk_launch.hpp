@@ -20,7 +20,7 @@ namespace dtk
 
   // dtk_raster.hip: one full raster iteration (four passes) with the exact
   // reference schedule; `changed` (device u32) is set when a pixel decreased.
-  hipError_t launch_raster_init(const ImgDesc& d, const KParams& p, hipStream_t s);
+  hipError_t launch_raster_init(const ImgDesc& d, const KParams& p, uint32_t* err, hipStream_t s);
   hipError_t launch_raster_iteration(const ImgDesc& d, const KParams& p, bool c8, uint32_t* changed,
                                      uint32_t* progress, hipStream_t s);
   size_t     raster_progress_words(const ImgDesc& d);

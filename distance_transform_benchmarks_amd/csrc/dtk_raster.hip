@@ -34,7 +34,7 @@ namespace dtk
     constexpr int kGhost  = 16; // ghost lines per side == steps per chunk
   }
 
-  __global__ void k_raster_init(ImgDesc d, KParams p)
+  __global__ void k_raster_init(ImgDesc d, KParams p, uint32_t* err)
   {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
@@ -44,7 +44,12 @@ namespace dtk
     if (p.mask_kind == MASK_U8)
       d0 = p.v * (float)(d.mask[(long long)y * d.mask_pitch + x] == 0);
     else
-      d0 = p.v * (1.f - reinterpret_cast<const float*>(d.mask + (long long)y * d.mask_pitch)[x]);
+    {
+      const float s = reinterpret_cast<const float*>(d.mask + (long long)y * d.mask_pitch)[x];
+      if (!(s >= 0.f && s <= 1.f))
+        __hip_atomic_fetch_or(err, (uint32_t)ERR_MASK_RANGE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      d0 = p.v * (1.f - s);
+    }
     reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(d.D) + (long long)y * d.D_pitch)[x] = d0;
   }
 
@@ -123,31 +128,30 @@ namespace dtk
       // Previous step: owned lanes keep their registers, the others reload.
       {
         const int s = start + dir * (k0 - 1);
+        // Clamped unconditional loads, masked afterwards.
+        const int   cl = min(max(line, 0), nlines - 1);
+        const int   x = ROWS ? s : cl, y = ROWS ? cl : s;
+        const float dv = gload<float>(d.D, (long long)y * d.D_pitch + 4ll * x);
+        const uint32_t iv = img_at<KIND>(d, x, y);
         if (c == 0 || !own)
         {
-          Dprev = DTK_INF;
-          Iprev = 0;
-          if (lin)
-          {
-            const int x = ROWS ? s : line, y = ROWS ? line : s;
-            Dprev      = d_at(d, x, y);
-            Iprev      = img_at<KIND>(d, x, y);
-          }
+          Dprev = lin ? dv : DTK_INF;
+          Iprev = lin ? iv : 0u;
         }
       }
       float    Dv[kGhost];
       uint32_t Iv[kGhost];
-#pragma unroll
-      for (int j = 0; j < kGhost; ++j)
       {
-        Dv[j] = DTK_INF;
-        Iv[j] = 0;
-        if (j < kn && lin)
+        const int cl = min(max(line, 0), nlines - 1);
+#pragma unroll
+        for (int j = 0; j < kGhost; ++j)
         {
-          const int s = start + dir * (k0 + j);
-          const int x = ROWS ? s : line, y = ROWS ? line : s;
-          Dv[j]      = d_at(d, x, y);
-          Iv[j]      = img_at<KIND>(d, x, y);
+          const int      s  = start + dir * (k0 + min(j, kn - 1));
+          const int      x = ROWS ? s : cl, y = ROWS ? cl : s;
+          const float    dv = gload<float>(d.D, (long long)y * d.D_pitch + 4ll * x);
+          const uint32_t iv = img_at<KIND>(d, x, y);
+          Dv[j]             = (j < kn && lin) ? dv : DTK_INF;
+          Iv[j]             = (j < kn && lin) ? iv : 0u;
         }
       }
 
@@ -163,7 +167,7 @@ namespace dtk
             g       = (g == g) ? g : DTK_INF;
             const float cost = p.zero_lambda ? ea : g;
             const float cur  = TWO ? (Dprev + ea) + cost : Dprev + cost;
-            nd               = fminf(nd, cur);
+            nd               = minf_nn(nd, cur);
           }
           if constexpr (C8)
           {
@@ -181,7 +185,7 @@ namespace dtk
             curr              = (linR && lane < 63) ? curr : DTK_INF;
             nd                = min3f(nd, curl, curr);
           }
-          if (lin && nd < Dv[j])
+          if (lin && lt_nn(nd, Dv[j]))
           {
             Dv[j] = nd;
             any |= own;
@@ -248,11 +252,11 @@ namespace dtk
                : raster_iter_t<KIND, false, false>(d, p, changed, progress, s);
   }
 
-  hipError_t launch_raster_init(const ImgDesc& d, const KParams& p, hipStream_t s)
+  hipError_t launch_raster_init(const ImgDesc& d, const KParams& p, uint32_t* err, hipStream_t s)
   {
     if (d.W <= 0 || d.H <= 0)
       return hipSuccess;
-    hipLaunchKernelGGL(k_raster_init, dim3((d.W + 255) / 256, d.H), dim3(256), 0, s, d, p);
+    hipLaunchKernelGGL(k_raster_init, dim3((d.W + 255) / 256, d.H), dim3(256), 0, s, d, p, err);
     return hipGetLastError();
   }
 

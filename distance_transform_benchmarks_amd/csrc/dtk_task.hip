@@ -17,9 +17,18 @@ namespace dtk
   __device__ __forceinline__ void resolve_tile(const Batch& b, int t, ImgDesc& d, int& tx, int& ty)
   {
     if (b.nimg == 1)
-      d = b.d0;
+      d = b.d0; // kernel argument: stays in scalar registers
     else
-      d = b.table[b.tile_img[t]];
+    {
+      // Table read through the global address space, 8 bytes at a time.
+      static_assert(sizeof(ImgDesc) % 8 == 0, "ImgDesc must be a whole number of 8-byte words");
+      const uint32_t     i = gload<uint32_t>(b.tile_img, 4ll * t);
+      unsigned long long w[sizeof(ImgDesc) / 8];
+#pragma unroll
+      for (int k = 0; k < (int)(sizeof(ImgDesc) / 8); ++k)
+        w[k] = gload<unsigned long long>(b.table, (long long)i * (long long)sizeof(ImgDesc) + 8ll * k);
+      __builtin_memcpy(&d, w, sizeof(ImgDesc));
+    }
     const int local = t - d.tile0;
     tx              = local % d.tiles_x;
     ty              = local / d.tiles_x;
@@ -40,6 +49,19 @@ namespace dtk
     return v;
   }
 
+  // tail and pending share one 64-bit word, so one atomic both reserves
+  // tickets and counts the new items as pending.
+  constexpr unsigned long long kOnePendingOneTicket = (1ull << 32) | 1ull;
+  constexpr unsigned long long kMinusOnePending     = ~0ull << 32; // adds -1 to the high half
+
+  __device__ __forceinline__ unsigned long long* tp_word(const Queue& q)
+  {
+    return reinterpret_cast<unsigned long long*>(q.ctrl + CTRL_TP);
+  }
+  __device__ __forceinline__ uint32_t pending_now(const Queue& q)
+  {
+    return (uint32_t)(__hip_atomic_load(tp_word(q), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 32);
+  }
   // One wave per tile: D0 of the owned pixels, tile min / max of D0.
   // D0 = v * (mask == 0) for a uint8 mask (mask != 0 marks a seed), or
   // nu * (1 - s) for a soft float mask s in [0, 1].
@@ -55,28 +77,27 @@ namespace dtk
     const int  Y0  = ty * (RH - 2) - 1;
     const int  gx  = X0 + lane;
     const bool own = lane >= 1 && lane <= kTileW && gx < d.W;
+    const int  cx  = min(max(gx, 0), d.W - 1); // in-bounds address for every lane
     float      mn = DTK_INF, mx = -DTK_INF;
     uint32_t   err = 0;
-    if (own)
+    // Unconditional clamped loads (all rows in flight), masked stores.
+    for (int r = 1; r < RH - 1; ++r)
     {
-      for (int r = 1; r < RH - 1; ++r)
+      const int  gy = Y0 + r;
+      const bool ok = own && gy < d.H;
+      const int  cy = min(max(gy, 0), d.H - 1);
+      float      d0;
+      if (p.mask_kind == MASK_U8)
+        d0 = p.v * (float)(gload<unsigned char>(d.mask, (long long)cy * d.mask_pitch + cx) == 0);
+      else
       {
-        const int gy = Y0 + r;
-        if (gy >= d.H)
-          break;
-        float d0;
-        if (p.mask_kind == MASK_U8)
-        {
-          const unsigned char m = d.mask[(long long)gy * d.mask_pitch + gx];
-          d0                    = p.v * (float)(m == 0);
-        }
-        else
-        {
-          const float s = reinterpret_cast<const float*>(d.mask + (long long)gy * d.mask_pitch)[gx];
-          if (!(s >= 0.f && s <= 1.f))
-            err |= ERR_MASK_RANGE;
-          d0 = p.v * (1.f - s);
-        }
+        const float s = gload<float>(d.mask, (long long)cy * d.mask_pitch + 4ll * cx);
+        if (ok && !(s >= 0.f && s <= 1.f))
+          err |= ERR_MASK_RANGE;
+        d0 = p.v * (1.f - s);
+      }
+      if (ok)
+      {
         reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(d.D) + (long long)gy * d.D_pitch)[gx] = d0;
         mn = fminf(mn, d0);
         mx = fmaxf(mx, d0);
@@ -132,8 +153,8 @@ namespace dtk
     if (nmin < q.tmax[t])
     {
       q.state[t] = TILE_QUEUED;
-      __hip_atomic_fetch_add(&q.ctrl[CTRL_PENDING], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const uint32_t ticket = __hip_atomic_fetch_add(&q.ctrl[CTRL_TAIL], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t ticket = (uint32_t)__hip_atomic_fetch_add(tp_word(q), kOnePendingOneTicket, __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_AGENT);
       q.slots[ticket & q.cap_mask] = ((unsigned long long)(2u * ticket + 1u) << 32) | (uint32_t)t;
     }
   }
@@ -161,7 +182,7 @@ namespace dtk
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         return t;
       }
-      if (__hip_atomic_load(&q.ctrl[CTRL_PENDING], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
+      if (pending_now(q) == 0u)
         return -1;
       if (spin >= (kSpinLimit >> 2))
       {
@@ -176,80 +197,85 @@ namespace dtk
     }
   }
 
-  // Put a tile whose state is already QUEUED into the ring.
-  __device__ void ring_insert(const Queue& q, int t)
+  // Publish a finished visit, wave-wide (called by all 64 lanes after a
+  // release fence): lanes 0..7 wake the neighbours whose halo changed
+  // (idle -> QUEUED, running -> RUNNING|DIRTY), lane 8 moves the tile itself
+  // to idle or, when a wake-up arrived while it ran (or the visit hit its
+  // sweep cap), back to QUEUED.  Tiles that became QUEUED go into the ring
+  // with one pending reservation and one ticket reservation for all of them.
+  __device__ void publish(const Queue& q, const ImgDesc& d, int t, int tx, int ty, uint32_t border, bool requeue)
   {
-    __hip_atomic_fetch_add(&q.ctrl[CTRL_PENDING], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t      ticket   = __hip_atomic_fetch_add(&q.ctrl[CTRL_TAIL], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned long long* slot     = &q.slots[ticket & q.cap_mask];
-    const uint32_t      free_seq = 2u * ticket;
-    for (uint32_t spin = 0;
-         (uint32_t)(__hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 32) != free_seq; ++spin)
+    const int lane   = threadIdx.x;
+    int       target = -1;
+    if (lane < 8)
     {
-      if (spin >= kSpinLimit)
-      {
-        __hip_atomic_fetch_or(&q.ctrl[CTRL_ERROR], (uint32_t)ERR_WATCHDOG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return;
-      }
-      __builtin_amdgcn_s_sleep(1);
+      const int nx = tx + kNbrDX[lane], ny = ty + kNbrDY[lane];
+      if (((border >> lane) & 1u) && nx >= 0 && ny >= 0 && nx < d.tiles_x && ny < d.tiles_y)
+        target = d.tile0 + ny * d.tiles_x + nx;
     }
-    __hip_atomic_store(slot, ((unsigned long long)(free_seq + 1u) << 32) | (uint32_t)t, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-  }
-
-  // Wake a tile (caller issued a release fence after its D stores):
-  //   idle    -> QUEUED and into the ring
-  //   queued  -> unchanged (it will load the new halo when popped)
-  //   running -> RUNNING|DIRTY (its wave re-queues it when the visit ends)
-  __device__ void queue_push(const Queue& q, int t)
-  {
-    uint32_t s = __hip_atomic_load(&q.state[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (;;)
+    else if (lane == 8)
+      target = t;
+    bool insert = false;
+    if (target >= 0)
     {
-      if (s & TILE_QUEUED)
-        return;
-      const uint32_t n = (s & TILE_RUNNING) ? (s | TILE_DIRTY) : (uint32_t)TILE_QUEUED;
-      if (n == s)
-        return;
-      if (__hip_atomic_compare_exchange_weak(&q.state[t], &s, n, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT))
+      // First attempt assumes the usual state (neighbour idle, own tile
+      // running) and saves a load round trip.
+      uint32_t st = lane == 8 ? (uint32_t)TILE_RUNNING : 0u;
+      for (;;)
       {
-        if (n == TILE_QUEUED)
-          ring_insert(q, t);
-        return;
-      }
-    }
-  }
-
-  // End of a visit (caller issued a release fence after its D stores):
-  // RUNNING -> idle, or RUNNING|DIRTY (or an unconverged visit) -> QUEUED.
-  __device__ void queue_finish(const Queue& q, int t, bool requeue)
-  {
-    uint32_t s = __hip_atomic_load(&q.state[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (;;)
-    {
-      const uint32_t n = (requeue || (s & TILE_DIRTY)) ? (uint32_t)TILE_QUEUED : 0u;
-      if (__hip_atomic_compare_exchange_weak(&q.state[t], &s, n, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT))
-      {
-        if (n == TILE_QUEUED)
-          ring_insert(q, t);
-        return;
+        uint32_t n;
+        if (lane == 8)
+          n = (requeue || (st & TILE_DIRTY)) ? (uint32_t)TILE_QUEUED : 0u;
+        else
+        {
+          if (st & TILE_QUEUED)
+            break;
+          n = (st & TILE_RUNNING) ? (st | TILE_DIRTY) : (uint32_t)TILE_QUEUED;
+          if (n == st)
+            break;
+        }
+        if (__hip_atomic_compare_exchange_weak(&q.state[target], &st, n, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT))
+        {
+          insert = n == TILE_QUEUED;
+          break;
+        }
       }
     }
-  }
-
-  __device__ __forceinline__ void wake_neighbours(const Queue& q, const ImgDesc& d, int tx, int ty, uint32_t border)
-  {
-    for (int k = 0; k < 8; ++k)
+    const uint64_t ins  = __ballot(insert);
+    const uint32_t n    = (uint32_t)__popcll(ins);
+    uint32_t       base = 0;
+    if (lane == 0 && n)
     {
-      if (!((border >> k) & 1u))
-        continue;
-      const int nx = tx + kNbrDX[k], ny = ty + kNbrDY[k];
-      if (nx < 0 || ny < 0 || nx >= d.tiles_x || ny >= d.tiles_y)
-        continue;
-      queue_push(q, d.tile0 + ny * d.tiles_x + nx);
+      // One atomic: n tickets and n more pending items.  The items become
+      // poppable only after this (their slots are written afterwards).
+      base = (uint32_t)__hip_atomic_fetch_add(tp_word(q), (unsigned long long)n * kOnePendingOneTicket,
+                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    base = __shfl(base, 0);
+    if (insert)
+    {
+      const uint32_t      ticket   = base + (uint32_t)__popcll(ins & ((1ull << lane) - 1ull));
+      unsigned long long* slot     = &q.slots[ticket & q.cap_mask];
+      const uint32_t      free_seq = 2u * ticket;
+      const unsigned long long full = ((unsigned long long)(free_seq + 1u) << 32) | (uint32_t)target;
+      for (uint32_t spin = 0;; ++spin)
+      {
+        unsigned long long expect = (unsigned long long)free_seq << 32;
+        if (__hip_atomic_compare_exchange_strong(slot, &expect, full, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+          break;
+        if (spin >= kSpinLimit)
+        {
+          __hip_atomic_fetch_or(&q.ctrl[CTRL_ERROR], (uint32_t)ERR_WATCHDOG, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    if (lane == 0)
+      __hip_atomic_fetch_add(tp_word(q), kMinusOnePending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 
   template <int S, bool C8, bool TWO>
@@ -259,13 +285,17 @@ namespace dtk
     __shared__ float Sv[2 * (64 / S)];
     __shared__ int   s_tile;
     uint32_t         visits = 0, sweeps = 0;
+    uint64_t         t_wait = 0, t_visit = 0, t_pub = 0, t_setup = 0, t_sweep = 0, t_store = 0;
     for (;;)
     {
+      const uint64_t c0 = __builtin_amdgcn_s_memtime();
       if (threadIdx.x == 0)
         s_tile = queue_pop(q);
       __syncthreads();
       const int t = s_tile;
       __syncthreads();
+      const uint64_t c1 = __builtin_amdgcn_s_memtime();
+      t_wait += c1 - c0;
       if (t < 0)
         break;
       ImgDesc d;
@@ -274,21 +304,31 @@ namespace dtk
       const VisitResult vr = visit_tile<S, C8, TWO>(d, p, tx, ty, T, Sv);
       ++visits;
       sweeps += (uint32_t)vr.sweeps;
+      t_setup += vr.cyc_setup;
+      t_sweep += vr.cyc_sweep;
+      t_store += vr.cyc_store;
       // Every lane's D stores are ordered before the release below.
       __syncthreads();
-      if (threadIdx.x == 0)
-      {
-        if (vr.changed)
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        wake_neighbours(q, d, tx, ty, vr.border);
-        queue_finish(q, t, !vr.converged);
-        __hip_atomic_fetch_sub(&q.ctrl[CTRL_PENDING], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
+      const uint64_t c2 = __builtin_amdgcn_s_memtime();
+      t_visit += c2 - c1;
+      if (vr.changed)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      publish(q, d, t, tx, ty, vr.border, !vr.converged);
+      t_pub += __builtin_amdgcn_s_memtime() - c2;
     }
     if (threadIdx.x == 0 && visits)
     {
       __hip_atomic_fetch_add(&q.ctrl[CTRL_VISITS], visits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_fetch_add(&q.ctrl[CTRL_SWEEPS], sweeps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (threadIdx.x == 0)
+    {
+      __hip_atomic_fetch_add(&q.ctrl[CTRL_T_WAIT], (uint32_t)(t_wait >> 10), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&q.ctrl[CTRL_T_VISIT], (uint32_t)(t_visit >> 10), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&q.ctrl[CTRL_T_PUB], (uint32_t)(t_pub >> 10), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&q.ctrl[CTRL_T_SETUP], (uint32_t)(t_setup >> 10), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&q.ctrl[CTRL_T_SWEEP], (uint32_t)(t_sweep >> 10), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&q.ctrl[CTRL_T_STORE], (uint32_t)(t_store >> 10), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 

@@ -85,7 +85,30 @@ namespace dtk
     return opaque_u((uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xF, 0xF, true));
   }
 
-  __device__ __forceinline__ float min3f(float a, float b, float c) { return fminf(fminf(a, b), c); }
+  // Minimum of non-negative floats through their bit patterns.  Every value
+  // the solver handles lies in [+0, +inf] (D0 >= 0, costs >= 0), where the
+  // unsigned order of the bits equals the float order, and a NaN (bits above
+  // +inf) never wins -- the same outcome as the strict compares of the
+  // reference (a NaN candidate never replaces D).  Unlike fminf this needs no
+  // NaN canonicalisation.
+  __device__ __forceinline__ float minf_nn(float a, float b)
+  {
+    return __uint_as_float(min(__float_as_uint(a), __float_as_uint(b)));
+  }
+  __device__ __forceinline__ float min3f(float a, float b, float c)
+  {
+    return __uint_as_float(min(min(__float_as_uint(a), __float_as_uint(b)), __float_as_uint(c)));
+  }
+  __device__ __forceinline__ bool lt_nn(float a, float b) { return __float_as_uint(a) < __float_as_uint(b); }
+
+  // Loads through the global address space (generic pointers would compile
+  // to FLAT loads, which also wait on the LDS counter).
+  typedef const __attribute__((address_space(1))) unsigned char* gbytes;
+  template <typename T>
+  __device__ __forceinline__ T gload(const void* base, long long off)
+  {
+    return *reinterpret_cast<const __attribute__((address_space(1))) T*>((gbytes)base + off);
+  }
 
   __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
@@ -94,14 +117,15 @@ namespace dtk
   __device__ __forceinline__ uint32_t load_pix(const unsigned char* row, int x)
   {
     if constexpr (KIND == IMG_U8)
-      return row[x];
+      return gload<unsigned char>(row, x);
     else if constexpr (KIND == IMG_RGB8)
     {
-      const unsigned char* q = row + 3 * x;
-      return (uint32_t)q[0] | ((uint32_t)q[1] << 8) | ((uint32_t)q[2] << 16);
+      const long long o = 3ll * x;
+      return (uint32_t)gload<unsigned char>(row, o) | ((uint32_t)gload<unsigned char>(row, o + 1) << 8) |
+             ((uint32_t)gload<unsigned char>(row, o + 2) << 16);
     }
     else
-      return reinterpret_cast<const uint32_t*>(row)[x];
+      return gload<uint32_t>(row, 4ll * x);
   }
 
   // |I(p) - I(q)|: uint8 abs difference (dt/src/utils.cuh:37-41); RGB8:
@@ -144,6 +168,9 @@ namespace dtk
     int      sweeps;   // sweeps run
     bool     converged;
     bool     changed;  // any owned pixel changed
+    uint32_t cyc_setup; // shader-clock cycles: loads + edge costs
+    uint32_t cyc_sweep; //   sweeps and layout changes
+    uint32_t cyc_store; //   border check and write-back
   };
 
   // Neighbour tiles in border-bit order: L, R, T, B, TL, TR, BL, BR.
@@ -213,8 +240,8 @@ namespace dtk
         const float b = shl_z(t) + cNE[r];
         m             = min3f(a, b, c);
       }
-      ch |= ballot(m < D[r]);
-      D[r] = fminf(m, D[r]);
+      ch |= ballot(lt_nn(m, D[r]));
+      D[r] = minf_nn(m, D[r]);
     }
     return ch;
   }
@@ -242,8 +269,8 @@ namespace dtk
         const float b   = shr_keep(D[r], pNE);
         m               = min3f(a, b, c);
       }
-      ch |= ballot(m < D[r]);
-      D[r] = fminf(m, D[r]);
+      ch |= ballot(lt_nn(m, D[r]));
+      D[r] = minf_nn(m, D[r]);
     }
     return ch;
   }
@@ -269,8 +296,8 @@ namespace dtk
         const float b = shl_z(t) + rSW[k];
         m             = min3f(a, b, c);
       }
-      ch |= ballot(m < R[k]);
-      R[k] = fminf(m, R[k]);
+      ch |= ballot(lt_nn(m, R[k]));
+      R[k] = minf_nn(m, R[k]);
     }
     return ch;
   }
@@ -298,8 +325,8 @@ namespace dtk
         const float b   = shl_keep(R[k], pNW);
         m               = min3f(a, b, c);
       }
-      ch |= ballot(m < R[k]);
-      R[k] = fminf(m, R[k]);
+      ch |= ballot(lt_nn(m, R[k]));
+      R[k] = minf_nn(m, R[k]);
     }
     return ch;
   }
@@ -309,15 +336,18 @@ namespace dtk
   template <int KIND, int RH>
   __device__ __forceinline__ void load_region_image(const ImgDesc& d, int X0, int Y0, int lane, uint32_t (&I)[RH])
   {
+    // Every load is issued unconditionally from a clamped (in-bounds)
+    // address and masked afterwards, so all rows are in flight together.
     const int  gx  = X0 + lane;
     const bool cin = gx >= 0 && gx < d.W;
+    const int  cx  = min(max(gx, 0), d.W - 1);
 #pragma unroll
     for (int r = 0; r < RH; ++r)
     {
-      const int gy = Y0 + r;
-      I[r]         = 0;
-      if (cin && gy >= 0 && gy < d.H)
-        I[r] = load_pix<KIND>(d.img + (long long)gy * d.img_pitch, gx);
+      const int      gy = Y0 + r;
+      const int      cy = min(max(gy, 0), d.H - 1);
+      const uint32_t v  = load_pix<KIND>(d.img + (long long)cy * d.img_pitch, cx);
+      I[r]              = (cin && gy >= 0 && gy < d.H) ? v : 0u;
     }
   }
 
@@ -422,7 +452,7 @@ namespace dtk
   // its local fixed point (or max_sweeps), writes back the owned pixels and
   // reports which neighbours saw a border pixel change.
   template <int S, bool C8, bool TWO>
-  __device__ VisitResult visit_tile(const ImgDesc& d, const KParams& p, int tx, int ty, float* T, float* Sv)
+  __device__ __attribute__((always_inline)) VisitResult visit_tile(const ImgDesc& d, const KParams& p, int tx, int ty, float* T, float* Sv)
   {
     constexpr int RH   = 64 / S;
     constexpr int SW   = 64 / S;
@@ -431,6 +461,7 @@ namespace dtk
     const int     Y0   = ty * (RH - 2) - 1;
     const int     gx   = X0 + lane;
     const bool    cin  = gx >= 0 && gx < d.W;
+    const uint64_t clk0 = __builtin_amdgcn_s_memtime();
 
     float cN[RH], cNW[RH], cNE[RH];
     float rW[SW + 2], rNW[SW + 2], rSW[SW + 2];
@@ -447,14 +478,15 @@ namespace dtk
       break;
     }
 
-    float Dc[RH];
+    float     Dc[RH];
+    const int cx = min(max(gx, 0), d.W - 1);
 #pragma unroll
     for (int r = 0; r < RH; ++r)
     {
-      const int gy = Y0 + r;
-      Dc[r]        = DTK_INF;
-      if (cin && gy >= 0 && gy < d.H)
-        Dc[r] = drow<RH>(d, gy)[gx];
+      const int   gy = Y0 + r;
+      const int   cy = min(max(gy, 0), d.H - 1);
+      const float v  = gload<float>(d.D, (long long)cy * d.D_pitch + 4ll * cx);
+      Dc[r]          = (cin && gy >= 0 && gy < d.H) ? v : DTK_INF;
     }
 
     // Initial values of the owned border (to decide which neighbours to wake).
@@ -469,6 +501,7 @@ namespace dtk
     }
 
     const float ea = p.e_axis, ed = p.e_diag;
+    const uint64_t clk1 = __builtin_amdgcn_s_memtime();
     // A visit needs four quiet sweeps to certify its region, so a cap below
     // four could never converge (the tile would re-queue itself forever).
     const int   cap = p.max_sweeps > 0 ? (p.max_sweeps < 4 ? 4 : p.max_sweeps) : 0x7fffffff;
@@ -525,6 +558,7 @@ namespace dtk
         break;
     }
 
+    const uint64_t clk2 = __builtin_amdgcn_s_memtime();
     VisitResult res;
     res.sweeps    = sweeps;
     res.converged = conv;
@@ -567,6 +601,9 @@ namespace dtk
         }
       }
     }
+    res.cyc_setup = (uint32_t)(clk1 - clk0);
+    res.cyc_sweep = (uint32_t)(clk2 - clk1);
+    res.cyc_store = (uint32_t)(__builtin_amdgcn_s_memtime() - clk2);
     return res;
   }
 } // namespace dtk

@@ -31,7 +31,7 @@ namespace dt
 
   // [reference] blocks schedule.  round_per_blocks (device int array of
   // dtk_tile_count(W, H, 0) entries, or nullptr) counts visits per tile of
-  // this library's tiling (62 x 14 owned pixels by default).
+  // this library's tiling (62 x 30 owned pixels by default).
   void generalised_distance_transform_blocks(const image2d_view<std::uint8_t>& img,
                                              const image2d_view<std::uint8_t>& mask, image2d_view<float>& dist,
                                              float lambda, float v = 1e10, int* round_per_blocks = nullptr);

@@ -97,6 +97,12 @@ typedef struct
   int      tiles;      /* tiles of the call */
   int      tile_rows;  /* region rows used */
   int      workers;    /* persistent workers launched (task) */
+  uint32_t cycles_wait;  /* task: summed worker shader-clock cycles / 1024 waiting for a tile */
+  uint32_t cycles_visit; /*       ... solving tiles */
+  uint32_t cycles_pub;   /*       ... publishing results (fence, wake-ups, re-queue) */
+  uint32_t cycles_setup; /*   inside visits: tile loads and edge costs */
+  uint32_t cycles_sweep; /*   inside visits: sweeps and layout changes */
+  uint32_t cycles_store; /*   inside visits: border check and write-back */
 } dtk_stats;
 
 void dtk_params_default(dtk_params* p);
