@@ -73,10 +73,10 @@ namespace dtk
   // Device work queue of dirty tiles (persistent kernel).
   //   slots[CAP]: (seq << 32) | tile; seq = 2*ticket while free for that
   //               ticket, 2*ticket + 1 once filled.
-  //   state[t]  : TILE_QUEUED | TILE_RUNNING | TILE_DIRTY bits of tile t.  A
-  //               tile is in the ring at most once and solved by at most one
-  //               wave at a time; a wake-up while it runs sets TILE_DIRTY and
-  //               the running wave re-queues it when it finishes.
+  //   state[t]  : TileState of tile t.  A tile is in the ring at most once
+  //               and solved by at most one wave at a time; a wake-up while
+  //               it runs sets TILE_DIRTY and the running wave re-queues it
+  //               when it finishes.
   //   ctrl      : counters, each on its own 64-byte line.
   enum CtrlWord : int
   {
@@ -96,9 +96,16 @@ namespace dtk
     CTRL_WORDS   = 128,
   };
 
+  // Tile states (one word per tile, changed only by read-modify-writes so
+  // every waker's D stores are in the release sequence the next visit
+  // acquires from):
+  //   0                  idle
+  //   TILE_DIRTY         queued (in the ring, not yet popped)
+  //   TILE_RUNNING       being solved by one wave
+  //   RUNNING|DIRTY      being solved, woken again meanwhile (re-queued at the end)
+  // A waker does fetch_or(TILE_DIRTY); the one that sees 0 inserts the tile.
   enum TileState : uint32_t
   {
-    TILE_QUEUED  = 1u,
     TILE_RUNNING = 2u,
     TILE_DIRTY   = 4u,
   };

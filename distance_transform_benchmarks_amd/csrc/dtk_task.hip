@@ -152,7 +152,7 @@ namespace dtk
     }
     if (nmin < q.tmax[t])
     {
-      q.state[t] = TILE_QUEUED;
+      q.state[t] = TILE_DIRTY; // queued
       const uint32_t ticket = (uint32_t)__hip_atomic_fetch_add(tp_word(q), kOnePendingOneTicket, __ATOMIC_RELAXED,
                                                                __HIP_MEMORY_SCOPE_AGENT);
       q.slots[ticket & q.cap_mask] = ((unsigned long long)(2u * ticket + 1u) << 32) | (uint32_t)t;
@@ -175,7 +175,7 @@ namespace dtk
         const int t = (int)(uint32_t)s;
         __hip_atomic_store(slot, (unsigned long long)(2u * (ticket + q.cap_mask + 1u)) << 32, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
-        // QUEUED -> RUNNING.  This RMW continues the release sequence of every
+        // queued -> RUNNING.  This RMW continues the release sequence of every
         // earlier state RMW (the wakers' and the previous visit's), so the
         // acquire fence makes their D stores visible to the loads of this visit.
         __hip_atomic_exchange(&q.state[t], (uint32_t)TILE_RUNNING, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -198,11 +198,12 @@ namespace dtk
   }
 
   // Publish a finished visit, wave-wide (called by all 64 lanes after a
-  // release fence): lanes 0..7 wake the neighbours whose halo changed
-  // (idle -> QUEUED, running -> RUNNING|DIRTY), lane 8 moves the tile itself
-  // to idle or, when a wake-up arrived while it ran (or the visit hit its
-  // sweep cap), back to QUEUED.  Tiles that became QUEUED go into the ring
-  // with one pending reservation and one ticket reservation for all of them.
+  // release fence): lanes 0..7 wake the neighbours whose pixels this visit
+  // improved (fetch_or DIRTY: idle -> queued, running -> running+dirty),
+  // lane 8 moves the tile itself to idle or, when a wake-up arrived while it
+  // ran (or the visit hit its sweep cap), back to queued.  Tiles that became
+  // queued go into the ring with one reservation of tickets and pending
+  // count for all of them.
   __device__ void publish(const Queue& q, const ImgDesc& d, int t, int tx, int ty, uint32_t border, bool requeue)
   {
     const int lane   = threadIdx.x;
@@ -218,29 +219,21 @@ namespace dtk
     bool insert = false;
     if (target >= 0)
     {
-      // First attempt assumes the usual state (neighbour idle, own tile
-      // running) and saves a load round trip.
-      uint32_t st = lane == 8 ? (uint32_t)TILE_RUNNING : 0u;
-      for (;;)
+      if (lane == 8)
       {
-        uint32_t n;
-        if (lane == 8)
-          n = (requeue || (st & TILE_DIRTY)) ? (uint32_t)TILE_QUEUED : 0u;
-        else
-        {
-          if (st & TILE_QUEUED)
-            break;
-          n = (st & TILE_RUNNING) ? (st | TILE_DIRTY) : (uint32_t)TILE_QUEUED;
-          if (n == st)
-            break;
-        }
-        if (__hip_atomic_compare_exchange_weak(&q.state[target], &st, n, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT))
-        {
-          insert = n == TILE_QUEUED;
-          break;
-        }
+        // RUNNING -> idle, or -> queued when woken meanwhile (DIRTY stays set)
+        // or when the visit hit its sweep cap.
+        const uint32_t old = __hip_atomic_fetch_and(&q.state[target], ~(uint32_t)TILE_RUNNING, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        if (old & TILE_DIRTY)
+          insert = true;
+        else if (requeue)
+          insert = __hip_atomic_fetch_or(&q.state[target], (uint32_t)TILE_DIRTY, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT) == 0u;
       }
+      else
+        insert = __hip_atomic_fetch_or(&q.state[target], (uint32_t)TILE_DIRTY, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT) == 0u;
     }
     const uint64_t ins  = __ballot(insert);
     const uint32_t n    = (uint32_t)__popcll(ins);

@@ -164,7 +164,7 @@ namespace dtk
 
   struct VisitResult
   {
-    uint32_t border;   // bit k: neighbour k must be revisited (see kNbrDX/kNbrDY)
+    uint32_t border;   // bit k: this visit improved a pixel of neighbour k (see kNbrDX/kNbrDY)
     int      sweeps;   // sweeps run
     bool     converged;
     bool     changed;  // any owned pixel changed
@@ -489,12 +489,14 @@ namespace dtk
       Dc[r]          = (cin && gy >= 0 && gy < d.H) ? v : DTK_INF;
     }
 
-    // Initial values of the owned border (to decide which neighbours to wake).
-    const float init_top = Dc[1];
-    const float init_bot = Dc[RH - 2];
-    if (lane == 1 || lane == 62)
+    // Halo as loaded: a neighbour needs a new visit exactly when one of its
+    // pixels in this region's halo ends up strictly smaller (this visit then
+    // found a shorter path into that neighbour).
+    const float init_top = Dc[0];
+    const float init_bot = Dc[RH - 1];
+    if (lane == 0 || lane == 63)
     {
-      const int o = (lane == 62) ? RH : 0;
+      const int o = (lane == 63) ? RH : 0;
 #pragma unroll
       for (int r = 0; r < RH; ++r)
         Sv[o + r] = Dc[r];
@@ -566,27 +568,27 @@ namespace dtk
     res.border    = 0;
     if (any)
     {
-      bool colc = false;
-      if (lane == 1 || lane == 62)
+      bool colimp = false;
+      if (lane == 0 || lane == 63)
       {
-        const int o = (lane == 62) ? RH : 0;
+        const int o = (lane == 63) ? RH : 0;
 #pragma unroll
         for (int r = 1; r < RH - 1; ++r)
-          colc |= Dc[r] != Sv[o + r];
+          colimp |= lt_nn(Dc[r], Sv[o + r]);
       }
-      const uint64_t own  = 0x7FFFFFFFFFFFFFFEull; // lanes 1..62
-      const uint64_t top  = ballot(Dc[1] != init_top) & own;
-      const uint64_t bot  = ballot(Dc[RH - 2] != init_bot) & own;
-      const uint64_t cols = ballot(colc);
+      const uint64_t top  = ballot(lt_nn(Dc[0], init_top));
+      const uint64_t bot  = ballot(lt_nn(Dc[RH - 1], init_bot));
+      const uint64_t cols = ballot(colimp);
+      const uint64_t mid  = 0x7FFFFFFFFFFFFFFEull; // lanes 1..62
       uint32_t       b    = 0;
-      b |= (cols >> 1) & 1u;                     // L
-      b |= ((cols >> 62) & 1u) << 1;             // R
-      b |= (top != 0 ? 1u : 0u) << 2;            // T
-      b |= (bot != 0 ? 1u : 0u) << 3;            // B
-      b |= (uint32_t)((top >> 1) & 1u) << 4;     // TL
-      b |= (uint32_t)((top >> 62) & 1u) << 5;    // TR
-      b |= (uint32_t)((bot >> 1) & 1u) << 6;     // BL
-      b |= (uint32_t)((bot >> 62) & 1u) << 7;    // BR
+      b |= (uint32_t)(cols & 1u);                     // L : left halo column
+      b |= (uint32_t)((cols >> 63) & 1u) << 1;        // R : right halo column
+      b |= ((top & mid) != 0 ? 1u : 0u) << 2;         // T : top halo row
+      b |= ((bot & mid) != 0 ? 1u : 0u) << 3;         // B : bottom halo row
+      b |= (uint32_t)(top & 1u) << 4;                 // TL: top-left corner
+      b |= (uint32_t)((top >> 63) & 1u) << 5;         // TR
+      b |= (uint32_t)(bot & 1u) << 6;                 // BL
+      b |= (uint32_t)((bot >> 63) & 1u) << 7;         // BR
       res.border = b;
 
       // Write back the owned pixels.
