@@ -1,0 +1,154 @@
+"""The C-ABI boundary on the CPU: the library loads, exports what the header
+declares, lays out its structs as the header does, and rejects invalid
+arguments before touching a device.  No compute call is made here."""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "dtk.h")
+PKG = os.path.join(ROOT, "distance_transform_benchmarks_amd")
+
+
+def _header_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*[A-Za-z_][\w\s\*]*?\b(dtk_\w+)\s*\(", text, flags=re.M)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from distance_transform_benchmarks_amd import _lib
+
+    if not os.path.exists(_lib.LIBDTK):
+        pytest.fail(f"{_lib.LIBDTK} missing: run __graft_entry__.build() first")
+    return _lib.load()
+
+
+def test_header_declares_the_binding_exports():
+    from distance_transform_benchmarks_amd import _lib
+
+    declared = _header_functions()
+    assert len(declared) >= 20
+    assert sorted(_lib.EXPORTS) == declared
+
+
+def test_library_exports_every_declared_symbol(lib):
+    from distance_transform_benchmarks_amd import _lib
+
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIBDTK], capture_output=True, text=True, check=True)
+    exported = set(line.split()[-1] for line in out.stdout.splitlines() if line.strip())
+    missing = [s for s in _header_functions() if s not in exported]
+    assert not missing, missing
+    for s in _header_functions():
+        assert hasattr(lib, s)
+
+
+def test_cpp_api_library_exports_reference_entry_points():
+    so = os.path.join(PKG, "lib", "libdt.so")
+    out = subprocess.run(["nm", "-DC", "--defined-only", so], capture_output=True, text=True, check=True).stdout
+    for name in ("dt::generalised_distance_transform(", "dt::generalised_distance_transform_blocks(",
+                 "dt::generalised_distance_transform_task(", "dt::host_to_device(", "dt::device_to_host("):
+        assert name in out, name
+
+
+def test_struct_layout_matches_header(tmp_path):
+    """sizeof/offsetof of the ABI structs as a C compiler sees include/dtk.h
+    equal the ctypes mirrors the Python binding uses."""
+    from distance_transform_benchmarks_amd._lib import DtkImage, DtkParams, DtkStats
+
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stddef.h>\n#include <stdio.h>\n#include "dtk.h"\n'
+        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu\\n\","
+        "sizeof(dtk_image), offsetof(dtk_image, pitch), offsetof(dtk_image, dtype),"
+        "sizeof(dtk_params), offsetof(dtk_params, round_per_blocks), offsetof(dtk_params, waves_per_cu),"
+        "sizeof(dtk_stats), offsetof(dtk_stats, cycles_store)); return 0;}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [ctypes.sizeof(DtkImage), DtkImage.pitch.offset, DtkImage.dtype.offset, ctypes.sizeof(DtkParams),
+            DtkParams.round_per_blocks.offset, DtkParams.waves_per_cu.offset, ctypes.sizeof(DtkStats),
+            DtkStats.cycles_store.offset]
+    assert got == want
+
+
+def test_params_default_and_version(lib):
+    from distance_transform_benchmarks_amd import _lib
+
+    p = _lib.default_params()
+    assert p.lambda_ == 0.0 and p.v == pytest.approx(1e10) and p.connectivity == 8
+    assert p.schedule == _lib.SCHED_TASK and p.max_iterations == 0 and p.tile_rows == 0
+    assert b"gfx950" in lib.dtk_version()
+
+
+@pytest.mark.parametrize("W,H,rows", [(1, 1, 0), (62, 30, 32), (63, 31, 32), (4096, 4096, 0), (4096, 4096, 16),
+                                      (124, 14, 16), (125, 15, 16)])
+def test_tile_count_closed_form(lib, W, H, rows):
+    rh = rows or 32
+    assert lib.dtk_tile_count(W, H, rows) == -(-W // 62) * -(-H // (rh - 2))
+
+
+def test_tile_count_rejects_bad_rows(lib):
+    assert lib.dtk_tile_count(10, 10, 8) == 0
+    assert lib.dtk_tile_count(0, 10, 0) == 0
+
+
+def _img(w, h, dtype, data=0x1000, pitch=None):
+    from distance_transform_benchmarks_amd._lib import DtkImage
+
+    es = {0: 1, 1: 3, 2: 4}.get(dtype, 1)
+    return DtkImage(data, w, h, pitch if pitch is not None else w * es, dtype)
+
+
+@pytest.mark.parametrize("case,msg", [
+    ("lambda", "lambda"), ("nu", "v must"), ("conn", "connectivity"), ("sched", "schedule"),
+    ("rows", "tile_rows"), ("fixed_task", "fixed iteration"), ("size", "sizes differ"),
+    ("dtype", "unsupported dtype"), ("pitch", "pitch"), ("null", "NULL"), ("align", "aligned"),
+])
+def test_invalid_arguments_rejected_on_host(lib, case, msg):
+    """Every check happens before the first HIP call, so these run without a GPU."""
+    from distance_transform_benchmarks_amd import _lib
+
+    p = _lib.default_params()
+    img, mask, dist = _img(8, 8, 0), _img(8, 8, 0), _img(8, 8, 2)
+    if case == "lambda":
+        p.lambda_ = 1.5
+    elif case == "nu":
+        p.v = float("inf")
+    elif case == "conn":
+        p.connectivity = 6
+    elif case == "sched":
+        p.schedule = 7
+    elif case == "rows":
+        p.tile_rows = 64
+    elif case == "fixed_task":
+        p.max_iterations = 3
+    elif case == "size":
+        mask = _img(8, 9, 0)
+    elif case == "dtype":
+        mask = _img(8, 8, 1)
+    elif case == "pitch":
+        img = _img(8, 8, 0, pitch=4)
+    elif case == "null":
+        img = _img(8, 8, 0, data=0)
+    elif case == "align":
+        dist = _img(8, 8, 2, data=0x1002)
+    st = lib.dtk_ggdt(ctypes.byref(img), ctypes.byref(mask), ctypes.byref(dist), ctypes.byref(p), None, None)
+    assert st == _lib.DTK_ERR_INVALID_ARGUMENT
+    assert msg in lib.dtk_last_error().decode()
+    with pytest.raises(ValueError):
+        _lib.check(st)
+
+
+def test_cpp_suite_cpu_part():
+    """The C++ API test runner (mirrors the reference's gtest cases) without --gpu."""
+    exe = os.path.join(PKG, "lib", "dt_tests")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert " 0 failed" in r.stdout
