@@ -1,0 +1,135 @@
+"""HBM traffic of a kernel from rocprofv3 PMC counters.
+
+Two sub-commands:
+
+  copy       one device-to-device copy of a known size (torch ``copy_`` of a
+             float32 tensor), the calibration workload: its FETCH_SIZE and
+             WRITE_SIZE are compared with the bytes it must move;
+  summarize  reads the rocprofv3 ``counter_collection.csv`` files of a
+             FETCH_SIZE run and a WRITE_SIZE run of the workload and of the
+             copy, scales each counter by the copy's known/measured ratio and
+             writes a JSON summary with the HBM bytes per dispatch of one
+             kernel.
+
+FETCH_SIZE and WRITE_SIZE are reported in KiB per dispatch.  They need
+separate runs (FETCH_SIZE takes three TCC counters, WRITE_SIZE two).
+
+Example (each rocprofv3 line is its own run):
+
+  rocprofv3 --pmc FETCH_SIZE -d out/f -o run --output-format csv -- python3 bench.py --steps 2 --warmup 1 --no-cpu --no-batched
+  rocprofv3 --pmc WRITE_SIZE -d out/w -o run --output-format csv -- python3 bench.py --steps 2 --warmup 1 --no-cpu --no-batched
+  rocprofv3 --pmc FETCH_SIZE -d out/cf -o run --output-format csv -- python3 -m distance_transform_benchmarks_amd.traffic copy
+  rocprofv3 --pmc WRITE_SIZE -d out/cw -o run --output-format csv -- python3 -m distance_transform_benchmarks_amd.traffic copy
+  python3 -m distance_transform_benchmarks_amd.traffic summarize --kernel k_task --fetch out/f --write out/w \\
+      --copy-fetch out/cf --copy-write out/cw --algorithmic-bytes 184549376 --out traffic.json
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import glob
+import json
+import os
+import sys
+
+COPY_BYTES = 1 << 30  # bytes read (and written) by the calibration copy
+
+
+def run_copy(nbytes: int = COPY_BYTES, reps: int = 3) -> None:
+    import torch
+
+    dev = torch.device("cuda:0")
+    n = nbytes // 4
+    src = torch.ones(n, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"copy: {reps} x {nbytes} bytes", flush=True)
+
+
+def _csv_rows(path: str):
+    files = [path] if os.path.isfile(path) else sorted(
+        glob.glob(os.path.join(path, "**", "*counter_collection.csv"), recursive=True))
+    if not files:
+        raise FileNotFoundError(f"no counter_collection.csv under {path}")
+    for f in files:
+        with open(f, newline="") as fh:
+            yield from csv.DictReader(fh)
+
+
+def per_dispatch(path: str, counter: str, kernel_substr: str):
+    """Counter value (KiB) of every dispatch whose kernel name contains kernel_substr."""
+    vals = {}
+    for r in _csv_rows(path):
+        name = r.get("Kernel_Name") or r.get("Kernel-Name") or ""
+        if kernel_substr not in name:
+            continue
+        if (r.get("Counter_Name") or "").strip() != counter:
+            continue
+        key = r.get("Dispatch_Id") or r.get("Correlation_Id") or str(len(vals))
+        vals[key] = vals.get(key, 0.0) + float(r["Counter_Value"])
+    return list(vals.values())
+
+
+def _mean(v):
+    return sum(v) / len(v) if v else float("nan")
+
+
+def summarize(a) -> dict:
+    kib = 1024.0
+    f = per_dispatch(a.fetch, "FETCH_SIZE", a.kernel)
+    w = per_dispatch(a.write, "WRITE_SIZE", a.kernel)
+    cf = per_dispatch(a.copy_fetch, "FETCH_SIZE", a.copy_kernel)
+    cw = per_dispatch(a.copy_write, "WRITE_SIZE", a.copy_kernel)
+    if not (f and w and cf and cw):
+        raise SystemExit(f"missing dispatches: {len(f)} {len(w)} {len(cf)} {len(cw)}")
+    # The calibration copy moves exactly COPY_BYTES each way; take the largest
+    # dispatches (the copy itself, not small fills).
+    cf_b, cw_b = max(cf) * kib, max(cw) * kib
+    kf, kw = a.copy_bytes / cf_b, a.copy_bytes / cw_b
+    fetch_b, write_b = _mean(f) * kib * kf, _mean(w) * kib * kw
+    out = {
+        "kernel": a.kernel, "dispatches_fetch": len(f), "dispatches_write": len(w),
+        "fetch_bytes_per_launch": round(fetch_b), "write_bytes_per_launch": round(write_b),
+        "bytes_per_launch": round(fetch_b + write_b),
+        "raw_fetch_kib_mean": round(_mean(f), 1), "raw_write_kib_mean": round(_mean(w), 1),
+        "copy_calibration": {"bytes": a.copy_bytes, "fetch_kib": max(cf), "write_kib": max(cw),
+                             "fetch_scale": round(kf, 4), "write_scale": round(kw, 4)},
+    }
+    if a.algorithmic_bytes:
+        out["algorithmic_bytes_per_launch"] = a.algorithmic_bytes
+        out["traffic_over_algorithmic"] = round((fetch_b + write_b) / a.algorithmic_bytes, 3)
+    return out
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    c = sub.add_parser("copy")
+    c.add_argument("--bytes", type=int, default=COPY_BYTES)
+    s = sub.add_parser("summarize")
+    s.add_argument("--kernel", default="k_task")
+    s.add_argument("--copy-kernel", default="")
+    s.add_argument("--fetch", required=True)
+    s.add_argument("--write", required=True)
+    s.add_argument("--copy-fetch", required=True)
+    s.add_argument("--copy-write", required=True)
+    s.add_argument("--copy-bytes", type=int, default=COPY_BYTES)
+    s.add_argument("--algorithmic-bytes", type=int, default=0)
+    s.add_argument("--out", default="")
+    a = ap.parse_args(argv)
+    if a.cmd == "copy":
+        run_copy(a.bytes)
+        return 0
+    res = summarize(a)
+    txt = json.dumps(res, indent=1)
+    if a.out:
+        with open(a.out, "w") as fh:
+            fh.write(txt + "\n")
+    print(txt)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
