@@ -235,6 +235,7 @@ namespace
     k.zero_lambda = p->lambda == 0.f;
     k.max_sweeps  = p->max_sweeps;
     k.rh          = rh;
+    k.c8          = p->connectivity == DTK_C8;
     return k;
   }
 
@@ -315,6 +316,25 @@ namespace
     if (out)
       *out = b;
     return c.off + 256;
+  }
+
+  // Cached edge-cost records of the tile schedules: one pitched plane per
+  // image (16 B/px for C8, 8 B/px for C4), written by k_init and read by
+  // every tile visit.  With base == NULL only the size is computed.
+  bool use_cost_cache(const dtk_params* p) { return p->lambda != 0.f && !p->recompute_costs; }
+
+  size_t place_costs(ImgDesc* descs, int n, bool c8, char* base)
+  {
+    size_t off = 0;
+    for (int i = 0; i < n; ++i)
+    {
+      ImgDesc& d    = descs[i];
+      d.cost_pitch  = (long long)align_up((size_t)d.W * (c8 ? 16 : 8), 256);
+      d.cost        = base ? reinterpret_cast<unsigned char*>(base + off) : nullptr;
+      off += (size_t)d.cost_pitch * (size_t)(d.H > 0 ? d.H : 0);
+      off = align_up(off, 256);
+    }
+    return off;
   }
 
   dtk_status fill_stats(dtk_stats* st, const uint32_t* ctrl_host, int rounds, int tiles, int rh, int workers)
@@ -489,11 +509,14 @@ dtk_status dtk_ggdt(const dtk_image* img, const dtk_image* mask, dtk_image* dist
 
   if (p->schedule == DTK_SCHED_BLOCKS)
   {
-    TaskBufs tb;
-    size_t   need = task_layout(nullptr, ntiles, 1, 1, nullptr) + 2 * align_up(ntiles, 256) + 512;
-    if ((st = ws_reserve(*w, need, 4096)) != DTK_OK)
+    TaskBufs     tb;
+    const size_t head   = align_up(task_layout(nullptr, ntiles, 1, 1, nullptr) + 2 * align_up(ntiles, 256) + 512, 256);
+    const size_t cbytes = use_cost_cache(p) ? place_costs(&b.d0, 1, c8, nullptr) : 0;
+    if ((st = ws_reserve(*w, head + cbytes, 4096)) != DTK_OK)
       return st;
     task_layout(w->dbuf, ntiles, 1, 1, &tb);
+    if (cbytes)
+      place_costs(&b.d0, 1, c8, static_cast<char*>(w->dbuf) + head);
     Queue q     = tb.q;
     q.slots     = nullptr; // blocks schedule: no queue
     uint8_t* a0 = static_cast<uint8_t*>(w->dbuf) + align_up(task_layout(nullptr, ntiles, 1, 1, nullptr), 256);
@@ -526,9 +549,13 @@ dtk_status dtk_ggdt(const dtk_image* img, const dtk_image* mask, dtk_image* dist
   const int      workers = task_workers(*w, p, S, c8, two, ntiles);
   const uint32_t cap     = next_pow2((uint32_t)(ntiles + workers + 1));
   TaskBufs       tb;
-  if ((st = ws_reserve(*w, task_layout(nullptr, ntiles, cap, 1, nullptr), 4096)) != DTK_OK)
+  const size_t   head   = align_up(task_layout(nullptr, ntiles, cap, 1, nullptr), 256);
+  const size_t   cbytes = use_cost_cache(p) ? place_costs(&b.d0, 1, c8, nullptr) : 0;
+  if ((st = ws_reserve(*w, head + cbytes, 4096)) != DTK_OK)
     return st;
   task_layout(w->dbuf, ntiles, cap, 1, &tb);
+  if (cbytes)
+    place_costs(&b.d0, 1, c8, static_cast<char*>(w->dbuf) + head);
   if ((st = enqueue_task(b, kp, tb.q, S, c8, two, workers, s)) != DTK_OK)
     return st;
   if (stats)
@@ -572,10 +599,14 @@ dtk_status dtk_ggdt_batched(int n, const dtk_image* imgs, const dtk_image* masks
   const int      workers = task_workers(*w, p, S, c8, two, ntiles);
   const uint32_t cap     = next_pow2((uint32_t)(ntiles + workers + 1));
   const size_t   hbytes  = descs.size() * sizeof(ImgDesc) + tile_img.size() * 4 + 1024;
-  if ((st = ws_reserve(*w, task_layout(nullptr, ntiles, cap, n > 1 ? n : 2, nullptr), hbytes)) != DTK_OK)
+  const size_t   head    = align_up(task_layout(nullptr, ntiles, cap, n > 1 ? n : 2, nullptr), 256);
+  const size_t   cbytes  = use_cost_cache(p) ? place_costs(descs.data(), n, c8, nullptr) : 0;
+  if ((st = ws_reserve(*w, head + cbytes, hbytes)) != DTK_OK)
     return st;
   TaskBufs tb;
   task_layout(w->dbuf, ntiles, cap, n > 1 ? n : 2, &tb);
+  if (cbytes)
+    place_costs(descs.data(), n, c8, static_cast<char*>(w->dbuf) + head);
   Batch b{};
   b.d0     = descs[0];
   b.nimg   = n;
@@ -650,8 +681,9 @@ dtk_status dtk_batch_plan_create(int n, const dtk_image* imgs, const dtk_image* 
   dtk_batch_plan* pl     = new dtk_batch_plan();
   pl->device             = dev;
   pl->ntiles             = ntiles;
-  const size_t bytes     = task_layout(nullptr, ntiles, cap, n > 1 ? n : 2, nullptr);
-  hipError_t   e         = hipMalloc(&pl->dbuf, bytes);
+  const size_t head      = align_up(task_layout(nullptr, ntiles, cap, n > 1 ? n : 2, nullptr), 256);
+  const size_t cbytes    = use_cost_cache(p) ? place_costs(descs.data(), n, c8, nullptr) : 0;
+  hipError_t   e         = hipMalloc(&pl->dbuf, head + cbytes);
   if (e != hipSuccess)
   {
     delete pl;
@@ -659,6 +691,8 @@ dtk_status dtk_batch_plan_create(int n, const dtk_image* imgs, const dtk_image* 
   }
   TaskBufs tb;
   task_layout(pl->dbuf, ntiles, cap, n > 1 ? n : 2, &tb);
+  if (cbytes)
+    place_costs(descs.data(), n, c8, static_cast<char*>(pl->dbuf) + head);
   Batch b{};
   b.d0     = descs[0];
   b.nimg   = n;

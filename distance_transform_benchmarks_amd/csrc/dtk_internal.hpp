@@ -36,6 +36,11 @@ namespace dtk
     long long            D_pitch;
     unsigned char*       pred;
     long long            pred_pitch;
+    // Cached edge costs (tile schedules, lambda > 0): per pixel the costs of
+    // its W, N, NW, NE edges (C8, 16 B) or W, N (C4, 8 B); NULL = recompute
+    // them from the image on every tile visit.
+    unsigned char*       cost;
+    long long            cost_pitch;
     int                  W, H;
     int                  tiles_x, tiles_y;
     int                  tile0; // first global tile id of this image
@@ -68,6 +73,7 @@ namespace dtk
     int   zero_lambda;
     int   max_sweeps; // sweep cap per tile visit (0 = until stable)
     int   rh;         // region rows (64 / S)
+    int   c8;         // 8-connectivity (layout of the cached cost records)
   };
 
   // Device work queue of dirty tiles (persistent kernel).

@@ -103,6 +103,31 @@ namespace dtk
         mx = fmaxf(mx, d0);
       }
     }
+    // Cached edge costs of the owned pixels (read back by every visit).
+    if (d.cost)
+    {
+      switch (p.img_kind * 2 + (p.c8 ? 1 : 0))
+      {
+      case IMG_U8 * 2:
+        write_cost_rows<IMG_U8, false>(d, p, X0, Y0, RH, lane);
+        break;
+      case IMG_U8 * 2 + 1:
+        write_cost_rows<IMG_U8, true>(d, p, X0, Y0, RH, lane);
+        break;
+      case IMG_RGB8 * 2:
+        write_cost_rows<IMG_RGB8, false>(d, p, X0, Y0, RH, lane);
+        break;
+      case IMG_RGB8 * 2 + 1:
+        write_cost_rows<IMG_RGB8, true>(d, p, X0, Y0, RH, lane);
+        break;
+      case IMG_F32 * 2:
+        write_cost_rows<IMG_F32, false>(d, p, X0, Y0, RH, lane);
+        break;
+      default:
+        write_cost_rows<IMG_F32, true>(d, p, X0, Y0, RH, lane);
+        break;
+      }
+    }
     mn = wave_min(mn);
     mx = wave_max(mx);
     if (err)

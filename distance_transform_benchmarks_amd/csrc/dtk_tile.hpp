@@ -447,6 +447,202 @@ namespace dtk
     edge_costs<KIND, RH, E_N>(d, p, X0, Y0, lane, I, cN);
   }
 
+  // ---- cached edge costs ------------------------------------------------------
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  typedef __attribute__((address_space(1))) unsigned char* gbytes_w;
+
+  // Writes the cost record of the pixels of one wave-wide row segment:
+  // column gx = X0 + lane for lanes 1..62 (lanes 0 and 63 only supply
+  // neighbour values), rows Y0+1 .. Y0+RH-2.  Record = {W, N, NW, NE} (C8)
+  // or {W, N} (C4), each the cost of the edge to that neighbour as
+  // edge_cost() defines it, +inf when the neighbour is outside the image.
+  // Edge costs are symmetric (|a - b| and the squared colour distance are),
+  // so the solver reads the cost of (p, q) from either end.
+  template <int KIND, bool C8>
+  __device__ __forceinline__ void write_cost_rows(const ImgDesc& d, const KParams& p, int X0, int Y0, int RH,
+                                                  int lane)
+  {
+    const int  gx  = X0 + lane;
+    const int  cx  = min(max(gx, 0), d.W - 1);
+    const bool own = lane >= 1 && lane <= kTileW && gx < d.W;
+    const bool inL = gx - 1 >= 0;
+    const bool inR = gx + 1 < d.W;
+    uint32_t   up  = load_pix<KIND>(d.img + (long long)min(max(Y0, 0), d.H - 1) * d.img_pitch, cx);
+    for (int r = 1; r < RH - 1; ++r)
+    {
+      const int      gy   = Y0 + r;
+      const int      cy   = min(max(gy, 0), d.H - 1);
+      const uint32_t cur  = load_pix<KIND>(d.img + (long long)cy * d.img_pitch, cx);
+      const bool     rinU = gy - 1 >= 0;
+      const float    cw   = edge_cost<KIND>(inL, cur, shr_u(cur), 0.f, p);
+      const float    cn   = edge_cost<KIND>(rinU, cur, up, 0.f, p);
+      if constexpr (C8)
+      {
+        const float cnw = edge_cost<KIND>(rinU && inL, cur, shr_u(up), 0.f, p);
+        const float cne = edge_cost<KIND>(rinU && inR, cur, shl_u(up), 0.f, p);
+        if (own && gy < d.H)
+        {
+          f4 v = {cw, cn, cnw, cne};
+          *reinterpret_cast<__attribute__((address_space(1))) f4*>((gbytes_w)d.cost + (long long)gy * d.cost_pitch +
+                                                                   16ll * gx) = v;
+        }
+      }
+      else if (own && gy < d.H)
+      {
+        f2 v = {cw, cn};
+        *reinterpret_cast<__attribute__((address_space(1))) f2*>((gbytes_w)d.cost + (long long)gy * d.cost_pitch +
+                                                                 8ll * gx) = v;
+      }
+      up = cur;
+    }
+  }
+
+  // lambda == 0: every edge costs its Euclidean step (e_axis / e_diag), +inf
+  // where an end lies outside the image or the region; no image access.
+  template <int S, bool C8>
+  __device__ __forceinline__ void setup_costs_const(const ImgDesc& d, const KParams& p, int X0, int Y0, int lane,
+                                                    float* T, float (&cN)[64 / S], float (&cNW)[64 / S],
+                                                    float (&cNE)[64 / S], float (&rW)[64 / S + 2],
+                                                    float (&rNW)[64 / S + 2], float (&rSW)[64 / S + 2])
+  {
+    constexpr int RH  = 64 / S, SW = 64 / S;
+    const int     gx  = X0 + lane;
+    const bool    cin = gx >= 0 && gx < d.W;
+    const bool    inL = lane >= 1 && gx - 1 >= 0 && gx - 1 < d.W;
+    const bool    inR = lane <= 62 && gx + 1 >= 0 && gx + 1 < d.W;
+    const float   ea = p.e_axis, ed = p.e_diag;
+    {
+      float tmp[RH];
+#pragma unroll
+      for (int r = 0; r < RH; ++r)
+      {
+        const int  gy   = Y0 + r;
+        const bool here = cin && gy >= 0 && gy < d.H;
+        tmp[r]          = (here && inL) ? ea : DTK_INF;
+      }
+      col_to_row<S>(tmp, rW, T, lane);
+    }
+#pragma unroll
+    for (int r = 0; r < RH; ++r)
+    {
+      const int  gy   = Y0 + r;
+      const bool here = cin && gy >= 0 && gy < d.H;
+      const bool rinU = r >= 1 && gy - 1 >= 0 && gy - 1 < d.H;
+      cN[r]           = (here && rinU) ? ea : DTK_INF;
+      cNW[r]          = (C8 && here && rinU && inL) ? ed : DTK_INF;
+      cNE[r]          = (C8 && here && rinU && inR) ? ed : DTK_INF;
+    }
+    if constexpr (C8)
+    {
+      col_to_row<S>(cNW, rNW, T, lane);
+      float tmp[RH];
+#pragma unroll
+      for (int r = 0; r < RH; ++r)
+      {
+        const int  gy   = Y0 + r;
+        const bool here = cin && gy >= 0 && gy < d.H;
+        const bool rinD = r <= RH - 2 && gy + 1 >= 0 && gy + 1 < d.H;
+        tmp[r]          = (here && inL && rinD) ? ed : DTK_INF;
+      }
+      col_to_row<S>(tmp, rSW, T, lane);
+    }
+    else
+    {
+#pragma unroll
+      for (int k = 0; k < SW + 2; ++k)
+        rNW[k] = rSW[k] = DTK_INF;
+    }
+  }
+
+  // lambda > 0 with cached costs: one 16-byte (C8) or 8-byte (C4) load per
+  // row and lane, then region masking.  The SW edge of (i, r) is the NE edge
+  // of (i - 1, r + 1), moved one lane right.
+  template <int S, bool C8>
+  __device__ __forceinline__ void setup_costs_cached(const ImgDesc& d, int X0, int Y0, int lane, float* T,
+                                                     float (&cN)[64 / S], float (&cNW)[64 / S], float (&cNE)[64 / S],
+                                                     float (&rW)[64 / S + 2], float (&rNW)[64 / S + 2],
+                                                     float (&rSW)[64 / S + 2])
+  {
+    constexpr int RH  = 64 / S, SW = 64 / S;
+    const int     gx  = X0 + lane;
+    const bool    cin = gx >= 0 && gx < d.W;
+    const int     cx  = min(max(gx, 0), d.W - 1);
+    if constexpr (C8)
+    {
+      f4 C[RH];
+#pragma unroll
+      for (int r = 0; r < RH; ++r)
+      {
+        const int cy = min(max(Y0 + r, 0), d.H - 1);
+        C[r]         = gload<f4>(d.cost, (long long)cy * d.cost_pitch + 16ll * cx);
+      }
+      bool here[RH];
+#pragma unroll
+      for (int r = 0; r < RH; ++r)
+        here[r] = cin && Y0 + r >= 0 && Y0 + r < d.H;
+      {
+        float tmp[RH];
+#pragma unroll
+        for (int r = 0; r < RH; ++r)
+          tmp[r] = (here[r] && lane >= 1) ? C[r].x : DTK_INF;
+        col_to_row<S>(tmp, rW, T, lane);
+      }
+#pragma unroll
+      for (int r = 0; r < RH; ++r)
+        cNW[r] = (here[r] && r >= 1 && lane >= 1) ? C[r].z : DTK_INF;
+      col_to_row<S>(cNW, rNW, T, lane);
+      {
+        float tmp[RH];
+#pragma unroll
+        for (int r = 0; r < RH - 1; ++r)
+        {
+          const float ne = here[r + 1] ? C[r + 1].w : DTK_INF;
+          const float sw = shr_z(ne);
+          tmp[r]         = (here[r] && lane >= 1) ? sw : DTK_INF;
+        }
+        tmp[RH - 1] = DTK_INF;
+        col_to_row<S>(tmp, rSW, T, lane);
+      }
+#pragma unroll
+      for (int r = 0; r < RH; ++r)
+      {
+        cNE[r] = (here[r] && r >= 1 && lane <= 62) ? C[r].w : DTK_INF;
+        cN[r]  = (here[r] && r >= 1) ? C[r].y : DTK_INF;
+      }
+    }
+    else
+    {
+      f2 C[RH];
+#pragma unroll
+      for (int r = 0; r < RH; ++r)
+      {
+        const int cy = min(max(Y0 + r, 0), d.H - 1);
+        C[r]         = gload<f2>(d.cost, (long long)cy * d.cost_pitch + 8ll * cx);
+      }
+      {
+        float tmp[RH];
+#pragma unroll
+        for (int r = 0; r < RH; ++r)
+        {
+          const bool here = cin && Y0 + r >= 0 && Y0 + r < d.H;
+          tmp[r]          = (here && lane >= 1) ? C[r].x : DTK_INF;
+        }
+        col_to_row<S>(tmp, rW, T, lane);
+      }
+#pragma unroll
+      for (int r = 0; r < RH; ++r)
+      {
+        const bool here = cin && Y0 + r >= 0 && Y0 + r < d.H;
+        cN[r]           = (here && r >= 1) ? C[r].y : DTK_INF;
+        cNW[r] = cNE[r] = DTK_INF;
+      }
+#pragma unroll
+      for (int k = 0; k < SW + 2; ++k)
+        rNW[k] = rSW[k] = DTK_INF;
+    }
+  }
+
   // ---- one visit of one tile --------------------------------------------------
   // T: LDS RH*65 floats; Sv: LDS 2*RH floats.  Loads the region, iterates to
   // its local fixed point (or max_sweeps), writes back the owned pixels and
@@ -465,6 +661,11 @@ namespace dtk
 
     float cN[RH], cNW[RH], cNE[RH];
     float rW[SW + 2], rNW[SW + 2], rSW[SW + 2];
+    if (p.zero_lambda)
+      setup_costs_const<S, C8>(d, p, X0, Y0, lane, T, cN, cNW, cNE, rW, rNW, rSW);
+    else if (d.cost)
+      setup_costs_cached<S, C8>(d, X0, Y0, lane, T, cN, cNW, cNE, rW, rNW, rSW);
+    else
     switch (p.img_kind)
     {
     case IMG_U8:

@@ -85,6 +85,8 @@ typedef struct
   int   tile_rows;      /* tile solver region rows: 32 or 16 (0 = autotuned/default) */
   int   max_sweeps;     /* sweep cap per tile visit (0 = until the tile is stable) */
   int   waves_per_cu;   /* persistent workers per CU (0 = occupancy limit) */
+  int   recompute_costs; /* tile schedules, lambda > 0: 0 = compute edge costs once per call into a
+                            workspace plane (16 B/px C8, 8 B/px C4), 1 = recompute them on every visit */
   int*  round_per_blocks; /* optional device int[dtk_tile_count()]: visits per tile (blocks schedule) */
 } dtk_params;
 
