@@ -48,7 +48,8 @@ class DtkStats(ctypes.Structure):
     _fields_ = [("rounds", ctypes.c_int), ("visits", ctypes.c_uint32), ("sweeps", ctypes.c_uint32),
                 ("tiles", ctypes.c_int), ("tile_rows", ctypes.c_int), ("workers", ctypes.c_int),
                 ("cycles_wait", ctypes.c_uint32), ("cycles_visit", ctypes.c_uint32), ("cycles_pub", ctypes.c_uint32),
-                ("cycles_setup", ctypes.c_uint32), ("cycles_sweep", ctypes.c_uint32), ("cycles_store", ctypes.c_uint32)]
+                ("cycles_setup", ctypes.c_uint32), ("cycles_sweep", ctypes.c_uint32), ("cycles_store", ctypes.c_uint32),
+                ("cycles_release", ctypes.c_uint32), ("cycles_acquire", ctypes.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -305,8 +305,8 @@ namespace
     b.q.ctrl     = c.take<uint32_t>(CTRL_WORDS);
     b.q.tmin     = c.take<float>(ntiles);
     b.q.tmax     = c.take<float>(ntiles);
-    b.q.state    = c.take<uint32_t>(ntiles);
-    b.q.slots    = c.take<unsigned long long>(cap);
+    b.q.state    = c.take<uint32_t>((size_t)ntiles * kStateStride);
+    b.q.slots    = c.take<unsigned long long>((size_t)cap * kSlotStride);
     b.q.cap_mask = cap - 1;
     if (nimg > 1)
     {
@@ -353,6 +353,8 @@ namespace
     st->cycles_setup = ctrl_host ? ctrl_host[CTRL_T_SETUP] : 0;
     st->cycles_sweep = ctrl_host ? ctrl_host[CTRL_T_SWEEP] : 0;
     st->cycles_store = ctrl_host ? ctrl_host[CTRL_T_STORE] : 0;
+    st->cycles_release = ctrl_host ? ctrl_host[CTRL_T_REL] : 0;
+    st->cycles_acquire = ctrl_host ? ctrl_host[CTRL_T_ACQ] : 0;
     return DTK_OK;
   }
 

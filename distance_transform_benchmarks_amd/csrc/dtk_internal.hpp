@@ -89,6 +89,7 @@ namespace dtk
     CTRL_HEAD    = 0,
     CTRL_TP      = 16, // 64-bit: low 32 bits = tail (tickets issued to pushers),
                        //         high 32 bits = pending (tiles queued or running)
+    CTRL_DONE    = 32, // set once pending reaches 0 (idle workers poll it, not CTRL_TP)
     CTRL_ERROR   = 48,
     CTRL_VISITS  = 64,
     CTRL_SWEEPS  = 80,
@@ -98,7 +99,9 @@ namespace dtk
     CTRL_T_PUB   = 102, //   publishing (release, wake-ups, re-queue)
     CTRL_T_SETUP = 103, //   inside visits: loads + edge costs,
     CTRL_T_SWEEP = 104, //   sweeps,
-    CTRL_T_STORE = 105, //   border check + write-back
+    CTRL_T_STORE = 105, //   border check + write-back,
+    CTRL_T_REL   = 106, //   inside publishing: the release fence,
+    CTRL_T_ACQ   = 107, //   inside waiting: claiming a popped tile + acquire fence
     CTRL_WORDS   = 128,
   };
 
@@ -137,4 +140,26 @@ namespace dtk
   // progress flags).  Legitimate waits end long before; an expired wait
   // flags ERR_WATCHDOG and the kernel finishes instead of hanging.
   constexpr uint32_t kSpinLimit = 1u << 26;
+
+  // Ring slots are kSlotStride 8-byte words apart, so the slot an idle wave
+  // polls shares its cache line with no other slot.
+#ifndef DTK_SLOT_STRIDE
+#define DTK_SLOT_STRIDE 16
+#endif
+  constexpr int kSlotStride = DTK_SLOT_STRIDE;
+
+// Tile state words are kStateStride 4-byte words apart.
+#ifndef DTK_STATE_STRIDE
+#define DTK_STATE_STRIDE 1
+#endif
+  constexpr int kStateStride = DTK_STATE_STRIDE;
+
+#ifndef DTK_POLL_NAPS
+#define DTK_POLL_NAPS 2
+#endif
+  constexpr uint32_t kPollNaps = DTK_POLL_NAPS;
+
+  // Longest wait of an idle persistent worker for a queued tile, in ticks of
+  // the 100 MHz s_memrealtime clock (30 s).
+  constexpr uint64_t kIdleLimitTicks = 30ull * 100000000ull;
 } // namespace dtk

@@ -142,9 +142,9 @@ namespace dtk
     {
       const uint32_t cap = q.cap_mask + 1u;
       for (uint32_t k = (uint32_t)t * 64u + lane; k < cap; k += gridDim.x * 64u)
-        q.slots[k] = (unsigned long long)(2u * k) << 32;
+        q.slots[(size_t)k * kSlotStride] = (unsigned long long)(2u * k) << 32;
       if (lane == 0)
-        q.state[t] = 0;
+        q.state[(size_t)t * kStateStride] = 0;
     }
   }
 
@@ -177,46 +177,64 @@ namespace dtk
     }
     if (nmin < q.tmax[t])
     {
-      q.state[t] = TILE_DIRTY; // queued
+      q.state[(size_t)t * kStateStride] = TILE_DIRTY; // queued
       const uint32_t ticket = (uint32_t)__hip_atomic_fetch_add(tp_word(q), kOnePendingOneTicket, __ATOMIC_RELAXED,
                                                                __HIP_MEMORY_SCOPE_AGENT);
-      q.slots[ticket & q.cap_mask] = ((unsigned long long)(2u * ticket + 1u) << 32) | (uint32_t)t;
+      q.slots[(size_t)(ticket & q.cap_mask) * kSlotStride] = ((unsigned long long)(2u * ticket + 1u) << 32) | (uint32_t)t;
     }
   }
 
   // ---- queue operations (lane 0 of a wave) --------------------------------
   // Pop: take a ticket, wait until its slot is filled or no work is pending.
   // The popped tile goes from QUEUED to RUNNING.
-  __device__ int queue_pop(const Queue& q)
+  __device__ int queue_pop(const Queue& q, uint64_t& t_acq)
   {
     const uint32_t      ticket = __hip_atomic_fetch_add(&q.ctrl[CTRL_HEAD], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned long long* slot   = &q.slots[ticket & q.cap_mask];
+    unsigned long long* slot   = &q.slots[(size_t)(ticket & q.cap_mask) * kSlotStride];
     const uint32_t      want   = 2u * ticket + 1u;
+    uint64_t            t_start = 0;
     for (uint32_t spin = 0;; ++spin)
     {
       const unsigned long long s = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if ((uint32_t)(s >> 32) == want)
       {
-        const int t = (int)(uint32_t)s;
+        const int      t  = (int)(uint32_t)s;
+        const uint64_t a0 = __builtin_amdgcn_s_memtime();
         __hip_atomic_store(slot, (unsigned long long)(2u * (ticket + q.cap_mask + 1u)) << 32, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
         // queued -> RUNNING.  This RMW continues the release sequence of every
         // earlier state RMW (the wakers' and the previous visit's), so the
         // acquire fence makes their D stores visible to the loads of this visit.
-        __hip_atomic_exchange(&q.state[t], (uint32_t)TILE_RUNNING, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_exchange(&q.state[(size_t)t * kStateStride], (uint32_t)TILE_RUNNING, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        t_acq += __builtin_amdgcn_s_memtime() - a0;
         return t;
       }
-      if (pending_now(q) == 0u)
+      // Termination: the wave whose publish took pending to 0 raised DONE.
+      // It is polled on its own line, every 8th poll, so idle waves do not
+      // contend with the publishers' atomics on CTRL_TP.
+      if ((spin & 7u) == 7u &&
+          __hip_atomic_load(&q.ctrl[CTRL_DONE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
         return -1;
-      if (spin >= (kSpinLimit >> 2))
+      // Watchdog on the constant 100 MHz clock: an idle wave legitimately
+      // waits as long as the whole solve runs, so only a wait far beyond any
+      // real solve (kIdleLimitTicks) is taken as a hang.
+      if ((spin & 63u) == 0u)
+      {
+        const uint64_t now = __builtin_amdgcn_s_memrealtime();
+        if (spin == 0u)
+          t_start = now;
+        else if (now - t_start > kIdleLimitTicks)
+          spin = kSpinLimit;
+      }
+      if (spin >= kSpinLimit)
       {
         __hip_atomic_fetch_or(&q.ctrl[CTRL_ERROR], (uint32_t)ERR_WATCHDOG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return -1;
       }
-      // Exponential backoff (up to ~16 x 512 cycles) keeps idle workers from
-      // flooding the L2 with polls while others work.
-      const uint32_t naps = spin < 4 ? 1u : (spin < 64 ? 4u : 16u);
+      // Short backoff (at most kPollNaps x 512 cycles between polls of the own
+      // slot): a tile queued on a serial chain is picked up within about 0.5 us.
+      const uint32_t naps = spin < 16 ? 1u : kPollNaps;
       for (uint32_t i = 0; i < naps; ++i)
         __builtin_amdgcn_s_sleep(8);
     }
@@ -248,33 +266,39 @@ namespace dtk
       {
         // RUNNING -> idle, or -> queued when woken meanwhile (DIRTY stays set)
         // or when the visit hit its sweep cap.
-        const uint32_t old = __hip_atomic_fetch_and(&q.state[target], ~(uint32_t)TILE_RUNNING, __ATOMIC_RELAXED,
+        const uint32_t old = __hip_atomic_fetch_and(&q.state[(size_t)target * kStateStride], ~(uint32_t)TILE_RUNNING, __ATOMIC_RELAXED,
                                                     __HIP_MEMORY_SCOPE_AGENT);
         if (old & TILE_DIRTY)
           insert = true;
         else if (requeue)
-          insert = __hip_atomic_fetch_or(&q.state[target], (uint32_t)TILE_DIRTY, __ATOMIC_RELAXED,
+          insert = __hip_atomic_fetch_or(&q.state[(size_t)target * kStateStride], (uint32_t)TILE_DIRTY, __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_AGENT) == 0u;
       }
       else
-        insert = __hip_atomic_fetch_or(&q.state[target], (uint32_t)TILE_DIRTY, __ATOMIC_RELAXED,
+        insert = __hip_atomic_fetch_or(&q.state[(size_t)target * kStateStride], (uint32_t)TILE_DIRTY, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT) == 0u;
     }
     const uint64_t ins  = __ballot(insert);
     const uint32_t n    = (uint32_t)__popcll(ins);
     uint32_t       base = 0;
-    if (lane == 0 && n)
+    if (lane == 0)
     {
-      // One atomic: n tickets and n more pending items.  The items become
-      // poppable only after this (their slots are written afterwards).
-      base = (uint32_t)__hip_atomic_fetch_add(tp_word(q), (unsigned long long)n * kOnePendingOneTicket,
-                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // One atomic: n tickets, n more pending items and this visit's item
+      // retired.  Pending stays >= n until the new items are popped and
+      // finished, and they become poppable only after their slots are
+      // written below.  The wave that takes pending to 0 raises DONE.
+      const unsigned long long old = __hip_atomic_fetch_add(
+        tp_word(q), (unsigned long long)n * kOnePendingOneTicket + kMinusOnePending, __ATOMIC_RELAXED,
+        __HIP_MEMORY_SCOPE_AGENT);
+      base = (uint32_t)old;
+      if (n == 0u && (uint32_t)(old >> 32) == 1u)
+        __hip_atomic_store(&q.ctrl[CTRL_DONE], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     base = __shfl(base, 0);
     if (insert)
     {
       const uint32_t      ticket   = base + (uint32_t)__popcll(ins & ((1ull << lane) - 1ull));
-      unsigned long long* slot     = &q.slots[ticket & q.cap_mask];
+      unsigned long long* slot     = &q.slots[(size_t)(ticket & q.cap_mask) * kSlotStride];
       const uint32_t      free_seq = 2u * ticket;
       const unsigned long long full = ((unsigned long long)(free_seq + 1u) << 32) | (uint32_t)target;
       for (uint32_t spin = 0;; ++spin)
@@ -292,8 +316,6 @@ namespace dtk
         __builtin_amdgcn_s_sleep(1);
       }
     }
-    if (lane == 0)
-      __hip_atomic_fetch_add(tp_word(q), kMinusOnePending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 
   template <int S, bool C8, bool TWO>
@@ -304,11 +326,15 @@ namespace dtk
     __shared__ int   s_tile;
     uint32_t         visits = 0, sweeps = 0;
     uint64_t         t_wait = 0, t_visit = 0, t_pub = 0, t_setup = 0, t_sweep = 0, t_store = 0;
+    uint64_t         t_rel = 0, t_acq = 0;
+    // Nothing seeded: no publish will ever raise DONE.
+    if (pending_now(q) == 0u)
+      return;
     for (;;)
     {
       const uint64_t c0 = __builtin_amdgcn_s_memtime();
       if (threadIdx.x == 0)
-        s_tile = queue_pop(q);
+        s_tile = queue_pop(q, t_acq);
       __syncthreads();
       const int t = s_tile;
       __syncthreads();
@@ -331,6 +357,7 @@ namespace dtk
       t_visit += c2 - c1;
       if (vr.changed)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      t_rel += __builtin_amdgcn_s_memtime() - c2;
       publish(q, d, t, tx, ty, vr.border, !vr.converged);
       t_pub += __builtin_amdgcn_s_memtime() - c2;
     }
@@ -347,6 +374,8 @@ namespace dtk
       __hip_atomic_fetch_add(&q.ctrl[CTRL_T_SETUP], (uint32_t)(t_setup >> 10), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_fetch_add(&q.ctrl[CTRL_T_SWEEP], (uint32_t)(t_sweep >> 10), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_fetch_add(&q.ctrl[CTRL_T_STORE], (uint32_t)(t_store >> 10), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&q.ctrl[CTRL_T_REL], (uint32_t)(t_rel >> 10), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&q.ctrl[CTRL_T_ACQ], (uint32_t)(t_acq >> 10), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 

@@ -799,8 +799,10 @@ namespace dtk
         for (int r = 1; r < RH - 1; ++r)
         {
           const int gy = Y0 + r;
-          if (gy < d.H)
-            const_cast<float*>(drow<RH>(d, gy))[gx] = Dc[r];
+          if (gy < d.H) // device-scope store: written through, so the
+                        // release fence has no dirty L2 lines of D to write back
+            __hip_atomic_store(const_cast<float*>(drow<RH>(d, gy)) + gx, Dc[r], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         }
       }
     }

@@ -105,6 +105,8 @@ typedef struct
   uint32_t cycles_setup; /*   inside visits: tile loads and edge costs */
   uint32_t cycles_sweep; /*   inside visits: sweeps and layout changes */
   uint32_t cycles_store; /*   inside visits: border check and write-back */
+  uint32_t cycles_release; /* inside publishing: the release fence after a changing visit */
+  uint32_t cycles_acquire; /* inside waiting: claiming the popped tile and the acquire fence */
 } dtk_stats;
 
 void dtk_params_default(dtk_params* p);

@@ -516,7 +516,8 @@ int ora_raster_f64(const double* img, const double* smask, double* D, int W, int
 /* Central differences of L = sum(Wt * D_f64) w.r.t. image pixels and mask
  * pixels listed in idx[0..m).  out_img[k], out_mask[k] receive dL/dI, dL/ds. */
 void ora_fd_grad(const float* img, const float* smask, const float* Wt, int W, int H, double lambda, double nu,
-                 int conn8, double eps, const long* idx, int m, double* out_img, double* out_mask)
+                 int conn8, double eps, double eps_mask, const long* idx, int m, double* out_img,
+                 double* out_mask)
 {
   const long n  = (long)W * H;
   double*    I  = (double*)malloc(sizeof(double) * (size_t)n);
@@ -546,18 +547,18 @@ void ora_fd_grad(const float* img, const float* smask, const float* Wt, int W, i
     out_img[k] = (lp - lm) / (2 * eps);
 
     const double s0 = S[p];
-    S[p]            = s0 + eps;
+    S[p]            = s0 + eps_mask;
     ora_raster_f64(I, S, D, W, H, lambda, nu, conn8);
     lp = 0;
     for (long i = 0; i < n; ++i)
       lp += (double)Wt[i] * D[i];
-    S[p] = s0 - eps;
+    S[p] = s0 - eps_mask;
     ora_raster_f64(I, S, D, W, H, lambda, nu, conn8);
     lm = 0;
     for (long i = 0; i < n; ++i)
       lm += (double)Wt[i] * D[i];
     S[p]        = s0;
-    out_mask[k] = (lp - lm) / (2 * eps);
+    out_mask[k] = (lp - lm) / (2 * eps_mask);
   }
   free(I);
   free(S);

@@ -50,7 +50,7 @@ def _load():
         lib.ora_backward.argtypes = [fp, fp, vp, fp, vp, vp, vp, i, i, f, f]
         lib.ora_raster_f64.argtypes = [vp, vp, vp, i, i, dp, dp, i]
         lib.ora_raster_f64.restype = i
-        lib.ora_fd_grad.argtypes = [fp, fp, fp, i, i, dp, dp, i, dp, vp, i, vp, vp]
+        lib.ora_fd_grad.argtypes = [fp, fp, fp, i, i, dp, dp, i, dp, dp, vp, i, vp, vp]
         _lib = lib
     return _lib
 
@@ -155,8 +155,12 @@ def raster_f64(img, smask, lam: float, nu: float, conn8: bool = True) -> np.ndar
     return D
 
 
-def fd_grad(img, smask, Wt, lam: float, nu: float, idx, eps: float = 1e-4, conn8: bool = True):
-    """FP64 central differences of sum(Wt*D) at flat pixel indices idx."""
+def fd_grad(img, smask, Wt, lam: float, nu: float, idx, eps: float = 1e-4, conn8: bool = True,
+            eps_mask: float = None):
+    """FP64 central differences of sum(Wt*D) at flat pixel indices idx
+    (image step eps, soft-mask step eps_mask, default eps)."""
+    if eps_mask is None:
+        eps_mask = eps
     img = np.ascontiguousarray(img, np.float32)
     smask = np.ascontiguousarray(smask, np.float32)
     Wt = np.ascontiguousarray(Wt, np.float32)
@@ -164,6 +168,6 @@ def fd_grad(img, smask, Wt, lam: float, nu: float, idx, eps: float = 1e-4, conn8
     H, W = img.shape
     gi = np.empty(len(idx), np.float64)
     gm = np.empty(len(idx), np.float64)
-    _load().ora_fd_grad(_ptr(img), _ptr(smask), _ptr(Wt), W, H, lam, nu, int(conn8), eps, _ptr(idx), len(idx),
+    _load().ora_fd_grad(_ptr(img), _ptr(smask), _ptr(Wt), W, H, lam, nu, int(conn8), eps, eps_mask, _ptr(idx), len(idx),
                         _ptr(gi), _ptr(gm))
     return gi, gm

@@ -53,3 +53,46 @@ def assert_bit_exact(got: np.ndarray, want: np.ndarray, what: str = ""):
         y, x = bad[0]
         raise AssertionError(
             f"{what}: {len(bad)} / {g.size} pixels differ; first at (x={x}, y={y}): got {got[y, x]!r} want {want[y, x]!r}")
+
+
+def diff_crops(n_crops: int = 2, size: int = 64):
+    """64x64 crops of the differentiable-GGDT workload (float32 blob fields
+    plus N(0, 1e-3) noise, soft masks, loss weights; 1024x1024 images, seed
+    1004): per image, the crop on a 96-pixel grid with the most fractional
+    mask pixels among those holding some, but few, full seeds."""
+    from distance_transform_benchmarks_amd import synth
+
+    imgs, masks, wts = synth.diff32(n_crops, 1024)
+    out = []
+    for k in range(n_crops):
+        best = None
+        for y in range(0, 1024 - size, 96):
+            for x in range(0, 1024 - size, 96):
+                sm = masks[k, y:y + size, x:x + size]
+                frac = int(((sm > 0) & (sm < 1)).sum())
+                full = int((sm >= 1).sum())
+                if 0 < full < size * size // 8 and (best is None or frac > best[0]):
+                    best = (frac, y, x)
+        _, y, x = best
+        out.append(tuple(np.ascontiguousarray(a[k, y:y + size, x:x + size]) for a in (imgs, masks, wts)))
+    return out
+
+
+def fd_indices(gi: np.ndarray, gm: np.ndarray, seed: int, n_random: int = 200, n_top: int = 28) -> np.ndarray:
+    """Random pixels plus those with the largest reference gradients."""
+    rng = np.random.default_rng(seed)
+    n = gi.size
+    return np.unique(np.concatenate([rng.choice(n, min(n, n_random), replace=False),
+                                     np.argsort(-np.abs(gi.ravel()))[:n_top],
+                                     np.argsort(-np.abs(gm.ravel()))[:n_top]]))
+
+
+def assert_rel(got: np.ndarray, want: np.ndarray, rtol: float, what: str = ""):
+    """|got - want| <= rtol * max(|want|, 1e-3 * max|want|), elementwise."""
+    got, want = np.asarray(got, np.float64), np.asarray(want, np.float64)
+    floor = 1e-3 * max(np.abs(want).max(), 1e-30)
+    rel = np.abs(got - want) / np.maximum(np.abs(want), floor)
+    if rel.max() > rtol:
+        i = int(rel.argmax())
+        raise AssertionError(f"{what}: max relative error {rel.max():.3e} > {rtol} at {i} "
+                             f"(got {got.ravel()[i]!r}, want {want.ravel()[i]!r}; {int((rel > rtol).sum())} bad)")

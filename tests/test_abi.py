@@ -68,13 +68,13 @@ def test_struct_layout_matches_header(tmp_path):
         "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu\\n\","
         "sizeof(dtk_image), offsetof(dtk_image, pitch), offsetof(dtk_image, dtype),"
         "sizeof(dtk_params), offsetof(dtk_params, round_per_blocks), offsetof(dtk_params, waves_per_cu),"
-        "sizeof(dtk_stats), offsetof(dtk_stats, cycles_store)); return 0;}\n")
+        "sizeof(dtk_stats), offsetof(dtk_stats, cycles_acquire)); return 0;}\n")
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     want = [ctypes.sizeof(DtkImage), DtkImage.pitch.offset, DtkImage.dtype.offset, ctypes.sizeof(DtkParams),
             DtkParams.round_per_blocks.offset, DtkParams.waves_per_cu.offset, ctypes.sizeof(DtkStats),
-            DtkStats.cycles_store.offset]
+            DtkStats.cycles_acquire.offset]
     assert got == want
 
 

@@ -26,7 +26,7 @@ import zlib
 import numpy as np
 import pytest
 
-from _inputs import assert_bit_exact, image, mask
+from _inputs import assert_bit_exact, assert_rel, diff_crops, fd_indices, image, mask
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -213,6 +213,28 @@ def test_pred_forest_and_backward_vs_fd(oracle, seed, lam):
     fi, fm = oracle.fd_grad(img, m, Wt, lam, nu, np.arange(H * W), eps=1e-4)
     assert np.allclose(gi.ravel(), fi, rtol=1e-3, atol=1e-3 * max(1.0, np.abs(fi).max()))
     assert np.allclose(gm.ravel(), fm, rtol=1e-3, atol=1e-3 * max(1.0, np.abs(fm).max()))
+
+
+# Finite-difference steps of the adjoint checks.  The inputs are tie-free
+# (continuous noise), but the margins between competing paths and the
+# smallest |I(p) - I(q)| on tree edges are ~1e-5 .. 1e-4 here, so a step of
+# 1e-4 straddles kinks of the piecewise-linear map; 1e-7 on the image and
+# 1e-10 on the mask (nu * 1e-10 = 1e-7 in distance units) stay inside the
+# linear pieces while FP64 keeps the quotient accurate to ~1e-7.
+FD_EPS_IMG, FD_EPS_MASK = 1e-7, 1e-10
+
+
+def test_backward_vs_fd_on_diff_crops(oracle):
+    """Reference adjoint vs FP64 central differences on 64x64 crops of the
+    differentiable workload (C8, lambda = 1, nu = 1e3), 1e-3 relative."""
+    for k, (I, M, Wt) in enumerate(diff_crops()):
+        D, _ = oracle.raster(I, M, 1.0, 1e3)
+        P = oracle.pred(I, M, D, 1.0, 1e3)
+        gi, gm, _ = oracle.backward(I, D, P, Wt, 1.0, 1e3)
+        idx = fd_indices(gi, gm, k)
+        fi, fm = oracle.fd_grad(I, M, Wt, 1.0, 1e3, idx, eps=FD_EPS_IMG, eps_mask=FD_EPS_MASK)
+        assert_rel(gi.ravel()[idx], fi, 1e-3, f"crop {k} dL/dimage")
+        assert_rel(gm.ravel()[idx], fm, 1e-3, f"crop {k} dL/dmask")
 
 
 def test_synth_reproducible():
