@@ -200,7 +200,9 @@ def bench_batched(dt, synth, torch, dev, count):
         for i in range(count):
             dt.ggdt(imgs[i], masks[i], lam, 1e10, out=outs_a[i])
 
-    plan = dt.BatchPlan(imgs, masks, lam, 1e10, outs=outs_b)
+    t = time.time()
+    plan = dt.BatchPlan(imgs, masks, lam, 1e10, outs=outs_b, tune=True)
+    log(f"batched plan autotuned in {time.time() - t:.1f}s: {plan.tuning}")
     for _ in range(2):
         per_image()
         plan.launch()
@@ -224,7 +226,9 @@ def bench_batched(dt, synth, torch, dev, count):
                      f"lambda=0.8", "pixels": px,
          "per_image_ms": round(per_ms, 3), "per_image_imgs_s": round(count / per_ms * 1e3, 1),
          "batched_graph_ms": round(bat_ms, 3), "batched_imgs_s": round(count / bat_ms * 1e3, 1),
-         "speedup": round(per_ms / bat_ms, 3), "outputs_identical": bool(same)}
+         "speedup": round(per_ms / bat_ms, 3), "outputs_identical": bool(same),
+         "autotuned": {k: plan.tuning[k] for k in ("tile_rows", "max_sweeps", "waves_per_cu", "candidates",
+                                                   "best_ms", "default_ms")}}
     log(f"batched: per-image {per_ms:.2f} ms, graph {bat_ms:.2f} ms, x{per_ms / bat_ms:.2f}, identical {same}")
     return r
 

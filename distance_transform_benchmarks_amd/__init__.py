@@ -11,6 +11,7 @@ The Python names mirror the reference's C++ functions; see ``api``.
 """
 from .api import (  # noqa: F401
     BatchPlan,
+    autotune,
     backward,
     generalised_distance_transform,
     generalised_distance_transform_blocks,

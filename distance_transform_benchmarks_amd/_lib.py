@@ -28,7 +28,7 @@ EXPORTS = (
     "dtk_batch_plan_create", "dtk_batch_plan_launch", "dtk_batch_plan_destroy", "dtk_ggdt_pred",
     "dtk_ggdt_backward", "dtk_tile_count", "dtk_malloc_pitch", "dtk_free", "dtk_host_to_device",
     "dtk_device_to_host", "dtk_copy2d", "dtk_selftest", "dtk_device_cu_count", "dtk_profile_begin",
-    "dtk_profile_end",
+    "dtk_profile_end", "dtk_batch_autotune",
 )
 
 
@@ -50,6 +50,14 @@ class DtkStats(ctypes.Structure):
                 ("cycles_wait", ctypes.c_uint32), ("cycles_visit", ctypes.c_uint32), ("cycles_pub", ctypes.c_uint32),
                 ("cycles_setup", ctypes.c_uint32), ("cycles_sweep", ctypes.c_uint32), ("cycles_store", ctypes.c_uint32),
                 ("cycles_release", ctypes.c_uint32), ("cycles_acquire", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class DtkTuneReport(ctypes.Structure):
+    _fields_ = [("candidates", ctypes.c_int), ("from_cache", ctypes.c_int), ("best_ms", ctypes.c_float),
+                ("default_ms", ctypes.c_float)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -85,6 +93,8 @@ def load() -> ctypes.CDLL:
                                           P(vp)]
     lib.dtk_batch_plan_launch.argtypes = [vp, vp]
     lib.dtk_batch_plan_destroy.argtypes = [vp]
+    lib.dtk_batch_autotune.argtypes = [ctypes.c_int, P(DtkImage), P(DtkImage), P(DtkImage), P(DtkParams), vp,
+                                       ctypes.c_int, P(DtkParams), P(DtkTuneReport)]
     lib.dtk_ggdt_pred.argtypes = [P(DtkImage), P(DtkImage), P(DtkImage), P(DtkImage), P(DtkParams), vp]
     lib.dtk_ggdt_backward.argtypes = [P(DtkImage), P(DtkImage), P(DtkImage), P(DtkImage), P(DtkImage),
                                       P(DtkImage), P(DtkImage), P(DtkParams), vp]
@@ -101,7 +111,7 @@ def load() -> ctypes.CDLL:
     lib.dtk_profile_begin.argtypes = []
     lib.dtk_profile_end.argtypes = [P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_int)]
     for name in ("dtk_ggdt", "dtk_ggdt_batched", "dtk_batch_plan_create", "dtk_batch_plan_launch",
-                 "dtk_batch_plan_destroy", "dtk_ggdt_pred", "dtk_ggdt_backward", "dtk_malloc_pitch", "dtk_free",
+                 "dtk_batch_plan_destroy", "dtk_batch_autotune", "dtk_ggdt_pred", "dtk_ggdt_backward", "dtk_malloc_pitch", "dtk_free",
                  "dtk_host_to_device", "dtk_device_to_host", "dtk_copy2d", "dtk_selftest", "dtk_profile_begin",
                  "dtk_profile_end"):
         getattr(lib, name).restype = ctypes.c_int

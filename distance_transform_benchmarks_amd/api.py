@@ -161,19 +161,45 @@ def ggdt_batched(imgs: Sequence[torch.Tensor], masks: Sequence[torch.Tensor], la
     return (outs, st.as_dict()) if stats else outs
 
 
+def autotune(imgs, masks, lambda_, v=1e10, *, connectivity=8, outs=None, reps=2, recompute_costs=False):
+    """Pick tile_rows / max_sweeps / waves_per_cu for a batched plan by timing
+    candidate plans on this batch (dtk_batch_autotune).  Returns
+    (dict of the chosen parameters, report dict).  Memoised per process."""
+    if outs is None:
+        outs = [torch.empty(m.shape[:2], dtype=torch.float32, device=m.device) for m in masks]
+    p = _params(lambda_, v, connectivity, "task", 0, 0, 0, 0, recompute_costs=recompute_costs)
+    ai, am, ad = _batch_arrays(list(imgs), list(masks), list(outs))
+    best, rep = DtkParams(), _lib.DtkTuneReport()
+    dev = imgs[0].device
+    with torch.cuda.device(dev):
+        check(_lib.load().dtk_batch_autotune(len(imgs), ai, am, ad, ctypes.byref(p), _stream_handle(dev), int(reps),
+                                             ctypes.byref(best), ctypes.byref(rep)))
+    return ({"tile_rows": best.tile_rows, "max_sweeps": best.max_sweeps, "waves_per_cu": best.waves_per_cu},
+            rep.as_dict())
+
+
 class BatchPlan:
     """A batched GGDT captured as a HIP graph over fixed buffers.
 
     The tensors given at creation are bound into the graph: refill them in
-    place and call ``launch()`` to recompute every ``outs[i]``.
+    place and call ``launch()`` to recompute every ``outs[i]``.  With
+    ``tune=True`` the tile height, sweep cap and workers per CU come from
+    ``autotune`` (timed on these buffers before capture).
     """
 
     def __init__(self, imgs, masks, lambda_, v=1e10, *, connectivity=8, tile_rows=0, max_sweeps=0,
-                 waves_per_cu=0, outs=None, recompute_costs=False):
+                 waves_per_cu=0, outs=None, recompute_costs=False, tune=False):
         if outs is None:
             outs = [torch.empty(m.shape[:2], dtype=torch.float32, device=m.device) for m in masks]
         self.imgs, self.masks, self.outs = list(imgs), list(masks), list(outs)
         self.device = self.imgs[0].device
+        self.tuning = None
+        if tune:
+            chosen, report = autotune(self.imgs, self.masks, lambda_, v, connectivity=connectivity, outs=self.outs,
+                                      recompute_costs=recompute_costs)
+            tile_rows, max_sweeps, waves_per_cu = chosen["tile_rows"], chosen["max_sweeps"], chosen["waves_per_cu"]
+            self.tuning = {**chosen, **report}
+        self.params = {"tile_rows": tile_rows, "max_sweeps": max_sweeps, "waves_per_cu": waves_per_cu}
         p = _params(lambda_, v, connectivity, "task", 0, tile_rows, max_sweeps, waves_per_cu,
                     recompute_costs=recompute_costs)
         ai, am, ad = _batch_arrays(self.imgs, self.masks, self.outs)

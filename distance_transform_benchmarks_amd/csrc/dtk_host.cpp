@@ -15,7 +15,9 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -761,6 +763,141 @@ dtk_status dtk_batch_plan_destroy(dtk_batch_plan* plan)
   if (plan->dbuf)
     hipFree(plan->dbuf);
   delete plan;
+  return DTK_OK;
+}
+
+namespace
+{
+  struct TuneEntry
+  {
+    dtk_params best;
+    float      best_ms;
+    float      default_ms;
+  };
+  std::mutex                       g_tune_mu;
+  std::map<std::string, TuneEntry> g_tune;
+
+  std::string tune_key(int dev, int n, const dtk_image* imgs, const dtk_image* masks, const dtk_params* p)
+  {
+    long long total = 0;
+    int       side  = 0;
+    for (int i = 0; i < n; ++i)
+    {
+      total += (long long)imgs[i].width * imgs[i].height;
+      side = std::max(side, std::max(imgs[i].width, imgs[i].height));
+    }
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), "%d|%d|%lld|%d|%d|%d|%d|%a|%a|%d", dev, n, total, side, imgs[0].dtype,
+                  masks[0].dtype, p->connectivity, (double)p->lambda, (double)p->v, p->recompute_costs);
+    return buf;
+  }
+
+  // Mean launch time of one candidate plan (warm-up launch excluded).
+  dtk_status time_plan(int n, const dtk_image* imgs, const dtk_image* masks, dtk_image* dists, const dtk_params* p,
+                       hipStream_t s, int reps, float* ms)
+  {
+    dtk_batch_plan* plan = nullptr;
+    dtk_status      st   = dtk_batch_plan_create(n, imgs, masks, dists, p, s, &plan);
+    if (st != DTK_OK)
+      return st;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e  = hipEventCreate(&e0);
+    if (e == hipSuccess)
+      e = hipEventCreate(&e1);
+    if (e == hipSuccess)
+      e = hipGraphLaunch(plan->exec, s);
+    if (e == hipSuccess)
+      e = hipEventRecord(e0, s);
+    for (int r = 0; r < reps && e == hipSuccess; ++r)
+      e = hipGraphLaunch(plan->exec, s);
+    if (e == hipSuccess)
+      e = hipEventRecord(e1, s);
+    if (e == hipSuccess)
+      e = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (e == hipSuccess)
+      e = hipEventElapsedTime(&t, e0, e1);
+    if (e0)
+      hipEventDestroy(e0);
+    if (e1)
+      hipEventDestroy(e1);
+    dtk_batch_plan_destroy(plan);
+    if (e != hipSuccess)
+      return fail(DTK_ERR_HIP, std::string("autotune timing: ") + hipGetErrorString(e));
+    *ms = t / (float)reps;
+    return DTK_OK;
+  }
+} // namespace
+
+dtk_status dtk_batch_autotune(int n, const dtk_image* imgs, const dtk_image* masks, dtk_image* dists,
+                              const dtk_params* p, void* stream, int reps, dtk_params* best, dtk_tune_report* report)
+{
+  dtk_status st;
+  if (!best)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "best is NULL");
+  if ((st = check_params(p)) != DTK_OK)
+    return st;
+  if (n <= 0 || !imgs || !masks || !dists)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "bad batch arrays");
+  if (p->schedule != DTK_SCHED_TASK)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "batched plans use the task schedule");
+  if (reps <= 0)
+    reps = 2;
+  int dev = 0;
+  DTK_HIP(hipGetDevice(&dev), "hipGetDevice");
+  hipStream_t       s   = static_cast<hipStream_t>(stream);
+  const std::string key = tune_key(dev, n, imgs, masks, p);
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto                        it = g_tune.find(key);
+    if (it != g_tune.end())
+    {
+      *best = it->second.best;
+      if (report)
+        *report = dtk_tune_report{0, 1, it->second.best_ms, it->second.default_ms};
+      return DTK_OK;
+    }
+  }
+  const bool c8  = p->connectivity == DTK_C8;
+  const bool two = two_adds(p);
+  dtk_params cand = *p;
+  dtk_params top  = *p;
+  float      top_ms = 0.f, default_ms = 0.f;
+  int        count  = 0;
+  for (int rows : {32, 16})
+  {
+    const int occ = std::max(1, task_occupancy(64 / rows, c8, two));
+    int       wpcs[3] = {occ, (occ * 3 + 3) / 4, (occ + 1) / 2};
+    for (int wi = 0; wi < 3; ++wi)
+    {
+      if (wi > 0 && wpcs[wi] == wpcs[wi - 1])
+        continue;
+      for (int cap : {0, 12})
+      {
+        cand.tile_rows    = rows;
+        cand.waves_per_cu = wi == 0 ? 0 : wpcs[wi];
+        cand.max_sweeps   = cap;
+        float ms          = 0.f;
+        if ((st = time_plan(n, imgs, masks, dists, &cand, s, reps, &ms)) != DTK_OK)
+          return st;
+        if (count == 0 || ms < top_ms)
+        {
+          top    = cand;
+          top_ms = ms;
+        }
+        if (rows == 32 && wi == 0 && cap == 0)
+          default_ms = ms;
+        ++count;
+      }
+    }
+  }
+  *best = top;
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune[key] = TuneEntry{top, top_ms, default_ms};
+  }
+  if (report)
+    *report = dtk_tune_report{count, 0, top_ms, default_ms};
   return DTK_OK;
 }
 

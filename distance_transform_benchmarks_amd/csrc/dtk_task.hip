@@ -318,8 +318,12 @@ namespace dtk
     }
   }
 
+#ifndef DTK_TASK_WAVES_PER_EU
+#define DTK_TASK_WAVES_PER_EU 1
+#endif
   template <int S, bool C8, bool TWO>
-  __global__ void __launch_bounds__(64) k_task(Batch b, KParams p, Queue q)
+  __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DTK_TASK_WAVES_PER_EU)))
+  k_task(Batch b, KParams p, Queue q)
   {
     __shared__ float T[(64 / S) * 65];
     __shared__ float Sv[2 * (64 / S)];

@@ -142,6 +142,27 @@ dtk_status dtk_batch_plan_create(int n, const dtk_image* imgs, const dtk_image* 
 dtk_status dtk_batch_plan_launch(dtk_batch_plan* plan, void* stream);
 dtk_status dtk_batch_plan_destroy(dtk_batch_plan* plan);
 
+/* Autotuner of batched plans.  Times candidate tile heights (tile_rows 32 /
+ * 16), sweep caps per visit (max_sweeps 0 / 12) and persistent workers per
+ * CU (the occupancy limit, 3/4 and 1/2 of it) on the given batch: each
+ * candidate is a captured plan, launched once to warm up and then `reps`
+ * times between HIP events on `stream`.  Writes *p with the fastest
+ * tile_rows / max_sweeps / waves_per_cu into *best.  Every candidate
+ * computes the same bit-exact maps (the fixed point does not depend on the
+ * schedule), so tuning never changes results; dists are overwritten.
+ * Results are memoised per process and device by a signature of the batch
+ * (count, pixel total, largest side, dtypes, connectivity, lambda, v).
+ * No reference counterpart. */
+typedef struct
+{
+  int   candidates; /* plans timed (0 when served from the memo) */
+  int   from_cache; /* 1: *best came from the memo */
+  float best_ms;    /* mean launch time of the chosen candidate */
+  float default_ms; /* mean launch time of the default parameters */
+} dtk_tune_report;
+dtk_status dtk_batch_autotune(int n, const dtk_image* imgs, const dtk_image* masks, dtk_image* dists,
+                              const dtk_params* p, void* stream, int reps, dtk_params* best, dtk_tune_report* report);
+
 /* Argmin predecessor codes of a converged map (uint8 image, same size as
  * dist): 0..7 = neighbour offset index in the order of dt::c8
  * (dt/include/dt/c8.hpp:19-28): (-1,0) (-1,-1) (0,-1) (1,-1) (1,0) (1,1)

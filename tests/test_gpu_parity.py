@@ -188,6 +188,22 @@ def test_maze_known_answer(dt, dev):
         assert np.array_equal(got, img.astype(np.float32))
 
 
+def test_maze8192_full_size(dt, dev):
+    """The adversarial 8192x8192 spiral maze (one 9-px corridor, 3-px walls at
+    255, seed at the centre), C8, lambda=1, task schedule: at full size the
+    known answer D = image still holds bit for bit."""
+    import torch
+
+    from distance_transform_benchmarks_amd import synth
+
+    img, m = synth.maze8192(8192)
+    _say("maze 8192 task")
+    di = _to_dev(img, dev)
+    D, st = dt.ggdt(di, _to_dev(m, dev), 1.0, 1e10, stats=True)
+    torch.cuda.synchronize()
+    assert torch.equal(D, di.float()), st
+
+
 def test_blob512_config0(dt, dev, oracle):
     """512x512 grey blob field, 8 seed discs, C4, lambda=1 (the CPU-runnable workload)."""
     from distance_transform_benchmarks_amd import synth
@@ -241,6 +257,37 @@ def test_batch_plan_graph_replays(dt, dev, oracle):
             assert_bit_exact(o.cpu().numpy(), w, f"replay {rep} image {i}")
         assert_bit_exact(eager.cpu().numpy(), wants[0])
     plan.close()
+
+
+def test_autotuned_plan(dt, dev, oracle):
+    """The autotuner times candidate plans on the batch; the tuned plan's maps
+    are bit-exact (tuning only changes the schedule), and a second request for
+    the same batch is served from the memo."""
+    import torch
+
+    rng = np.random.default_rng(21)
+    imgs, masks = [], []
+    for i in range(10):
+        W, H = int(rng.integers(40, 400)), int(rng.integers(40, 400))
+        imgs.append(image("rgb8", W, H, 400 + i))
+        masks.append(mask("f32", W, H, 400 + i, 3))
+    wants = [oracle.raster(a, m, 0.5, 1e3)[0] for a, m in zip(imgs, masks)]
+    dimgs = [_to_dev(a, dev) for a in imgs]
+    dmasks = [_to_dev(a, dev) for a in masks]
+    _say("autotune")
+    plan = dt.BatchPlan(dimgs, dmasks, 0.5, 1e3, tune=True)
+    assert plan.tuning["candidates"] >= 6 and plan.tuning["from_cache"] == 0
+    assert plan.params["tile_rows"] in (16, 32) and plan.params["max_sweeps"] in (0, 12)
+    for o in plan.outs:
+        o.fill_(-1.0)
+    outs = plan.launch()
+    torch.cuda.synchronize()
+    for i, (o, w) in enumerate(zip(outs, wants)):
+        assert_bit_exact(o.cpu().numpy(), w, f"image {i}")
+    plan.close()
+    chosen, report = dt.autotune(dimgs, dmasks, 0.5, 1e3)
+    assert report["from_cache"] == 1 and report["candidates"] == 0
+    assert chosen == {k: plan.params[k] for k in chosen}
 
 
 @pytest.mark.parametrize("conn", [4, 8])
