@@ -71,6 +71,7 @@ def main(argv=None) -> int:
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--max-sweeps", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--tile-waves", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched comparison")
     ap.add_argument("--batch-count", type=int, default=256)
@@ -89,7 +90,8 @@ def main(argv=None) -> int:
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     lambdas = (0.0, 0.5, 1.0)
-    kw = dict(connectivity=8, tile_rows=a.tile_rows, max_sweeps=a.max_sweeps, waves_per_cu=a.waves_per_cu)
+    kw = dict(connectivity=8, tile_rows=a.tile_rows, max_sweeps=a.max_sweeps, waves_per_cu=a.waves_per_cu,
+              tile_waves=a.tile_waves)
 
     t0 = time.time()
     img_np, mask_np = synth.rgb4096(a.size)
@@ -227,7 +229,7 @@ def bench_batched(dt, synth, torch, dev, count):
          "per_image_ms": round(per_ms, 3), "per_image_imgs_s": round(count / per_ms * 1e3, 1),
          "batched_graph_ms": round(bat_ms, 3), "batched_imgs_s": round(count / bat_ms * 1e3, 1),
          "speedup": round(per_ms / bat_ms, 3), "outputs_identical": bool(same),
-         "autotuned": {k: plan.tuning[k] for k in ("tile_rows", "max_sweeps", "waves_per_cu", "candidates",
+         "autotuned": {k: plan.tuning[k] for k in ("tile_rows", "tile_waves", "max_sweeps", "waves_per_cu", "candidates",
                                                    "best_ms", "default_ms")}}
     log(f"batched: per-image {per_ms:.2f} ms, graph {bat_ms:.2f} ms, x{per_ms / bat_ms:.2f}, identical {same}")
     return r

@@ -217,11 +217,23 @@ namespace
       return fail(DTK_ERR_INVALID_ARGUMENT, "unknown schedule");
     if (p->tile_rows != 0 && p->tile_rows != 32 && p->tile_rows != 16)
       return fail(DTK_ERR_INVALID_ARGUMENT, "tile_rows must be 0, 16 or 32");
+    if (!(p->tile_waves == 0 || p->tile_waves == 1 || p->tile_waves == 2 || p->tile_waves == 4) ||
+        (p->tile_waves > 1 && p->tile_rows == 16))
+      return fail(DTK_ERR_INVALID_ARGUMENT, "tile_waves must be 0, 1, 2 or 4 (2 and 4 need 32-row regions)");
     if (p->max_iterations < 0 || p->max_sweeps < 0 || p->waves_per_cu < 0)
       return fail(DTK_ERR_INVALID_ARGUMENT, "negative iteration / sweep / wave count");
     if (p->max_iterations > 0 && p->schedule != DTK_SCHED_RASTER)
       return fail(DTK_ERR_INVALID_ARGUMENT, "a fixed iteration count needs the raster schedule");
     return DTK_OK;
+  }
+
+  // Stacked waves per tile: the task schedule stacks two 32-row regions
+  // (a 62 x 60 owned block) unless told otherwise; other schedules use one.
+  int tile_waves(const dtk_params* p, int rh)
+  {
+    if (p->schedule != DTK_SCHED_TASK || rh != 32)
+      return 1;
+    return p->tile_waves == 0 ? kDefaultTileWaves : p->tile_waves;
   }
 
   KParams make_kparams(const dtk_params* p, int img_dtype, int mask_dtype, int rh)
@@ -238,6 +250,7 @@ namespace
     k.max_sweeps  = p->max_sweeps;
     k.rh          = rh;
     k.c8          = p->connectivity == DTK_C8;
+    k.nw          = tile_waves(p, rh);
     return k;
   }
 
@@ -245,7 +258,7 @@ namespace
 
   int region_rows(const dtk_params* p) { return p->tile_rows == 0 ? 32 : p->tile_rows; }
 
-  ImgDesc make_desc(const dtk_image* img, const dtk_image* mask, dtk_image* dist, int rh)
+  ImgDesc make_desc(const dtk_image* img, const dtk_image* mask, dtk_image* dist, int rh, int nw)
   {
     ImgDesc d{};
     d.img        = static_cast<const unsigned char*>(img->data);
@@ -259,7 +272,7 @@ namespace
     d.W          = img->width;
     d.H          = img->height;
     d.tiles_x    = (d.W + kTileW - 1) / kTileW;
-    d.tiles_y    = (d.H + (rh - 2) - 1) / (rh - 2);
+    d.tiles_y    = (d.H + nw * (rh - 2) - 1) / (nw * (rh - 2));
     d.tile0      = 0;
     return d;
   }
@@ -279,13 +292,15 @@ namespace
     return DTK_OK;
   }
 
+  // Persistent workers (workgroups of nw waves) of a task-schedule call.
   int task_workers(const Workspace& w, const dtk_params* p, int S, bool c8, bool two, int ntiles)
   {
-    int per_cu = task_occupancy(S, c8, two);
+    const int nw     = tile_waves(p, 64 / S);
+    int       per_cu = task_occupancy(S, c8, two, nw);
     if (per_cu <= 0)
       per_cu = 1;
-    if (p->waves_per_cu > 0 && p->waves_per_cu < per_cu)
-      per_cu = p->waves_per_cu;
+    if (p->waves_per_cu > 0 && p->waves_per_cu / nw < per_cu)
+      per_cu = std::max(1, p->waves_per_cu / nw);
     long long n = (long long)per_cu * w.cu;
     if (n > ntiles)
       n = ntiles;
@@ -373,7 +388,7 @@ namespace
         return fail(s, "image " + std::to_string(i) + ": " + g_err);
       if (imgs[i].dtype != imgs[0].dtype || masks[i].dtype != masks[0].dtype)
         return fail(DTK_ERR_INVALID_ARGUMENT, "all images of a batch must share image and mask dtypes");
-      descs[i]       = make_desc(&imgs[i], &masks[i], &dists[i], rh);
+      descs[i]       = make_desc(&imgs[i], &masks[i], &dists[i], rh, tile_waves(p, rh));
       descs[i].tile0 = (int)total;
       if (descs[i].W > 0 && descs[i].H > 0)
         total += (long long)descs[i].tiles_x * descs[i].tiles_y;
@@ -473,7 +488,7 @@ dtk_status dtk_ggdt(const dtk_image* img, const dtk_image* mask, dtk_image* dist
   const bool two = two_adds(p);
   const int  S   = 64 / rh;
   KParams    kp  = make_kparams(p, img->dtype, mask->dtype, rh);
-  ImgDesc    d   = make_desc(img, mask, dist, rh);
+  ImgDesc    d   = make_desc(img, mask, dist, rh, tile_waves(p, rh));
 
   if (p->schedule == DTK_SCHED_RASTER)
   {
@@ -864,10 +879,12 @@ dtk_status dtk_batch_autotune(int n, const dtk_image* imgs, const dtk_image* mas
   dtk_params top  = *p;
   float      top_ms = 0.f, default_ms = 0.f;
   int        count  = 0;
-  for (int rows : {32, 16})
+  const int shapes[4][2] = {{32, 2}, {32, 4}, {32, 1}, {16, 1}}; // tile_rows, tile_waves
+  for (const auto& shp : shapes)
   {
-    const int occ = std::max(1, task_occupancy(64 / rows, c8, two));
-    int       wpcs[3] = {occ, (occ * 3 + 3) / 4, (occ + 1) / 2};
+    const int rows = shp[0], nw = shp[1];
+    const int occ  = std::max(1, task_occupancy(64 / rows, c8, two, nw)) * nw; // waves per CU
+    int       wpcs[3] = {occ, (occ * 3 / 4 + nw - 1) / nw * nw, (occ / 2 + nw - 1) / nw * nw};
     for (int wi = 0; wi < 3; ++wi)
     {
       if (wi > 0 && wpcs[wi] == wpcs[wi - 1])
@@ -875,6 +892,7 @@ dtk_status dtk_batch_autotune(int n, const dtk_image* imgs, const dtk_image* mas
       for (int cap : {0, 12})
       {
         cand.tile_rows    = rows;
+        cand.tile_waves   = nw;
         cand.waves_per_cu = wi == 0 ? 0 : wpcs[wi];
         cand.max_sweeps   = cap;
         float ms          = 0.f;
@@ -885,7 +903,7 @@ dtk_status dtk_batch_autotune(int n, const dtk_image* imgs, const dtk_image* mas
           top    = cand;
           top_ms = ms;
         }
-        if (rows == 32 && wi == 0 && cap == 0)
+        if (rows == 32 && nw == kDefaultTileWaves && wi == 0 && cap == 0)
           default_ms = ms;
         ++count;
       }
@@ -914,7 +932,7 @@ dtk_status dtk_ggdt_pred(const dtk_image* img, const dtk_image* mask, const dtk_
     return st;
   if (pred->width != img->width || pred->height != img->height)
     return fail(DTK_ERR_INVALID_ARGUMENT, "pred size differs");
-  ImgDesc d    = make_desc(img, mask, &dcopy, 32);
+  ImgDesc d    = make_desc(img, mask, &dcopy, 32, 1);
   d.pred       = static_cast<unsigned char*>(pred->data);
   d.pred_pitch = pred->pitch;
   KParams kp   = make_kparams(p, img->dtype, mask->dtype, 32);
@@ -958,7 +976,7 @@ dtk_status dtk_ggdt_backward(const dtk_image* img, const dtk_image* mask, const 
   float*    G      = c.take<float>(npx);
   uint32_t* counts = c.take<uint32_t>(npx);
   float*    acc    = c.take<float>(npx);
-  ImgDesc   d      = make_desc(img, mask, &dcopy, 32);
+  ImgDesc   d      = make_desc(img, mask, &dcopy, 32, 1);
   d.pred           = static_cast<unsigned char*>(pred->data);
   d.pred_pitch     = pred->pitch;
   KParams kp       = make_kparams(p, img->dtype, mask->dtype, 32);

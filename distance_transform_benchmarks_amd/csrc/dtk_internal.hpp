@@ -25,6 +25,10 @@ namespace dtk
 
   constexpr int kTileW = 62; // owned columns per tile (64 lanes minus the halo)
 
+  // Stacked 32-row waves per tile of the task schedule unless a call says
+  // otherwise (dtk_params.tile_waves).
+  constexpr int kDefaultTileWaves = 2;
+
   // One image of a call (or of a batch).  Pitches are in bytes.
   struct ImgDesc
   {
@@ -74,6 +78,8 @@ namespace dtk
     int   max_sweeps; // sweep cap per tile visit (0 = until stable)
     int   rh;         // region rows (64 / S)
     int   c8;         // 8-connectivity (layout of the cached cost records)
+    int   nw;         // stacked waves per tile (task schedule, rh == 32): a tile
+                      // owns nw * (rh - 2) rows
   };
 
   // Device work queue of dirty tiles (persistent kernel).

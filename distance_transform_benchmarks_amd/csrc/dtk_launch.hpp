@@ -16,7 +16,7 @@ namespace dtk
                          hipStream_t s);
   hipError_t launch_blocks(int S, bool c8, bool two, const Batch& b, const KParams& p, uint8_t* active, uint8_t* next,
                            uint32_t* ctrl, int* rpt, hipStream_t s);
-  int        task_occupancy(int S, bool c8, bool two);
+  int        task_occupancy(int S, bool c8, bool two, int nw);
 
   // dtk_raster.hip: one full raster iteration (four passes) with the exact
   // reference schedule; `changed` (device u32) is set when a pixel decreased.

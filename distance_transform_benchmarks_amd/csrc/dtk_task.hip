@@ -72,7 +72,7 @@ namespace dtk
     ImgDesc   d;
     int       tx, ty;
     resolve_tile(b, t, d, tx, ty);
-    const int  RH  = p.rh;
+    const int  RH  = p.nw * (p.rh - 2) + 2; // rows of the tile with its halo
     const int  X0  = tx * kTileW - 1;
     const int  Y0  = ty * (RH - 2) - 1;
     const int  gx  = X0 + lane;
@@ -321,13 +321,17 @@ namespace dtk
 #ifndef DTK_TASK_WAVES_PER_EU
 #define DTK_TASK_WAVES_PER_EU 1
 #endif
-  template <int S, bool C8, bool TWO>
-  __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DTK_TASK_WAVES_PER_EU)))
+  // One workgroup of NW waves per worker; NW stacked waves solve one tile.
+  template <int S, bool C8, bool TWO, int NW>
+  __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(DTK_TASK_WAVES_PER_EU)))
   k_task(Batch b, KParams p, Queue q)
   {
-    __shared__ float T[(64 / S) * 65];
-    __shared__ float Sv[2 * (64 / S)];
-    __shared__ int   s_tile;
+    __shared__ float    T[NW][(64 / S) * 65];
+    __shared__ float    Sv[NW][2 * (64 / S)];
+    __shared__ float    X[NW > 1 ? 2 * NW * 4 * 64 : 1];
+    __shared__ uint32_t F[3 * NW];
+    __shared__ int      s_tile;
+    const int           w = NW > 1 ? (int)(threadIdx.x >> 6) : 0;
     uint32_t         visits = 0, sweeps = 0;
     uint64_t         t_wait = 0, t_visit = 0, t_pub = 0, t_setup = 0, t_sweep = 0, t_store = 0;
     uint64_t         t_rel = 0, t_acq = 0;
@@ -349,20 +353,24 @@ namespace dtk
       ImgDesc d;
       int     tx, ty;
       resolve_tile(b, t, d, tx, ty);
-      const VisitResult vr = visit_tile<S, C8, TWO>(d, p, tx, ty, T, Sv);
+      const VisitResult vr = visit_tile<S, C8, TWO, NW>(d, p, tx, ty, T[w], Sv[w], X, F);
       ++visits;
       sweeps += (uint32_t)vr.sweeps;
       t_setup += vr.cyc_setup;
       t_sweep += vr.cyc_sweep;
       t_store += vr.cyc_store;
-      // Every lane's D stores are ordered before the release below.
+      // Every wave's D stores are ordered (workgroup barrier) before the
+      // release below; wave 0 publishes for the whole tile.
       __syncthreads();
       const uint64_t c2 = __builtin_amdgcn_s_memtime();
       t_visit += c2 - c1;
-      if (vr.changed)
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      t_rel += __builtin_amdgcn_s_memtime() - c2;
-      publish(q, d, t, tx, ty, vr.border, !vr.converged);
+      if (w == 0)
+      {
+        if (vr.changed)
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        t_rel += __builtin_amdgcn_s_memtime() - c2;
+        publish(q, d, t, tx, ty, vr.border, !vr.converged);
+      }
       t_pub += __builtin_amdgcn_s_memtime() - c2;
     }
     if (threadIdx.x == 0 && visits)
@@ -404,7 +412,7 @@ namespace dtk
     ImgDesc d;
     int     tx, ty;
     resolve_tile(b, t, d, tx, ty);
-    const VisitResult vr = visit_tile<S, C8, TWO>(d, p, tx, ty, T, Sv);
+    const VisitResult vr = visit_tile<S, C8, TWO, 1>(d, p, tx, ty, T, Sv, nullptr, nullptr);
     if (threadIdx.x == 0)
     {
       __hip_atomic_fetch_add(&ctrl[CTRL_VISITS], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -450,7 +458,20 @@ namespace dtk
   template <int S, bool C8, bool TWO>
   static hipError_t launch_task_t(const Batch& b, const KParams& p, const Queue& q, int workers, hipStream_t s)
   {
-    hipLaunchKernelGGL((k_task<S, C8, TWO>), dim3(workers), dim3(64), 0, s, b, p, q);
+    if constexpr (S == 2)
+    {
+      if (p.nw == 2)
+      {
+        hipLaunchKernelGGL((k_task<S, C8, TWO, 2>), dim3(workers), dim3(128), 0, s, b, p, q);
+        return hipGetLastError();
+      }
+      if (p.nw == 4)
+      {
+        hipLaunchKernelGGL((k_task<S, C8, TWO, 4>), dim3(workers), dim3(256), 0, s, b, p, q);
+        return hipGetLastError();
+      }
+    }
+    hipLaunchKernelGGL((k_task<S, C8, TWO, 1>), dim3(workers), dim3(64), 0, s, b, p, q);
     return hipGetLastError();
   }
 
@@ -463,12 +484,17 @@ namespace dtk
   }
 
   template <int S, bool C8, bool TWO>
-  static int occupancy_t()
+  static int occupancy_t(int nw)
   {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_task<S, C8, TWO>, 64, 0) != hipSuccess)
-      return 0;
-    return n;
+    if constexpr (S == 2)
+    {
+      if (nw == 2)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_task<S, C8, TWO, 2>, 128, 0) == hipSuccess ? n : 0;
+      if (nw == 4)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_task<S, C8, TWO, 4>, 256, 0) == hipSuccess ? n : 0;
+    }
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_task<S, C8, TWO, 1>, 64, 0) == hipSuccess ? n : 0;
   }
 
   hipError_t launch_reset_ctrl(uint32_t* ctrl, hipStream_t s)
@@ -527,16 +553,16 @@ namespace dtk
                : launch_blocks_t<4, false, false>(b, p, active, next, ctrl, rpt, s);
   }
 
-  int task_occupancy(int S, bool c8, bool two)
+  int task_occupancy(int S, bool c8, bool two, int nw)
   {
     if (S == 2)
     {
       if (c8)
-        return two ? occupancy_t<2, true, true>() : occupancy_t<2, true, false>();
-      return two ? occupancy_t<2, false, true>() : occupancy_t<2, false, false>();
+        return two ? occupancy_t<2, true, true>(nw) : occupancy_t<2, true, false>(nw);
+      return two ? occupancy_t<2, false, true>(nw) : occupancy_t<2, false, false>(nw);
     }
     if (c8)
-      return two ? occupancy_t<4, true, true>() : occupancy_t<4, true, false>();
-    return two ? occupancy_t<4, false, true>() : occupancy_t<4, false, false>();
+      return two ? occupancy_t<4, true, true>(1) : occupancy_t<4, true, false>(1);
+    return two ? occupancy_t<4, false, true>(1) : occupancy_t<4, false, false>(1);
   }
 } // namespace dtk

@@ -644,17 +644,65 @@ namespace dtk
   }
 
   // ---- one visit of one tile --------------------------------------------------
-  // T: LDS RH*65 floats; Sv: LDS 2*RH floats.  Loads the region, iterates to
-  // its local fixed point (or max_sweeps), writes back the owned pixels and
-  // reports which neighbours saw a border pixel change.
-  template <int S, bool C8, bool TWO>
-  __device__ __attribute__((always_inline)) VisitResult visit_tile(const ImgDesc& d, const KParams& p, int tx, int ty, float* T, float* Sv)
+  // Boundary-row exchange between the NW stacked waves of a tile.  Wave w's
+  // rows 0 and 1 are the same image rows as wave w-1's rows RH-2 and RH-1
+  // (its halo and first owned row), so after each pair of sweeps both copies
+  // of those rows are merged by min through LDS (X: 2 parities x NW x 4 rows
+  // x 64 floats; F: 2 x NW change flags).  Returns whether any wave changed
+  // a pixel in the pair just swept (the same in every wave).
+  template <int RH, int NW>
+  __device__ __forceinline__ bool exchange_rows(float (&Dc)[RH], float* X, uint32_t* F, int w, int lane, int par,
+                                                bool changed, bool& any)
+  {
+    float* Xp = X + par * (NW * 4 * 64);
+    Xp[(w * 4 + 0) * 64 + lane] = Dc[0];
+    Xp[(w * 4 + 1) * 64 + lane] = Dc[1];
+    Xp[(w * 4 + 2) * 64 + lane] = Dc[RH - 2];
+    Xp[(w * 4 + 3) * 64 + lane] = Dc[RH - 1];
+    if (lane == 0)
+      F[par * NW + w] = changed ? 1u : 0u;
+    __syncthreads();
+    bool lowered = false;
+    if (w > 0)
+    {
+      const float a = Xp[((w - 1) * 4 + 2) * 64 + lane];
+      const float b = Xp[((w - 1) * 4 + 3) * 64 + lane];
+      lowered |= lt_nn(a, Dc[0]) | lt_nn(b, Dc[1]);
+      Dc[0] = minf_nn(Dc[0], a);
+      Dc[1] = minf_nn(Dc[1], b);
+    }
+    if (w < NW - 1)
+    {
+      const float a = Xp[((w + 1) * 4 + 0) * 64 + lane];
+      const float b = Xp[((w + 1) * 4 + 1) * 64 + lane];
+      lowered |= lt_nn(a, Dc[RH - 2]) | lt_nn(b, Dc[RH - 1]);
+      Dc[RH - 2] = minf_nn(Dc[RH - 2], a);
+      Dc[RH - 1] = minf_nn(Dc[RH - 1], b);
+    }
+    any |= ballot(lowered) != 0;
+    bool wg = false;
+#pragma unroll
+    for (int k = 0; k < NW; ++k)
+      wg |= F[par * NW + k] != 0u;
+    return wg;
+  }
+
+  // A tile is solved by NW stacked waves (one 64 x RH region each, RH - 2
+  // owned rows each).  T: LDS RH*65 floats of this wave; Sv: LDS 2*RH floats
+  // of this wave; X, F: shared exchange buffers (NW > 1 only).  Loads the
+  // regions, iterates to the tile's local fixed point (or max_sweeps),
+  // writes back the owned pixels and reports which neighbours saw a border
+  // pixel improve (border and changed are the same in every wave).
+  template <int S, bool C8, bool TWO, int NW>
+  __device__ __attribute__((always_inline)) VisitResult visit_tile(const ImgDesc& d, const KParams& p, int tx, int ty,
+                                                                   float* T, float* Sv, float* X, uint32_t* F)
   {
     constexpr int RH   = 64 / S;
     constexpr int SW   = 64 / S;
-    const int     lane = threadIdx.x;
+    const int     lane = threadIdx.x & 63;
+    const int     w    = NW > 1 ? (int)(threadIdx.x >> 6) : 0;
     const int     X0   = tx * kTileW - 1;
-    const int     Y0   = ty * (RH - 2) - 1;
+    const int     Y0   = ty * (NW * (RH - 2)) - 1 + w * (RH - 2);
     const int     gx   = X0 + lane;
     const bool    cin  = gx >= 0 && gx < d.W;
     const uint64_t clk0 = __builtin_amdgcn_s_memtime();
@@ -691,7 +739,7 @@ namespace dtk
     }
 
     // Halo as loaded: a neighbour needs a new visit exactly when one of its
-    // pixels in this region's halo ends up strictly smaller (this visit then
+    // pixels in this tile's halo ends up strictly smaller (this visit then
     // found a shorter path into that neighbour).
     const float init_top = Dc[0];
     const float init_bot = Dc[RH - 1];
@@ -711,54 +759,97 @@ namespace dtk
     int         quiet = 0, sweeps = 0;
     bool        conv = false, any = false;
     float       R[SW + 2];
-    for (;;)
+    if constexpr (NW == 1)
     {
-      uint64_t ch = sweep_col_fwd<RH, C8, TWO>(Dc, cN, cNW, cNE, ea, ed);
-      ++sweeps;
-      any |= ch != 0;
-      quiet = ch ? 0 : quiet + 1;
-      if (quiet >= 4)
+      for (;;)
       {
-        conv = true;
-        break;
-      }
-      if (sweeps >= cap)
-        break;
-      ch = sweep_col_bwd<RH, C8, TWO>(Dc, cN, cNW, cNE, ea, ed);
-      ++sweeps;
-      any |= ch != 0;
-      quiet = ch ? 0 : quiet + 1;
-      if (quiet >= 4)
-      {
-        conv = true;
-        break;
-      }
-      if (sweeps >= cap)
-        break;
+        uint64_t ch = sweep_col_fwd<RH, C8, TWO>(Dc, cN, cNW, cNE, ea, ed);
+        ++sweeps;
+        any |= ch != 0;
+        quiet = ch ? 0 : quiet + 1;
+        if (quiet >= 4)
+        {
+          conv = true;
+          break;
+        }
+        if (sweeps >= cap)
+          break;
+        ch = sweep_col_bwd<RH, C8, TWO>(Dc, cN, cNW, cNE, ea, ed);
+        ++sweeps;
+        any |= ch != 0;
+        quiet = ch ? 0 : quiet + 1;
+        if (quiet >= 4)
+        {
+          conv = true;
+          break;
+        }
+        if (sweeps >= cap)
+          break;
 
-      col_to_row<S>(Dc, R, T, lane);
-      const uint64_t cf = sweep_row_fwd<SW, C8, TWO>(R, rW, rNW, rSW, ea, ed);
-      ++sweeps;
-      any |= cf != 0;
-      quiet = cf ? 0 : quiet + 1;
-      if (quiet >= 4 || sweeps >= cap)
-      {
-        conv = quiet >= 4;
+        col_to_row<S>(Dc, R, T, lane);
+        const uint64_t cf = sweep_row_fwd<SW, C8, TWO>(R, rW, rNW, rSW, ea, ed);
+        ++sweeps;
+        any |= cf != 0;
+        quiet = cf ? 0 : quiet + 1;
+        if (quiet >= 4 || sweeps >= cap)
+        {
+          conv = quiet >= 4;
+          row_to_col<S>(R, Dc, T, lane);
+          break;
+        }
+        const uint64_t cb = sweep_row_bwd<SW, C8, TWO>(R, rW, rNW, rSW, ea, ed);
+        ++sweeps;
+        any |= cb != 0;
+        quiet = (cb | cf) ? 0 : quiet + 1;
         row_to_col<S>(R, Dc, T, lane);
-        break;
+        if (quiet >= 4)
+        {
+          conv = true;
+          break;
+        }
+        if (sweeps >= cap)
+          break;
       }
-      const uint64_t cb = sweep_row_bwd<SW, C8, TWO>(R, rW, rNW, rSW, ea, ed);
-      ++sweeps;
-      any |= cb != 0;
-      quiet = (cb | cf) ? 0 : quiet + 1;
-      row_to_col<S>(R, Dc, T, lane);
-      if (quiet >= 4)
+    }
+    else
+    {
+      // Sweep pairs (column down+up, row right+left) in every wave at once;
+      // the shared rows are merged after each pair, and the tile is stable
+      // after two consecutive pairs without a change in any wave.
+      int par = 0;
+      for (;;)
       {
-        conv = true;
-        break;
+        uint64_t ch = sweep_col_fwd<RH, C8, TWO>(Dc, cN, cNW, cNE, ea, ed);
+        ch |= sweep_col_bwd<RH, C8, TWO>(Dc, cN, cNW, cNE, ea, ed);
+        sweeps += 2;
+        any |= ch != 0;
+        bool wg = exchange_rows<RH, NW>(Dc, X, F, w, lane, par, ch != 0, any);
+        par ^= 1;
+        quiet = wg ? 0 : quiet + 2;
+        if (quiet >= 4)
+        {
+          conv = true;
+          break;
+        }
+        if (sweeps >= cap)
+          break;
+        col_to_row<S>(Dc, R, T, lane);
+        ch = sweep_row_fwd<SW, C8, TWO>(R, rW, rNW, rSW, ea, ed);
+        ch |= sweep_row_bwd<SW, C8, TWO>(R, rW, rNW, rSW, ea, ed);
+        row_to_col<S>(R, Dc, T, lane);
+        sweeps += 2;
+        any |= ch != 0;
+        wg = exchange_rows<RH, NW>(Dc, X, F, w, lane, par, ch != 0, any);
+        par ^= 1;
+        quiet = wg ? 0 : quiet + 2;
+        if (quiet >= 4)
+        {
+          conv = true;
+          break;
+        }
+        if (sweeps >= cap)
+          break;
       }
-      if (sweeps >= cap)
-        break;
     }
 
     const uint64_t clk2 = __builtin_amdgcn_s_memtime();
@@ -767,6 +858,7 @@ namespace dtk
     res.converged = conv;
     res.changed   = any;
     res.border    = 0;
+    uint32_t b    = 0;
     if (any)
     {
       bool colimp = false;
@@ -781,16 +873,20 @@ namespace dtk
       const uint64_t bot  = ballot(lt_nn(Dc[RH - 1], init_bot));
       const uint64_t cols = ballot(colimp);
       const uint64_t mid  = 0x7FFFFFFFFFFFFFFEull; // lanes 1..62
-      uint32_t       b    = 0;
-      b |= (uint32_t)(cols & 1u);                     // L : left halo column
-      b |= (uint32_t)((cols >> 63) & 1u) << 1;        // R : right halo column
-      b |= ((top & mid) != 0 ? 1u : 0u) << 2;         // T : top halo row
-      b |= ((bot & mid) != 0 ? 1u : 0u) << 3;         // B : bottom halo row
-      b |= (uint32_t)(top & 1u) << 4;                 // TL: top-left corner
-      b |= (uint32_t)((top >> 63) & 1u) << 5;         // TR
-      b |= (uint32_t)(bot & 1u) << 6;                 // BL
-      b |= (uint32_t)((bot >> 63) & 1u) << 7;         // BR
-      res.border = b;
+      b |= (uint32_t)(cols & 1u);                 // L : left halo column
+      b |= (uint32_t)((cols >> 63) & 1u) << 1;    // R : right halo column
+      if (w == 0)
+      {
+        b |= ((top & mid) != 0 ? 1u : 0u) << 2;   // T : top halo row
+        b |= (uint32_t)(top & 1u) << 4;           // TL: top-left corner
+        b |= (uint32_t)((top >> 63) & 1u) << 5;   // TR
+      }
+      if (w == NW - 1)
+      {
+        b |= ((bot & mid) != 0 ? 1u : 0u) << 3;   // B : bottom halo row
+        b |= (uint32_t)(bot & 1u) << 6;           // BL
+        b |= (uint32_t)((bot >> 63) & 1u) << 7;   // BR
+      }
 
       // Write back the owned pixels.
       if (lane >= 1 && lane <= 62 && cin)
@@ -805,6 +901,21 @@ namespace dtk
                                __HIP_MEMORY_SCOPE_AGENT);
         }
       }
+    }
+    if constexpr (NW == 1)
+      res.border = b;
+    else
+    {
+      // Border bits and change flags of all waves (F[2*NW ..]).
+      if (lane == 0)
+        F[2 * NW + w] = b | (any ? 0x100u : 0u);
+      __syncthreads();
+      uint32_t all = 0;
+#pragma unroll
+      for (int k = 0; k < NW; ++k)
+        all |= F[2 * NW + k];
+      res.border  = all & 0xFFu;
+      res.changed = (all & 0x100u) != 0;
     }
     res.cyc_setup = (uint32_t)(clk1 - clk0);
     res.cyc_sweep = (uint32_t)(clk2 - clk1);

@@ -47,6 +47,16 @@ def _run(dt, dev, img, m, lam, v, sched, **kw):
     return D.cpu().numpy()
 
 
+def _variants(sched, tile_rows=None):
+    """Tile shapes a schedule can run: tile_rows 16 / 32, and for the task
+    schedule also two stacked 32-row waves per tile."""
+    rows = (16, 32) if tile_rows is None else (tile_rows,)
+    out = [dict(tile_rows=r) for r in rows]
+    if sched == "task" and 32 in rows:
+        out += [dict(tile_rows=32, tile_waves=2), dict(tile_rows=32, tile_waves=4)]
+    return out
+
+
 def test_selftest(dt):
     _say("selftest")
     r = dt.selftest()
@@ -71,9 +81,10 @@ def test_fastgeodis_golden(dt, dev, sched, tile_rows):
             img = np.array(fx["image_u8"], np.uint8).reshape(H, W)
             m[fx["seed"][1], fx["seed"][0]] = 1
         want = np.array(fx["expected_f32_bits"], np.uint32).view(np.float32).reshape(H, W)
-        _say(name, sched, tile_rows)
-        got = _run(dt, dev, img, m, fx["lambda"], fx["v"], sched, tile_rows=tile_rows)
-        assert_bit_exact(got, want, f"{name} {sched}")
+        for kw in _variants(sched, tile_rows):
+            _say(name, sched, kw)
+            got = _run(dt, dev, img, m, fx["lambda"], fx["v"], sched, **kw)
+            assert_bit_exact(got, want, f"{name} {sched} {kw}")
 
 
 CASES = [
@@ -101,10 +112,10 @@ def test_random_vs_oracle(dt, dev, oracle, case, sched):
     img, m = image(ik, W, H, seed), mask(mk, W, H, seed)
     v = 1e10 if mk == "u8" else 1e3
     want, _ = oracle.raster(img, m, lam, v, conn8=conn == 8)
-    for tr in (16, 32):
-        _say(case, sched, tr)
-        got = _run(dt, dev, img, m, lam, v, sched, connectivity=conn, tile_rows=tr)
-        assert_bit_exact(got, want, f"{case} {sched} rows={tr}")
+    for kw in _variants(sched):
+        _say(case, sched, kw)
+        got = _run(dt, dev, img, m, lam, v, sched, connectivity=conn, **kw)
+        assert_bit_exact(got, want, f"{case} {sched} {kw}")
 
 
 @pytest.mark.parametrize("max_sweeps", [1, 2, 5])
@@ -112,9 +123,10 @@ def test_task_sweep_cap(dt, dev, oracle, max_sweeps):
     """A sweep cap re-queues unconverged tiles; the fixed point is unchanged."""
     img, m = image("u8", 300, 211, 5), mask("u8", 300, 211, 5, 4)
     want, _ = oracle.raster(img, m, 1.0, 1e10)
-    _say("sweep cap", max_sweeps)
-    got = _run(dt, dev, img, m, 1.0, 1e10, "task", max_sweeps=max_sweeps)
-    assert_bit_exact(got, want)
+    for tw in (1, 2, 4):
+        _say("sweep cap", max_sweeps, "tile_waves", tw)
+        got = _run(dt, dev, img, m, 1.0, 1e10, "task", max_sweeps=max_sweeps, tile_waves=tw)
+        assert_bit_exact(got, want)
 
 
 @pytest.mark.parametrize("iters", [1, 2, 3])
@@ -129,15 +141,17 @@ def test_fixed_iterations(dt, dev, oracle, iters, conn):
     assert_bit_exact(got, want)
 
 
-@pytest.mark.parametrize("shape", [(1, 1), (1, 9), (9, 1), (62, 14), (63, 15), (64, 16), (2, 70), (70, 2)])
+@pytest.mark.parametrize("shape", [(1, 1), (1, 9), (9, 1), (62, 14), (63, 15), (64, 16), (2, 70), (70, 2),
+                                   (62, 60), (63, 61), (64, 62), (70, 121), (125, 31), (130, 241)])
 @pytest.mark.parametrize("sched", ["raster", "blocks", "task"])
 def test_edge_shapes(dt, dev, oracle, shape, sched):
     W, H = shape
     img, m = image("u8", W, H, W * 31 + H), mask("u8", W, H, W + H, 1)
     want, _ = oracle.raster(img, m, 0.5, 1e10)
-    _say("edge", shape, sched)
-    got = _run(dt, dev, img, m, 0.5, 1e10, sched)
-    assert_bit_exact(got, want)
+    for kw in _variants(sched):
+        _say("edge", shape, sched, kw)
+        got = _run(dt, dev, img, m, 0.5, 1e10, sched, **kw)
+        assert_bit_exact(got, want)
 
 
 @pytest.mark.parametrize("sched", ["raster", "blocks", "task"])
@@ -276,8 +290,9 @@ def test_autotuned_plan(dt, dev, oracle):
     dmasks = [_to_dev(a, dev) for a in masks]
     _say("autotune")
     plan = dt.BatchPlan(dimgs, dmasks, 0.5, 1e3, tune=True)
-    assert plan.tuning["candidates"] >= 6 and plan.tuning["from_cache"] == 0
+    assert plan.tuning["candidates"] >= 9 and plan.tuning["from_cache"] == 0
     assert plan.params["tile_rows"] in (16, 32) and plan.params["max_sweeps"] in (0, 12)
+    assert plan.params["tile_waves"] in (1, 2, 4)
     for o in plan.outs:
         o.fill_(-1.0)
     outs = plan.launch()
