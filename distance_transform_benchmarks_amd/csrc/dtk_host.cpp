@@ -217,9 +217,8 @@ namespace
       return fail(DTK_ERR_INVALID_ARGUMENT, "unknown schedule");
     if (p->tile_rows != 0 && p->tile_rows != 32 && p->tile_rows != 16)
       return fail(DTK_ERR_INVALID_ARGUMENT, "tile_rows must be 0, 16 or 32");
-    if (!(p->tile_waves == 0 || p->tile_waves == 1 || p->tile_waves == 2 || p->tile_waves == 4) ||
-        (p->tile_waves > 1 && p->tile_rows == 16))
-      return fail(DTK_ERR_INVALID_ARGUMENT, "tile_waves must be 0, 1, 2 or 4 (2 and 4 need 32-row regions)");
+    if (!(p->tile_waves == 0 || p->tile_waves == 1 || p->tile_waves == 2 || p->tile_waves == 4))
+      return fail(DTK_ERR_INVALID_ARGUMENT, "tile_waves must be 0, 1, 2 or 4");
     if (p->max_iterations < 0 || p->max_sweeps < 0 || p->waves_per_cu < 0)
       return fail(DTK_ERR_INVALID_ARGUMENT, "negative iteration / sweep / wave count");
     if (p->max_iterations > 0 && p->schedule != DTK_SCHED_RASTER)
@@ -231,9 +230,11 @@ namespace
   // (a 62 x 60 owned block) unless told otherwise; other schedules use one.
   int tile_waves(const dtk_params* p, int rh)
   {
-    if (p->schedule != DTK_SCHED_TASK || rh != 32)
+    if (p->schedule != DTK_SCHED_TASK)
       return 1;
-    return p->tile_waves == 0 ? kDefaultTileWaves : p->tile_waves;
+    if (p->tile_waves != 0)
+      return p->tile_waves;
+    return rh == 32 ? 2 : 4;
   }
 
   KParams make_kparams(const dtk_params* p, int img_dtype, int mask_dtype, int rh)
@@ -256,7 +257,15 @@ namespace
 
   bool two_adds(const dtk_params* p) { return !(p->lambda == 0.f || p->lambda == 1.f); }
 
-  int region_rows(const dtk_params* p) { return p->tile_rows == 0 ? 32 : p->tile_rows; }
+  // Region rows per wave: the task schedule defaults to 16-row waves stacked
+  // four high (62 x 56 owned pixels, two waves per SIMD), the blocks
+  // schedule to single 32-row waves (62 x 30).
+  int region_rows(const dtk_params* p)
+  {
+    if (p->tile_rows != 0)
+      return p->tile_rows;
+    return p->schedule == DTK_SCHED_TASK ? kDefaultTaskRows : 32;
+  }
 
   ImgDesc make_desc(const dtk_image* img, const dtk_image* mask, dtk_image* dist, int rh, int nw)
   {
@@ -879,7 +888,7 @@ dtk_status dtk_batch_autotune(int n, const dtk_image* imgs, const dtk_image* mas
   dtk_params top  = *p;
   float      top_ms = 0.f, default_ms = 0.f;
   int        count  = 0;
-  const int shapes[4][2] = {{32, 2}, {32, 4}, {32, 1}, {16, 1}}; // tile_rows, tile_waves
+  const int shapes[5][2] = {{16, 4}, {32, 2}, {16, 2}, {32, 1}, {16, 1}}; // tile_rows, tile_waves
   for (const auto& shp : shapes)
   {
     const int rows = shp[0], nw = shp[1];
@@ -903,7 +912,7 @@ dtk_status dtk_batch_autotune(int n, const dtk_image* imgs, const dtk_image* mas
           top    = cand;
           top_ms = ms;
         }
-        if (rows == 32 && nw == kDefaultTileWaves && wi == 0 && cap == 0)
+        if (rows == kDefaultTaskRows && nw == 4 && wi == 0 && cap == 0)
           default_ms = ms;
         ++count;
       }

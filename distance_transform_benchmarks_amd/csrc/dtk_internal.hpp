@@ -25,9 +25,8 @@ namespace dtk
 
   constexpr int kTileW = 62; // owned columns per tile (64 lanes minus the halo)
 
-  // Stacked 32-row waves per tile of the task schedule unless a call says
-  // otherwise (dtk_params.tile_waves).
-  constexpr int kDefaultTileWaves = 2;
+  // Default region rows per wave of the task schedule (stacked four high).
+  constexpr int kDefaultTaskRows = 16;
 
   // One image of a call (or of a batch).  Pitches are in bytes.
   struct ImgDesc

@@ -458,18 +458,15 @@ namespace dtk
   template <int S, bool C8, bool TWO>
   static hipError_t launch_task_t(const Batch& b, const KParams& p, const Queue& q, int workers, hipStream_t s)
   {
-    if constexpr (S == 2)
+    if (p.nw == 2)
     {
-      if (p.nw == 2)
-      {
-        hipLaunchKernelGGL((k_task<S, C8, TWO, 2>), dim3(workers), dim3(128), 0, s, b, p, q);
-        return hipGetLastError();
-      }
-      if (p.nw == 4)
-      {
-        hipLaunchKernelGGL((k_task<S, C8, TWO, 4>), dim3(workers), dim3(256), 0, s, b, p, q);
-        return hipGetLastError();
-      }
+      hipLaunchKernelGGL((k_task<S, C8, TWO, 2>), dim3(workers), dim3(128), 0, s, b, p, q);
+      return hipGetLastError();
+    }
+    if (p.nw == 4)
+    {
+      hipLaunchKernelGGL((k_task<S, C8, TWO, 4>), dim3(workers), dim3(256), 0, s, b, p, q);
+      return hipGetLastError();
     }
     hipLaunchKernelGGL((k_task<S, C8, TWO, 1>), dim3(workers), dim3(64), 0, s, b, p, q);
     return hipGetLastError();
@@ -487,13 +484,10 @@ namespace dtk
   static int occupancy_t(int nw)
   {
     int n = 0;
-    if constexpr (S == 2)
-    {
-      if (nw == 2)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_task<S, C8, TWO, 2>, 128, 0) == hipSuccess ? n : 0;
-      if (nw == 4)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_task<S, C8, TWO, 4>, 256, 0) == hipSuccess ? n : 0;
-    }
+    if (nw == 2)
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_task<S, C8, TWO, 2>, 128, 0) == hipSuccess ? n : 0;
+    if (nw == 4)
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_task<S, C8, TWO, 4>, 256, 0) == hipSuccess ? n : 0;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_task<S, C8, TWO, 1>, 64, 0) == hipSuccess ? n : 0;
   }
 
@@ -562,7 +556,7 @@ namespace dtk
       return two ? occupancy_t<2, false, true>(nw) : occupancy_t<2, false, false>(nw);
     }
     if (c8)
-      return two ? occupancy_t<4, true, true>(1) : occupancy_t<4, true, false>(1);
-    return two ? occupancy_t<4, false, true>(1) : occupancy_t<4, false, false>(1);
+      return two ? occupancy_t<4, true, true>(nw) : occupancy_t<4, true, false>(nw);
+    return two ? occupancy_t<4, false, true>(nw) : occupancy_t<4, false, false>(nw);
   }
 } // namespace dtk

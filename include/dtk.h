@@ -82,13 +82,13 @@ typedef struct
   int   connectivity;   /* DTK_C8 (reference) or DTK_C4 */
   int   schedule;       /* dtk_schedule */
   int   max_iterations; /* 0: run to convergence; N > 0: exactly N raster iterations (raster only) */
-  int   tile_rows;      /* tile solver region rows: 32 or 16 (0 = autotuned/default) */
+  int   tile_rows;      /* rows of one wave's region: 32 or 16 (0 = default: 16 for task, 32 for blocks) */
   int   max_sweeps;     /* sweep cap per tile visit (0 = until the tile is stable) */
   int   waves_per_cu;   /* persistent waves per CU (0 = occupancy limit) */
   int   recompute_costs; /* tile schedules, lambda > 0: 0 = compute edge costs once per call into a
                             workspace plane (16 B/px C8, 8 B/px C4), 1 = recompute them on every visit */
-  int   tile_waves;      /* task schedule: 32-row waves stacked per tile, 1, 2 or 4 (62 x 30, 62 x 60 or 62 x 120
-                            owned pixels; 2 and 4 need tile_rows 32); 0 = library default (2) */
+  int   tile_waves;      /* task schedule: waves stacked per tile, 1, 2 or 4; a tile owns 62 x tile_waves *
+                            (tile_rows - 2) pixels.  0 = library default (4 with 16-row, 2 with 32-row waves) */
   int*  round_per_blocks; /* optional device int[dtk_tile_count()]: visits per tile (blocks schedule) */
 } dtk_params;
 
@@ -144,8 +144,9 @@ dtk_status dtk_batch_plan_create(int n, const dtk_image* imgs, const dtk_image* 
 dtk_status dtk_batch_plan_launch(dtk_batch_plan* plan, void* stream);
 dtk_status dtk_batch_plan_destroy(dtk_batch_plan* plan);
 
-/* Autotuner of batched plans.  Times candidate tile shapes (tile_rows 32 with
- * 2, 4 or 1 stacked waves, tile_rows 16), sweep caps per visit (max_sweeps 0 /
+/* Autotuner of batched plans.  Times candidate tile shapes (16-row waves
+ * stacked 4, 2 or 1 high, 32-row waves stacked 2 or 1 high), sweep caps per
+ * visit (max_sweeps 0 /
  * 12) and persistent waves per CU (the occupancy limit, 3/4 and 1/2 of it)
  * on the given batch: each
  * candidate is a captured plan, launched once to warm up and then `reps`

@@ -48,13 +48,12 @@ def _run(dt, dev, img, m, lam, v, sched, **kw):
 
 
 def _variants(sched, tile_rows=None):
-    """Tile shapes a schedule can run: tile_rows 16 / 32, and for the task
-    schedule also two stacked 32-row waves per tile."""
+    """Tile shapes a schedule can run: one wave of 16 or 32 rows per tile,
+    and for the task schedule 2 or 4 stacked waves of either height."""
     rows = (16, 32) if tile_rows is None else (tile_rows,)
-    out = [dict(tile_rows=r) for r in rows]
-    if sched == "task" and 32 in rows:
-        out += [dict(tile_rows=32, tile_waves=2), dict(tile_rows=32, tile_waves=4)]
-    return out
+    if sched != "task":
+        return [dict(tile_rows=r) for r in rows]
+    return [dict(tile_rows=r, tile_waves=w) for r in rows for w in (1, 2, 4)]
 
 
 def test_selftest(dt):
