@@ -318,12 +318,12 @@ namespace dtk
     }
   }
 
-#ifndef DTK_TASK_WAVES_PER_EU
-#define DTK_TASK_WAVES_PER_EU 1
+#ifndef DTK_TASK_WAVES_PER_EU_16
+#define DTK_TASK_WAVES_PER_EU_16 2
 #endif
   // One workgroup of NW waves per worker; NW stacked waves solve one tile.
   template <int S, bool C8, bool TWO, int NW>
-  __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(DTK_TASK_WAVES_PER_EU)))
+  __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(S == 4 ? DTK_TASK_WAVES_PER_EU_16 : 1)))
   k_task(Batch b, KParams p, Queue q)
   {
     __shared__ float    T[NW][(64 / S) * 65];
