@@ -1,10 +1,12 @@
 """HBM traffic of a kernel from rocprofv3 PMC counters.
 
-Two sub-commands:
+Three sub-commands:
 
   copy       one device-to-device copy of a known size (torch ``copy_`` of a
              float32 tensor), the calibration workload: its FETCH_SIZE and
              WRITE_SIZE are compared with the bytes it must move;
+  report     writes a text summary of a kernel-trace run, the bench line of
+             the same run and a traffic summary (the files under profiles/);
   summarize  reads the rocprofv3 ``counter_collection.csv`` files of a
              FETCH_SIZE run and a WRITE_SIZE run of the workload and of the
              copy, scales each counter by the copy's known/measured ratio and
@@ -103,6 +105,48 @@ def summarize(a) -> dict:
     return out
 
 
+def report(a) -> str:
+    """Text summary of a kernel-trace run (rocprofv3 --kernel-trace --stats
+    kernel_stats.csv), the bench line of the same run and a traffic summary."""
+    files = sorted(glob.glob(os.path.join(a.trace, "**", "*kernel_stats.csv"), recursive=True))
+    if not files:
+        raise FileNotFoundError(f"no kernel_stats.csv under {a.trace}")
+    rows = list(csv.DictReader(open(files[0], newline="")))
+    bench = json.load(open(a.bench)) if a.bench else None
+    traffic = json.load(open(a.traffic)) if a.traffic else None
+    out = [f"# {a.title}", "", "Kernel statistics (rocprofv3 --kernel-trace --stats; every dispatch of the run):", "",
+           f"{'kernel':<70} {'calls':>5} {'avg us':>10} {'min us':>10} {'max us':>10} {'%':>6}"]
+    for r in rows:
+        out.append(f"{r['Name'][:70]:<70} {r['Calls']:>5} {float(r['AverageNs']) / 1e3:>10.1f} "
+                   f"{float(r['MinNs']) / 1e3:>10.1f} {float(r['MaxNs']) / 1e3:>10.1f} {float(r['Percentage']):>6.2f}")
+    kt = [r for r in rows if a.kernel in r["Name"]]
+    if kt:
+        tot = sum(float(r["TotalDurationNs"]) for r in kt)
+        n = sum(int(r["Calls"]) for r in kt)
+        out += ["", f"{a.kernel}: {n} dispatches, average {tot / n / 1e6:.4f} ms"]
+    if bench:
+        rl = bench["roofline"]
+        out += [f"bench line of the same run: {a.kernel} average from HIP events on its stream {rl['kernel_avg_ms']} ms "
+                f"over {rl['kernel_launches']} timed launches -> algorithmic {rl['achieved']} GB/s = "
+                f"{rl['frac']} of {rl['peak']} GB/s ({rl['algorithmic_bytes_per_launch']} B per launch)",
+                f"value {bench['value']} {bench['unit']}, {bench['ms_per_step']} ms per step"]
+    if traffic:
+        c = traffic["copy_calibration"]
+        out += ["", "HBM traffic per dispatch (FETCH_SIZE and WRITE_SIZE in separate runs, each scaled by its reading "
+                "for a device copy of known size):",
+                f"  copy of {c['bytes']} B: FETCH_SIZE {c['fetch_kib']} KiB (scale {c['fetch_scale']}), "
+                f"WRITE_SIZE {c['write_kib']} KiB (scale {c['write_scale']})",
+                f"  {traffic['kernel']}: raw FETCH {traffic['raw_fetch_kib_mean']} KiB, WRITE "
+                f"{traffic['raw_write_kib_mean']} KiB per dispatch ({traffic['dispatches_fetch']} dispatches)",
+                f"  scaled: fetch {traffic['fetch_bytes_per_launch'] / 1e6:.1f} MB + write "
+                f"{traffic['write_bytes_per_launch'] / 1e6:.1f} MB = {traffic['bytes_per_launch'] / 1e6:.1f} MB per launch"]
+        if "traffic_over_algorithmic" in traffic:
+            out.append(f"  = {traffic['traffic_over_algorithmic']} x the algorithmic bytes per launch")
+    if a.commands:
+        out += ["", "Commands (repository root, TMPDIR=/tmp):"] + [f"  {c}" for c in a.commands]
+    return "\n".join(out) + "\n"
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -118,7 +162,22 @@ def main(argv=None) -> int:
     s.add_argument("--copy-bytes", type=int, default=COPY_BYTES)
     s.add_argument("--algorithmic-bytes", type=int, default=0)
     s.add_argument("--out", default="")
+    r = sub.add_parser("report")
+    r.add_argument("--trace", required=True)
+    r.add_argument("--kernel", default="k_task")
+    r.add_argument("--bench", default="")
+    r.add_argument("--traffic", default="")
+    r.add_argument("--title", default="kernel trace")
+    r.add_argument("--commands", nargs="*", default=[])
+    r.add_argument("--out", default="")
     a = ap.parse_args(argv)
+    if a.cmd == "report":
+        txt = report(a)
+        if a.out:
+            with open(a.out, "w") as fh:
+                fh.write(txt)
+        print(txt, end="")
+        return 0
     if a.cmd == "copy":
         run_copy(a.bytes)
         return 0
