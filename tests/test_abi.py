@@ -152,3 +152,19 @@ def test_cpp_suite_cpu_part():
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert " 0 failed" in r.stdout
+
+
+def test_gbench_corpus_and_summary():
+    """The benchmark harness's synthetic corpus matches the reference's image
+    sizes and its summary is the median Mpix/s per family and lambda."""
+    from distance_transform_benchmarks_amd import gbench
+
+    for n in gbench.REFERENCE_SIZES:
+        w, h = gbench.corpus_shape(n)
+        assert abs(w * h - n) <= w and 1.3 < w / h < 1.37
+    doc = {"benchmarks": [
+        {"name": "DistanceTransformTask/0/1/manual_time", "lambda": 0.5, "bytes_per_second": 3e8},
+        {"name": "DistanceTransformTask/1/1/manual_time", "lambda": 0.5, "bytes_per_second": 5e8},
+        {"name": "DistanceTransformTask/2/1/manual_time", "lambda": 0.5, "bytes_per_second": 4e8},
+        {"name": "DistanceTransformGeos/0/0/manual_time", "lambda": 0.0, "bytes_per_second": 1e7}]}
+    assert gbench.summarize(doc) == {"DistanceTransformGeos lambda=0.0": 10.0, "DistanceTransformTask lambda=0.5": 400.0}

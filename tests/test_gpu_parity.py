@@ -217,6 +217,19 @@ def test_maze8192_full_size(dt, dev):
     assert torch.equal(D, di.float()), st
 
 
+def test_gbench_corpus_image_vs_oracle(dt, dev, oracle):
+    """Image 0 of the benchmark harness's corpus (8.5 Mpx grey blob field,
+    one centre seed), the reference benchmark's call: C8, v = 1e10."""
+    from distance_transform_benchmarks_amd import gbench
+
+    img, m = gbench.corpus_image(0)
+    want, _ = oracle.raster(img, m, 0.5, 1e10, threads=oracle.max_threads())
+    for sched in ("task", "blocks"):
+        _say("gbench corpus 0", sched)
+        got = _run(dt, dev, img, m, 0.5, 1e10, sched)
+        assert_bit_exact(got, want, sched)
+
+
 def test_blob512_config0(dt, dev, oracle):
     """512x512 grey blob field, 8 seed discs, C4, lambda=1 (the CPU-runnable workload)."""
     from distance_transform_benchmarks_amd import synth
