@@ -22,6 +22,8 @@ from .api import (  # noqa: F401
     round_per_blocks_buffer,
     selftest,
     tile_count,
+    tile_grid,
+    visits_buffer,
 )
 from .autograd import GGDTFunction, ggdt_differentiable  # noqa: F401
 

@@ -28,7 +28,7 @@ EXPORTS = (
     "dtk_batch_plan_create", "dtk_batch_plan_launch", "dtk_batch_plan_destroy", "dtk_ggdt_pred",
     "dtk_ggdt_backward", "dtk_tile_count", "dtk_malloc_pitch", "dtk_free", "dtk_host_to_device",
     "dtk_device_to_host", "dtk_copy2d", "dtk_selftest", "dtk_device_cu_count", "dtk_profile_begin",
-    "dtk_profile_end", "dtk_batch_autotune",
+    "dtk_profile_end", "dtk_batch_autotune", "dtk_tile_count_for",
 )
 
 
@@ -101,6 +101,8 @@ def load() -> ctypes.CDLL:
                                       P(DtkImage), P(DtkImage), P(DtkParams), vp]
     lib.dtk_tile_count.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.dtk_tile_count.restype = ctypes.c_int
+    lib.dtk_tile_count_for.argtypes = [ctypes.c_int, ctypes.c_int, P(DtkParams)]
+    lib.dtk_tile_count_for.restype = ctypes.c_int
     lib.dtk_malloc_pitch.argtypes = [P(vp), P(ctypes.c_longlong), ctypes.c_longlong, ctypes.c_int]
     lib.dtk_free.argtypes = [vp]
     lib.dtk_host_to_device.argtypes = [P(DtkImage), P(DtkImage)]

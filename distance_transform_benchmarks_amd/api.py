@@ -35,7 +35,7 @@ from ._lib import DtkImage, DtkParams, DtkStats, check
 __all__ = [
     "generalised_distance_transform", "generalised_distance_transform_blocks",
     "generalised_distance_transform_task", "ggdt", "ggdt_batched", "BatchPlan", "predecessors",
-    "backward", "selftest", "tile_count", "round_per_blocks_buffer",
+    "backward", "selftest", "tile_count", "round_per_blocks_buffer", "tile_grid", "visits_buffer",
 ]
 
 
@@ -100,6 +100,10 @@ def ggdt(img: torch.Tensor, mask: torch.Tensor, lambda_: float, v: float = 1e10,
         out = torch.empty((H, W), dtype=torch.float32, device=img.device)
     p = _params(lambda_, v, connectivity, schedule, max_iterations, tile_rows, max_sweeps, waves_per_cu,
                 round_per_blocks, recompute_costs, tile_waves)
+    if round_per_blocks is not None:
+        need = _lib.load().dtk_tile_count_for(W, H, ctypes.byref(p))
+        if not round_per_blocks.is_contiguous() or round_per_blocks.numel() < need:
+            raise ValueError(f"round_per_blocks needs {need} contiguous int32 counters for this call")
     di, dm, dd = _as_image(img, "image"), _as_image(mask, "mask"), _as_image(out, "dist")
     st = DtkStats()
     with torch.cuda.device(img.device):
@@ -130,6 +134,21 @@ def tile_count(width: int, height: int, tile_rows: int = 0) -> int:
 def round_per_blocks_buffer(width: int, height: int, device="cuda") -> torch.Tensor:
     """Zeroed int32 visit counters for the blocks schedule's instrumentation."""
     return torch.zeros(tile_count(width, height, 0), dtype=torch.int32, device=device)
+
+
+def tile_grid(width: int, height: int, *, schedule: str = "task", tile_rows: int = 0, tile_waves: int = 0):
+    """(tiles_y, tiles_x) of a call's tiling: the shape of its per-tile visit
+    counts (``round_per_blocks``)."""
+    p = _params(0.0, 1e10, 8, schedule, 0, tile_rows, 0, 0, tile_waves=tile_waves)
+    n = _lib.load().dtk_tile_count_for(width, height, ctypes.byref(p))
+    tx = -(-width // 62)
+    return (n // tx, tx) if n else (0, 0)
+
+
+def visits_buffer(width: int, height: int, device="cuda", **tiling) -> torch.Tensor:
+    """Zeroed int32 per-tile visit counters (tiles_y, tiles_x) for a call with
+    the given schedule / tile_rows / tile_waves; pass as round_per_blocks."""
+    return torch.zeros(tile_grid(width, height, **tiling), dtype=torch.int32, device=device)
 
 
 def _batch_arrays(imgs, masks, outs):

@@ -471,6 +471,14 @@ int dtk_tile_count(int width, int height, int tile_rows)
   return ((width + kTileW - 1) / kTileW) * ((height + rh - 3) / (rh - 2));
 }
 
+int dtk_tile_count_for(int width, int height, const dtk_params* p)
+{
+  if (!p || width <= 0 || height <= 0 || check_params(p) != DTK_OK)
+    return 0;
+  const int rh = region_rows(p), nw = tile_waves(p, rh);
+  return ((width + kTileW - 1) / kTileW) * ((height + nw * (rh - 2) - 1) / (nw * (rh - 2)));
+}
+
 int dtk_device_cu_count(void)
 {
   Workspace* w = nullptr;
@@ -501,8 +509,9 @@ dtk_status dtk_ggdt(const dtk_image* img, const dtk_image* mask, dtk_image* dist
 
   if (p->schedule == DTK_SCHED_RASTER)
   {
+    // Strips of a pass wait on each other, so all of them must be resident.
     const size_t words = raster_progress_words(d);
-    if ((long long)words > (long long)w->cu * 8)
+    if ((long long)words > (long long)w->cu * raster_occupancy(kp, c8))
       return fail(DTK_ERR_UNSUPPORTED, "image too large for the resident raster pipeline");
     if ((st = ws_reserve(*w, 4096 + words * 4, 4096)) != DTK_OK)
       return st;
@@ -584,6 +593,7 @@ dtk_status dtk_ggdt(const dtk_image* img, const dtk_image* mask, dtk_image* dist
   task_layout(w->dbuf, ntiles, cap, 1, &tb);
   if (cbytes)
     place_costs(&b.d0, 1, c8, static_cast<char*>(w->dbuf) + head);
+  tb.q.tile_visits = p->round_per_blocks;
   if ((st = enqueue_task(b, kp, tb.q, S, c8, two, workers, s)) != DTK_OK)
     return st;
   if (stats)

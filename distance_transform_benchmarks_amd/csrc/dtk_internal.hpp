@@ -131,6 +131,7 @@ namespace dtk
     uint32_t*           ctrl;
     float*              tmin; // per tile min / max of the initial map (seed selection)
     float*              tmax;
+    int*                tile_visits; // optional: visits per tile (instrumentation)
     uint32_t            cap_mask;
     int                 pad_;
   };

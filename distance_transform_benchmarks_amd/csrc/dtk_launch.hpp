@@ -24,6 +24,9 @@ namespace dtk
   hipError_t launch_raster_iteration(const ImgDesc& d, const KParams& p, bool c8, uint32_t* changed,
                                      uint32_t* progress, hipStream_t s);
   size_t     raster_progress_words(const ImgDesc& d);
+  // Strips (one 64-lane workgroup each) of one raster pass kernel that can be
+  // resident on one CU at once; every strip of a pass must be resident.
+  int        raster_occupancy(const KParams& p, bool c8);
 
   // dtk_adjoint.hip
   hipError_t launch_pred(const ImgDesc& d, const KParams& p, bool c8, hipStream_t s);

@@ -30,8 +30,14 @@ namespace dtk
 {
   namespace
   {
-    constexpr int kStripR = 32; // owned lines per strip
-    constexpr int kGhost  = 16; // ghost lines per side == steps per chunk
+#ifndef DTK_RASTER_GHOST
+#define DTK_RASTER_GHOST 16
+#endif
+    constexpr int kGhost  = DTK_RASTER_GHOST; // ghost lines per side == steps per chunk
+    constexpr int kStripR = 64 - 2 * kGhost;  // owned lines per strip
+    // A strip waits only for its two neighbours, so its ghost lines must lie
+    // within them.
+    static_assert(kGhost <= kStripR, "ghost lines must not reach beyond the neighbour strips");
   }
 
   __global__ void k_raster_init(ImgDesc d, KParams p, uint32_t* err)
@@ -109,36 +115,13 @@ namespace dtk
 
     for (int c = 0; c < nchunks; ++c)
     {
-      // Wait for the neighbour strips to finish chunk c-1.
-      if (c > 0)
-      {
-        if (lane == 0)
-        {
-          if (strip > 0)
-            wait_progress(progress, strip - 1, (uint32_t)c, changed);
-          if (strip + 1 < nstrips)
-            wait_progress(progress, strip + 1, (uint32_t)c, changed);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        __syncthreads();
-      }
       const int k0 = c * kGhost;
       const int kn = min(kGhost, len - k0);
 
-      // Previous step: owned lanes keep their registers, the others reload.
-      {
-        const int s = start + dir * (k0 - 1);
-        // Clamped unconditional loads, masked afterwards.
-        const int   cl = min(max(line, 0), nlines - 1);
-        const int   x = ROWS ? s : cl, y = ROWS ? cl : s;
-        const float dv = gload<float>(d.D, (long long)y * d.D_pitch + 4ll * x);
-        const uint32_t iv = img_at<KIND>(d, x, y);
-        if (c == 0 || !own)
-        {
-          Dprev = lin ? dv : DTK_INF;
-          Iprev = lin ? iv : 0u;
-        }
-      }
+      // The chunk's own steps are loaded before waiting: no other strip
+      // stores an owned line's pixels, and a ghost line read before or after
+      // its owner stored this chunk gives the same result (the owner's value
+      // is min(old value, the relaxations recomputed here)).
       float    Dv[kGhost];
       uint32_t Iv[kGhost];
       {
@@ -155,6 +138,35 @@ namespace dtk
         }
       }
 
+      // Wait for the neighbour strips to finish chunk c-1: the previous
+      // step's ghost values below must be final.
+      if (c > 0)
+      {
+        if (lane == 0)
+        {
+          if (strip > 0)
+            wait_progress(progress, strip - 1, (uint32_t)c, changed);
+          if (strip + 1 < nstrips)
+            wait_progress(progress, strip + 1, (uint32_t)c, changed);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __syncthreads();
+      }
+
+      // Previous step: owned lanes keep their registers, the others reload.
+      {
+        const int s = start + dir * (k0 - 1);
+        // Clamped unconditional loads, masked afterwards.
+        const int   cl = min(max(line, 0), nlines - 1);
+        const int   x = ROWS ? s : cl, y = ROWS ? cl : s;
+        const float dv = gload<float>(d.D, (long long)y * d.D_pitch + 4ll * x);
+        const uint32_t iv = img_at<KIND>(d, x, y);
+        if (c == 0 || !own)
+        {
+          Dprev = lin ? dv : DTK_INF;
+          Iprev = lin ? iv : 0u;
+        }
+      }
 #pragma unroll
       for (int j = 0; j < kGhost; ++j)
       {
@@ -205,7 +217,9 @@ namespace dtk
           {
             const int s = start + dir * (k0 + j);
             const int x = ROWS ? s : line, y = ROWS ? line : s;
-            d_at(d, x, y) = Dv[j];
+            // Device-scope store: written through, so the release fence
+            // below finds no dirty L2 lines to write back.
+            __hip_atomic_store(&d_at(d, x, y), Dv[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           }
         }
       }
@@ -271,6 +285,43 @@ namespace dtk
       return raster_iter_k<IMG_RGB8>(d, p, c8, changed, progress, s);
     default:
       return raster_iter_k<IMG_F32>(d, p, c8, changed, progress, s);
+    }
+  }
+
+  template <int KIND, bool C8, bool TWO>
+  static int raster_occ_t()
+  {
+    int n = 1 << 30, k = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, k_raster_pass<KIND, true, true, C8, TWO>, 64, 0) == hipSuccess)
+      n = min(n, k);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, k_raster_pass<KIND, true, false, C8, TWO>, 64, 0) == hipSuccess)
+      n = min(n, k);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, k_raster_pass<KIND, false, true, C8, TWO>, 64, 0) == hipSuccess)
+      n = min(n, k);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, k_raster_pass<KIND, false, false, C8, TWO>, 64, 0) == hipSuccess)
+      n = min(n, k);
+    return n == (1 << 30) ? 0 : n;
+  }
+
+  template <int KIND>
+  static int raster_occ_k(bool c8, bool two)
+  {
+    if (c8)
+      return two ? raster_occ_t<KIND, true, true>() : raster_occ_t<KIND, true, false>();
+    return two ? raster_occ_t<KIND, false, true>() : raster_occ_t<KIND, false, false>();
+  }
+
+  int raster_occupancy(const KParams& p, bool c8)
+  {
+    const bool two = !(p.zero_lambda || p.l_grad == 1.f);
+    switch (p.img_kind)
+    {
+    case IMG_U8:
+      return raster_occ_k<IMG_U8>(c8, two);
+    case IMG_RGB8:
+      return raster_occ_k<IMG_RGB8>(c8, two);
+    default:
+      return raster_occ_k<IMG_F32>(c8, two);
     }
   }
 

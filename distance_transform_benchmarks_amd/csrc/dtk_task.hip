@@ -355,6 +355,8 @@ namespace dtk
       resolve_tile(b, t, d, tx, ty);
       const VisitResult vr = visit_tile<S, C8, TWO, NW>(d, p, tx, ty, T[w], Sv[w], X, F);
       ++visits;
+      if (q.tile_visits && threadIdx.x == 0)
+        __hip_atomic_fetch_add(&q.tile_visits[t], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       sweeps += (uint32_t)vr.sweeps;
       t_setup += vr.cyc_setup;
       t_sweep += vr.cyc_sweep;

@@ -89,7 +89,8 @@ typedef struct
                             workspace plane (16 B/px C8, 8 B/px C4), 1 = recompute them on every visit */
   int   tile_waves;      /* task schedule: waves stacked per tile, 1, 2 or 4; a tile owns 62 x tile_waves *
                             (tile_rows - 2) pixels.  0 = library default (4 with 16-row, 2 with 32-row waves) */
-  int*  round_per_blocks; /* optional device int[dtk_tile_count()]: visits per tile (blocks schedule) */
+  int*  round_per_blocks; /* optional device int[dtk_tile_count_for()]: visits per tile (blocks and task
+                            schedules, single-image calls), added to the existing counts */
 } dtk_params;
 
 /* Filled by calls that report statistics (may be NULL). */
@@ -181,8 +182,16 @@ dtk_status dtk_ggdt_backward(const dtk_image* img, const dtk_image* mask, const 
                              const dtk_image* pred, const dtk_image* grad_dist, dtk_image* grad_img,
                              dtk_image* grad_mask, const dtk_params* p, void* stream);
 
-/* Tiles a call on a width x height image uses (size of round_per_blocks). */
+/* Tiles of the blocks schedule on a width x height image with single-wave
+ * regions of tile_rows rows (0 = 32): the size of round_per_blocks for
+ * dt::generalised_distance_transform_blocks
+ * (dt/src/generalised_distance_transform_blocks.cu:18-19). */
 int dtk_tile_count(int width, int height, int tile_rows);
+
+/* Tiles a call with parameters *p uses on a width x height image, for any
+ * schedule and tile shape: the size of round_per_blocks (visit counts per
+ * tile of the blocks and task schedules).  0 for invalid arguments. */
+int dtk_tile_count_for(int width, int height, const dtk_params* p);
 
 /* Pitched device allocation (replaces cudaMallocPitch in
  * dt/include/dt/details/image2d_data_gpu.hpp:28). */

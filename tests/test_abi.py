@@ -94,6 +94,17 @@ def test_tile_count_closed_form(lib, W, H, rows):
     assert lib.dtk_tile_count(W, H, rows) == -(-W // 62) * -(-H // (rh - 2))
 
 
+@pytest.mark.parametrize("sched,rows,waves,owned_rows", [(2, 0, 0, 56), (2, 32, 0, 60), (2, 16, 2, 28), (2, 32, 1, 30),
+                                                         (1, 0, 0, 30), (0, 16, 0, 14)])
+def test_tile_count_for_schedules(lib, sched, rows, waves, owned_rows):
+    from distance_transform_benchmarks_amd import _lib
+
+    p = _lib.default_params()
+    p.schedule, p.tile_rows, p.tile_waves = sched, rows, waves
+    W, H = 4096, 4096
+    assert lib.dtk_tile_count_for(W, H, ctypes.byref(p)) == -(-W // 62) * -(-H // owned_rows)
+
+
 def test_tile_count_rejects_bad_rows(lib):
     assert lib.dtk_tile_count(10, 10, 8) == 0
     assert lib.dtk_tile_count(0, 10, 0) == 0

@@ -367,6 +367,26 @@ def test_round_per_blocks(dt, dev):
     assert rpb.max().item() <= st["rounds"]
 
 
+def test_task_visit_counts(dt, dev, oracle):
+    """Per-tile visit counters of the task schedule add up to the visit count
+    of the call's statistics; the map is unchanged by counting."""
+    import torch
+
+    img, m = image("rgb8", 700, 500, 12), mask("f32", 700, 500, 12, 5)
+    want, _ = oracle.raster(img, m, 0.5, 1e3)
+    for tiling in (dict(), dict(tile_rows=32, tile_waves=2), dict(tile_rows=32, tile_waves=1)):
+        vis = dt.visits_buffer(700, 500, dev, **tiling)
+        _say("visit counts", tiling)
+        D, st = dt.ggdt(_to_dev(img, dev), _to_dev(m, dev), 0.5, 1e3, round_per_blocks=vis, stats=True, **tiling)
+        torch.cuda.synchronize()
+        assert vis.shape == (st["tiles"] // vis.shape[1], vis.shape[1])
+        assert int(vis.sum().item()) == st["visits"] and int(vis.min().item()) >= 0
+        assert_bit_exact(D.cpu().numpy(), want)
+    with pytest.raises(ValueError):
+        dt.ggdt(_to_dev(img, dev), _to_dev(m, dev), 0.5, 1e3, round_per_blocks=torch.zeros(3, dtype=torch.int32,
+                                                                                          device=dev))
+
+
 def test_cpp_api_gpu_suite():
     """The C++ test runner (reference gtest suites) on the device."""
     import subprocess
