@@ -183,6 +183,15 @@ namespace dtk
     return reinterpret_cast<const float*>(reinterpret_cast<const unsigned char*>(d.D) + (long long)y * d.D_pitch);
   }
 
+  // The transpose buffer T belongs to one wave, and the LDS operations of a
+  // wave execute in order, so a transpose needs no workgroup barrier: this
+  // only keeps the compiler from moving LDS accesses across it.
+  __device__ __forceinline__ void lds_wave_sync()
+  {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+
   // Column layout -> LDS -> row layout of one array.  k = 0..SW+1.
   template <int S>
   __device__ __forceinline__ void col_to_row(const float (&c)[64 / S], float (&rw)[64 / S + 2], float* T, int lane)
@@ -191,7 +200,7 @@ namespace dtk
 #pragma unroll
     for (int r = 0; r < RH; ++r)
       T[r * 65 + lane] = c[r];
-    __syncthreads();
+    lds_wave_sync();
     const int h = lane / RH, rr = lane % RH;
 #pragma unroll
     for (int k = 0; k < SW + 2; ++k)
@@ -201,7 +210,7 @@ namespace dtk
       const float v  = T[rr * 65 + (ok ? col : 0)];
       rw[k]          = ok ? v : DTK_INF;
     }
-    __syncthreads();
+    lds_wave_sync();
   }
 
   template <int S>
@@ -212,11 +221,11 @@ namespace dtk
 #pragma unroll
     for (int k = 1; k <= SW; ++k)
       T[rr * 65 + SW * h + k - 1] = rw[k];
-    __syncthreads();
+    lds_wave_sync();
 #pragma unroll
     for (int r = 0; r < RH; ++r)
       c[r] = T[r * 65 + lane];
-    __syncthreads();
+    lds_wave_sync();
   }
 
   // ---- sweeps --------------------------------------------------------------
