@@ -136,19 +136,23 @@ def round_per_blocks_buffer(width: int, height: int, device="cuda") -> torch.Ten
     return torch.zeros(tile_count(width, height, 0), dtype=torch.int32, device=device)
 
 
-def tile_grid(width: int, height: int, *, schedule: str = "task", tile_rows: int = 0, tile_waves: int = 0):
+def tile_grid(width: int, height: int, lambda_: float, *, schedule: str = "task", tile_rows: int = 0,
+              tile_waves: int = 0, max_sweeps: int = 0):
     """(tiles_y, tiles_x) of a call's tiling: the shape of its per-tile visit
-    counts (``round_per_blocks``)."""
-    p = _params(0.0, 1e10, 8, schedule, 0, tile_rows, 0, 0, tile_waves=tile_waves)
+    counts (``round_per_blocks``).  The task schedule's default tiling
+    depends on lambda_ and max_sweeps (lambda 0 without a sweep cap uses
+    taller tiles), so pass the call's values."""
+    p = _params(lambda_, 1e10, 8, schedule, 0, tile_rows, max_sweeps, 0, tile_waves=tile_waves)
     n = _lib.load().dtk_tile_count_for(width, height, ctypes.byref(p))
     tx = -(-width // 62)
     return (n // tx, tx) if n else (0, 0)
 
 
-def visits_buffer(width: int, height: int, device="cuda", **tiling) -> torch.Tensor:
+def visits_buffer(width: int, height: int, lambda_: float, device="cuda", **tiling) -> torch.Tensor:
     """Zeroed int32 per-tile visit counters (tiles_y, tiles_x) for a call with
-    the given schedule / tile_rows / tile_waves; pass as round_per_blocks."""
-    return torch.zeros(tile_grid(width, height, **tiling), dtype=torch.int32, device=device)
+    the given lambda_ and schedule / tile_rows / tile_waves / max_sweeps; pass
+    as round_per_blocks."""
+    return torch.zeros(tile_grid(width, height, lambda_, **tiling), dtype=torch.int32, device=device)
 
 
 def _batch_arrays(imgs, masks, outs):

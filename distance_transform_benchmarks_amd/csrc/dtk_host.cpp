@@ -226,15 +226,25 @@ namespace
     return DTK_OK;
   }
 
-  // Stacked waves per tile: the task schedule stacks two 32-row regions
-  // (a 62 x 60 owned block) unless told otherwise; other schedules use one.
+  // lambda 0 without a sweep cap: the tile kernels use constant chamfer
+  // costs and leave the pixels outside the image unmasked.  Those pixels
+  // restart at +inf on every visit and settle during it, so a visit must
+  // be free to run until it is quiet: under a sweep cap an edge tile could
+  // never finish a visit (and would re-queue itself forever), so capped
+  // calls keep the masked per-pixel costs.
+  bool const_costs(const dtk_params* p) { return p->lambda == 0.f && p->max_sweeps == 0; }
+
+  // Stacked waves per tile: the task schedule stacks four 16-row regions
+  // (62 x 56 owned pixels), four 32-row regions with constant costs
+  // (62 x 120) and two 32-row regions when 32 rows are asked for otherwise
+  // (62 x 60), unless told otherwise; other schedules use one.
   int tile_waves(const dtk_params* p, int rh)
   {
     if (p->schedule != DTK_SCHED_TASK)
       return 1;
     if (p->tile_waves != 0)
       return p->tile_waves;
-    return rh == 32 ? 2 : 4;
+    return rh == 32 && !const_costs(p) ? 2 : 4;
   }
 
   KParams make_kparams(const dtk_params* p, int img_dtype, int mask_dtype, int rh)
@@ -255,16 +265,27 @@ namespace
     return k;
   }
 
-  bool two_adds(const dtk_params* p) { return !(p->lambda == 0.f || p->lambda == 1.f); }
+  // Edge-cost mode of the tile kernels: 2 = constant chamfer costs, 0 = one
+  // add (lambda 1, or lambda 0 with masked costs), 1 = two adds.
+  int cost_mode(const dtk_params* p)
+  {
+    if (const_costs(p))
+      return 2;
+    return p->lambda == 0.f || p->lambda == 1.f ? 0 : 1;
+  }
 
-  // Region rows per wave: the task schedule defaults to 16-row waves stacked
-  // four high (62 x 56 owned pixels, two waves per SIMD), the blocks
-  // schedule to single 32-row waves (62 x 30).
+  // Region rows per wave: the task schedule defaults to 16-row waves (the
+  // per-pixel cost arrays leave room for two waves per SIMD) and to 32-row
+  // waves with constant costs (105 VGPRs); the blocks schedule to single
+  // 32-row waves (62 x 30).  Measured on 4096^2 RGB8: 16 x 4 is fastest at
+  // lambda 0.5 and 1, 32 x 4 at lambda 0.
   int region_rows(const dtk_params* p)
   {
     if (p->tile_rows != 0)
       return p->tile_rows;
-    return p->schedule == DTK_SCHED_TASK ? kDefaultTaskRows : 32;
+    if (p->schedule != DTK_SCHED_TASK)
+      return 32;
+    return const_costs(p) ? 32 : kDefaultTaskRows;
   }
 
   ImgDesc make_desc(const dtk_image* img, const dtk_image* mask, dtk_image* dist, int rh, int nw)
@@ -302,14 +323,19 @@ namespace
   }
 
   // Persistent workers (workgroups of nw waves) of a task-schedule call.
-  int task_workers(const Workspace& w, const dtk_params* p, int S, bool c8, bool two, int ntiles)
+  int task_workers(const Workspace& w, const dtk_params* p, int S, bool c8, int cm, int ntiles)
   {
     const int nw     = tile_waves(p, 64 / S);
-    int       per_cu = task_occupancy(S, c8, two, nw);
+    int       per_cu = task_occupancy(S, c8, cm, nw);
     if (per_cu <= 0)
       per_cu = 1;
     if (p->waves_per_cu > 0 && p->waves_per_cu / nw < per_cu)
       per_cu = std::max(1, p->waves_per_cu / nw);
+    // Four-wave workers: two per CU by default.  More (the lambda-0 kernels
+    // fit three or four) only add redundant visits: 4096^2 RGB8 at lambda 0
+    // takes 0.56 ms with two per CU and 0.60 ms with three.
+    if (p->waves_per_cu == 0 && nw == 4 && per_cu > 2)
+      per_cu = 2;
     long long n = (long long)per_cu * w.cu;
     if (n > ntiles)
       n = ntiles;
@@ -416,7 +442,7 @@ namespace
   }
 
   // Enqueue reset + init + seed + persistent solve for a prepared batch.
-  dtk_status enqueue_task(const Batch& b, const KParams& kp, const Queue& q, int S, bool c8, bool two, int workers,
+  dtk_status enqueue_task(const Batch& b, const KParams& kp, const Queue& q, int S, bool c8, int cm, int workers,
                           hipStream_t s, bool profile = true)
   {
     if (profile)
@@ -426,7 +452,7 @@ namespace
     DTK_HIP(launch_seed(b, q, s), "launch seed");
     if (profile)
       prof_mark(s);
-    DTK_HIP(launch_task(S, c8, two, b, kp, q, workers, s), "launch task");
+    DTK_HIP(launch_task(S, c8, cm, b, kp, q, workers, s), "launch task");
     if (profile)
       prof_mark(s);
     return DTK_OK;
@@ -502,7 +528,7 @@ dtk_status dtk_ggdt(const dtk_image* img, const dtk_image* mask, dtk_image* dist
     return st;
   std::lock_guard<std::mutex> lock(w->mu);
   const bool c8  = p->connectivity == DTK_C8;
-  const bool two = two_adds(p);
+  const int  cm  = cost_mode(p);
   const int  S   = 64 / rh;
   KParams    kp  = make_kparams(p, img->dtype, mask->dtype, rh);
   ImgDesc    d   = make_desc(img, mask, dist, rh, tile_waves(p, rh));
@@ -569,7 +595,7 @@ dtk_status dtk_ggdt(const dtk_image* img, const dtk_image* mask, dtk_image* dist
     for (;;)
     {
       DTK_HIP(hipMemsetAsync(q.ctrl + CTRL_CHANGED, 0, sizeof(uint32_t), s), "memset");
-      DTK_HIP(launch_blocks(S, c8, two, b, kb, a0, a1, q.ctrl, p->round_per_blocks, s), "launch blocks");
+      DTK_HIP(launch_blocks(S, c8, cm, b, kb, a0, a1, q.ctrl, p->round_per_blocks, s), "launch blocks");
       DTK_HIP(hipMemcpyAsync(h, q.ctrl, CTRL_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read flag");
       DTK_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
       ++rounds;
@@ -577,13 +603,18 @@ dtk_status dtk_ggdt(const dtk_image* img, const dtk_image* mask, dtk_image* dist
         return fail(DTK_ERR_INVALID_ARGUMENT, "soft mask values must be in [0, 1]");
       if (!h[CTRL_CHANGED])
         break;
+      // Each round with a wake lowers some stored pixel, so the loop ends;
+      // this bound (far above any measured round count) turns a protocol
+      // fault into an error instead of an endless loop.
+      if (rounds > 1000 + 64ll * ntiles)
+        return fail(DTK_ERR_NOT_CONVERGED, "blocks schedule: no convergence after " + std::to_string(rounds) + " rounds");
       std::swap(a0, a1);
     }
     return fill_stats(stats, h, rounds, ntiles, rh, 0);
   }
 
   // Task schedule.
-  const int      workers = task_workers(*w, p, S, c8, two, ntiles);
+  const int      workers = task_workers(*w, p, S, c8, cm, ntiles);
   const uint32_t cap     = next_pow2((uint32_t)(ntiles + workers + 1));
   TaskBufs       tb;
   const size_t   head   = align_up(task_layout(nullptr, ntiles, cap, 1, nullptr), 256);
@@ -594,7 +625,7 @@ dtk_status dtk_ggdt(const dtk_image* img, const dtk_image* mask, dtk_image* dist
   if (cbytes)
     place_costs(&b.d0, 1, c8, static_cast<char*>(w->dbuf) + head);
   tb.q.tile_visits = p->round_per_blocks;
-  if ((st = enqueue_task(b, kp, tb.q, S, c8, two, workers, s)) != DTK_OK)
+  if ((st = enqueue_task(b, kp, tb.q, S, c8, cm, workers, s)) != DTK_OK)
     return st;
   if (stats)
   {
@@ -632,9 +663,9 @@ dtk_status dtk_ggdt_batched(int n, const dtk_image* imgs, const dtk_image* masks
   std::lock_guard<std::mutex> lock(w->mu);
   hipStream_t    s   = static_cast<hipStream_t>(stream);
   const bool     c8  = p->connectivity == DTK_C8;
-  const bool     two = two_adds(p);
+  const int      cm  = cost_mode(p);
   const int      S   = 64 / rh;
-  const int      workers = task_workers(*w, p, S, c8, two, ntiles);
+  const int      workers = task_workers(*w, p, S, c8, cm, ntiles);
   const uint32_t cap     = next_pow2((uint32_t)(ntiles + workers + 1));
   const size_t   hbytes  = descs.size() * sizeof(ImgDesc) + tile_img.size() * 4 + 1024;
   const size_t   head    = align_up(task_layout(nullptr, ntiles, cap, n > 1 ? n : 2, nullptr), 256);
@@ -664,7 +695,7 @@ dtk_status dtk_ggdt_batched(int n, const dtk_image* imgs, const dtk_image* masks
     b.tile_img = tb.tile_img;
   }
   KParams kp = make_kparams(p, imgs[0].dtype, masks[0].dtype, rh);
-  if ((st = enqueue_task(b, kp, tb.q, S, c8, two, workers, s)) != DTK_OK)
+  if ((st = enqueue_task(b, kp, tb.q, S, c8, cm, workers, s)) != DTK_OK)
     return st;
   if (stats)
   {
@@ -712,9 +743,9 @@ dtk_status dtk_batch_plan_create(int n, const dtk_image* imgs, const dtk_image* 
   if ((st = current_ws(&w, &dev)) != DTK_OK)
     return st;
   const bool     c8      = p->connectivity == DTK_C8;
-  const bool     two     = two_adds(p);
+  const int      cm      = cost_mode(p);
   const int      S       = 64 / rh;
-  const int      workers = task_workers(*w, p, S, c8, two, ntiles);
+  const int      workers = task_workers(*w, p, S, c8, cm, ntiles);
   const uint32_t cap     = next_pow2((uint32_t)(ntiles + workers + 1));
   dtk_batch_plan* pl     = new dtk_batch_plan();
   pl->device             = dev;
@@ -754,7 +785,7 @@ dtk_status dtk_batch_plan_create(int n, const dtk_image* imgs, const dtk_image* 
   e          = hipStreamBeginCapture(pl->cstream, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess)
   {
-    st            = enqueue_task(b, kp, tb.q, S, c8, two, workers, pl->cstream, false);
+    st            = enqueue_task(b, kp, tb.q, S, c8, cm, workers, pl->cstream, false);
     hipError_t e2 = hipStreamEndCapture(pl->cstream, &pl->graph);
     if (st != DTK_OK)
     {
@@ -892,28 +923,38 @@ dtk_status dtk_batch_autotune(int n, const dtk_image* imgs, const dtk_image* mas
       return DTK_OK;
     }
   }
-  const bool c8  = p->connectivity == DTK_C8;
-  const bool two = two_adds(p);
+  const bool c8   = p->connectivity == DTK_C8;
   dtk_params cand = *p;
   dtk_params top  = *p;
   float      top_ms = 0.f, default_ms = 0.f;
   int        count  = 0;
-  const int shapes[5][2] = {{16, 4}, {32, 2}, {16, 2}, {32, 1}, {16, 1}}; // tile_rows, tile_waves
+  dtk_params dflt   = *p;
+  dflt.tile_rows = dflt.tile_waves = dflt.waves_per_cu = dflt.max_sweeps = 0;
+  const int drows = region_rows(&dflt), dnw = tile_waves(&dflt, drows);
+  const int shapes[6][2] = {{16, 4}, {32, 4}, {32, 2}, {16, 2}, {32, 1}, {16, 1}}; // tile_rows, tile_waves
   for (const auto& shp : shapes)
   {
     const int rows = shp[0], nw = shp[1];
-    const int occ  = std::max(1, task_occupancy(64 / rows, c8, two, nw)) * nw; // waves per CU
-    int       wpcs[3] = {occ, (occ * 3 / 4 + nw - 1) / nw * nw, (occ / 2 + nw - 1) / nw * nw};
-    for (int wi = 0; wi < 3; ++wi)
+    for (int cap : {0, 12})
     {
-      if (wi > 0 && wpcs[wi] == wpcs[wi - 1])
-        continue;
-      for (int cap : {0, 12})
+      cand.tile_rows  = rows;
+      cand.tile_waves = nw;
+      cand.max_sweeps = cap;
+      const int occ   = std::max(1, task_occupancy(64 / rows, c8, cost_mode(&cand), nw)); // workers per CU
+      // Worker counts per CU: the default (0), then all, 3/4 and 1/2 of
+      // what fits, each tried once.
+      const int wpcs[4] = {0, occ * nw, (occ * 3 + 3) / 4 * nw, (occ + 1) / 2 * nw};
+      int       seen[4] = {0, 0, 0, 0};
+      for (int wi = 0; wi < 4; ++wi)
       {
-        cand.tile_rows    = rows;
-        cand.tile_waves   = nw;
-        cand.waves_per_cu = wi == 0 ? 0 : wpcs[wi];
-        cand.max_sweeps   = cap;
+        const int per_cu = wi == 0 ? (nw == 4 ? std::min(occ, 2) : occ) : std::max(1, std::min(occ, wpcs[wi] / nw));
+        bool      dup    = false;
+        for (int k = 0; k < wi; ++k)
+          dup |= seen[k] == per_cu;
+        seen[wi] = per_cu;
+        if (dup)
+          continue;
+        cand.waves_per_cu = wpcs[wi];
         float ms          = 0.f;
         if ((st = time_plan(n, imgs, masks, dists, &cand, s, reps, &ms)) != DTK_OK)
           return st;
@@ -922,7 +963,7 @@ dtk_status dtk_batch_autotune(int n, const dtk_image* imgs, const dtk_image* mas
           top    = cand;
           top_ms = ms;
         }
-        if (rows == kDefaultTaskRows && nw == 4 && wi == 0 && cap == 0)
+        if (rows == drows && nw == dnw && wi == 0 && cap == 0)
           default_ms = ms;
         ++count;
       }

@@ -12,11 +12,11 @@ namespace dtk
   hipError_t launch_init(const Batch& b, const KParams& p, const Queue& q, hipStream_t s);
   hipError_t launch_seed(const Batch& b, const Queue& q, hipStream_t s);
   hipError_t launch_blocks_seed(const Batch& b, const Queue& q, uint8_t* active, hipStream_t s);
-  hipError_t launch_task(int S, bool c8, bool two, const Batch& b, const KParams& p, const Queue& q, int workers,
+  hipError_t launch_task(int S, bool c8, int cm, const Batch& b, const KParams& p, const Queue& q, int workers,
                          hipStream_t s);
-  hipError_t launch_blocks(int S, bool c8, bool two, const Batch& b, const KParams& p, uint8_t* active, uint8_t* next,
+  hipError_t launch_blocks(int S, bool c8, int cm, const Batch& b, const KParams& p, uint8_t* active, uint8_t* next,
                            uint32_t* ctrl, int* rpt, hipStream_t s);
-  int        task_occupancy(int S, bool c8, bool two, int nw);
+  int        task_occupancy(int S, bool c8, int cm, int nw);
 
   // dtk_raster.hip: one full raster iteration (four passes) with the exact
   // reference schedule; `changed` (device u32) is set when a pixel decreased.

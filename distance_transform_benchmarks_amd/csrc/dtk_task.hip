@@ -322,7 +322,7 @@ namespace dtk
 #define DTK_TASK_WAVES_PER_EU_16 2
 #endif
   // One workgroup of NW waves per worker; NW stacked waves solve one tile.
-  template <int S, bool C8, bool TWO, int NW>
+  template <int S, bool C8, int CM, int NW>
   __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(S == 4 ? DTK_TASK_WAVES_PER_EU_16 : 1)))
   k_task(Batch b, KParams p, Queue q)
   {
@@ -353,7 +353,7 @@ namespace dtk
       ImgDesc d;
       int     tx, ty;
       resolve_tile(b, t, d, tx, ty);
-      const VisitResult vr = visit_tile<S, C8, TWO, NW>(d, p, tx, ty, T[w], Sv[w], X, F);
+      const VisitResult vr = visit_tile<S, C8, CM, NW>(d, p, tx, ty, T[w], Sv[w], X, F);
       ++visits;
       if (q.tile_visits && threadIdx.x == 0)
         __hip_atomic_fetch_add(&q.tile_visits[t], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -395,7 +395,7 @@ namespace dtk
 
   // One host-driven round: every active tile solves to its local fixed point
   // and activates the neighbours whose halo it changed.
-  template <int S, bool C8, bool TWO>
+  template <int S, bool C8, int CM>
   __global__ void __launch_bounds__(64) k_blocks(Batch b, KParams p, uint8_t* active, uint8_t* next, uint32_t* ctrl,
                                                  int* rounds_per_tile)
   {
@@ -414,7 +414,7 @@ namespace dtk
     ImgDesc d;
     int     tx, ty;
     resolve_tile(b, t, d, tx, ty);
-    const VisitResult vr = visit_tile<S, C8, TWO, 1>(d, p, tx, ty, T, Sv, nullptr, nullptr);
+    const VisitResult vr = visit_tile<S, C8, CM, 1>(d, p, tx, ty, T, Sv, nullptr, nullptr);
     if (threadIdx.x == 0)
     {
       __hip_atomic_fetch_add(&ctrl[CTRL_VISITS], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -457,40 +457,40 @@ namespace dtk
   }
 
   // ---- host launchers --------------------------------------------------------
-  template <int S, bool C8, bool TWO>
+  template <int S, bool C8, int CM>
   static hipError_t launch_task_t(const Batch& b, const KParams& p, const Queue& q, int workers, hipStream_t s)
   {
     if (p.nw == 2)
     {
-      hipLaunchKernelGGL((k_task<S, C8, TWO, 2>), dim3(workers), dim3(128), 0, s, b, p, q);
+      hipLaunchKernelGGL((k_task<S, C8, CM, 2>), dim3(workers), dim3(128), 0, s, b, p, q);
       return hipGetLastError();
     }
     if (p.nw == 4)
     {
-      hipLaunchKernelGGL((k_task<S, C8, TWO, 4>), dim3(workers), dim3(256), 0, s, b, p, q);
+      hipLaunchKernelGGL((k_task<S, C8, CM, 4>), dim3(workers), dim3(256), 0, s, b, p, q);
       return hipGetLastError();
     }
-    hipLaunchKernelGGL((k_task<S, C8, TWO, 1>), dim3(workers), dim3(64), 0, s, b, p, q);
+    hipLaunchKernelGGL((k_task<S, C8, CM, 1>), dim3(workers), dim3(64), 0, s, b, p, q);
     return hipGetLastError();
   }
 
-  template <int S, bool C8, bool TWO>
+  template <int S, bool C8, int CM>
   static hipError_t launch_blocks_t(const Batch& b, const KParams& p, uint8_t* active, uint8_t* next, uint32_t* ctrl,
                                     int* rpt, hipStream_t s)
   {
-    hipLaunchKernelGGL((k_blocks<S, C8, TWO>), dim3(b.ntiles), dim3(64), 0, s, b, p, active, next, ctrl, rpt);
+    hipLaunchKernelGGL((k_blocks<S, C8, CM>), dim3(b.ntiles), dim3(64), 0, s, b, p, active, next, ctrl, rpt);
     return hipGetLastError();
   }
 
-  template <int S, bool C8, bool TWO>
+  template <int S, bool C8, int CM>
   static int occupancy_t(int nw)
   {
     int n = 0;
     if (nw == 2)
-      return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_task<S, C8, TWO, 2>, 128, 0) == hipSuccess ? n : 0;
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_task<S, C8, CM, 2>, 128, 0) == hipSuccess ? n : 0;
     if (nw == 4)
-      return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_task<S, C8, TWO, 4>, 256, 0) == hipSuccess ? n : 0;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_task<S, C8, TWO, 1>, 64, 0) == hipSuccess ? n : 0;
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_task<S, C8, CM, 4>, 256, 0) == hipSuccess ? n : 0;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_task<S, C8, CM, 1>, 64, 0) == hipSuccess ? n : 0;
   }
 
   hipError_t launch_reset_ctrl(uint32_t* ctrl, hipStream_t s)
@@ -517,48 +517,55 @@ namespace dtk
     return hipGetLastError();
   }
 
-  hipError_t launch_task(int S, bool c8, bool two, const Batch& b, const KParams& p, const Queue& q, int workers,
+  // Run F<S, C8, CM>() for the runtime (S, c8, cm); cm: 0 one add, 1 two
+  // adds, 2 constant lambda-0 costs.
+  template <template <int, bool, int> class F, class... A>
+  static auto dispatch(int S, bool c8, int cm, A&&... a)
+  {
+    if (S == 2)
+    {
+      if (c8)
+        return cm == 2 ? F<2, true, 2>::run(a...) : cm == 1 ? F<2, true, 1>::run(a...) : F<2, true, 0>::run(a...);
+      return cm == 2 ? F<2, false, 2>::run(a...) : cm == 1 ? F<2, false, 1>::run(a...) : F<2, false, 0>::run(a...);
+    }
+    if (c8)
+      return cm == 2 ? F<4, true, 2>::run(a...) : cm == 1 ? F<4, true, 1>::run(a...) : F<4, true, 0>::run(a...);
+    return cm == 2 ? F<4, false, 2>::run(a...) : cm == 1 ? F<4, false, 1>::run(a...) : F<4, false, 0>::run(a...);
+  }
+  template <int S, bool C8, int CM>
+  struct TaskLauncher
+  {
+    static hipError_t run(const Batch& b, const KParams& p, const Queue& q, int workers, hipStream_t s)
+    {
+      return launch_task_t<S, C8, CM>(b, p, q, workers, s);
+    }
+  };
+  template <int S, bool C8, int CM>
+  struct BlocksLauncher
+  {
+    static hipError_t run(const Batch& b, const KParams& p, uint8_t* active, uint8_t* next, uint32_t* ctrl, int* rpt,
+                          hipStream_t s)
+    {
+      return launch_blocks_t<S, C8, CM>(b, p, active, next, ctrl, rpt, s);
+    }
+  };
+  template <int S, bool C8, int CM>
+  struct Occupancy
+  {
+    static int run(int nw) { return occupancy_t<S, C8, CM>(nw); }
+  };
+
+  hipError_t launch_task(int S, bool c8, int cm, const Batch& b, const KParams& p, const Queue& q, int workers,
                          hipStream_t s)
   {
-    if (S == 2)
-    {
-      if (c8)
-        return two ? launch_task_t<2, true, true>(b, p, q, workers, s) : launch_task_t<2, true, false>(b, p, q, workers, s);
-      return two ? launch_task_t<2, false, true>(b, p, q, workers, s) : launch_task_t<2, false, false>(b, p, q, workers, s);
-    }
-    if (c8)
-      return two ? launch_task_t<4, true, true>(b, p, q, workers, s) : launch_task_t<4, true, false>(b, p, q, workers, s);
-    return two ? launch_task_t<4, false, true>(b, p, q, workers, s) : launch_task_t<4, false, false>(b, p, q, workers, s);
+    return dispatch<TaskLauncher>(S, c8, cm, b, p, q, workers, s);
   }
 
-  hipError_t launch_blocks(int S, bool c8, bool two, const Batch& b, const KParams& p, uint8_t* active, uint8_t* next,
+  hipError_t launch_blocks(int S, bool c8, int cm, const Batch& b, const KParams& p, uint8_t* active, uint8_t* next,
                            uint32_t* ctrl, int* rpt, hipStream_t s)
   {
-    if (S == 2)
-    {
-      if (c8)
-        return two ? launch_blocks_t<2, true, true>(b, p, active, next, ctrl, rpt, s)
-                   : launch_blocks_t<2, true, false>(b, p, active, next, ctrl, rpt, s);
-      return two ? launch_blocks_t<2, false, true>(b, p, active, next, ctrl, rpt, s)
-                 : launch_blocks_t<2, false, false>(b, p, active, next, ctrl, rpt, s);
-    }
-    if (c8)
-      return two ? launch_blocks_t<4, true, true>(b, p, active, next, ctrl, rpt, s)
-                 : launch_blocks_t<4, true, false>(b, p, active, next, ctrl, rpt, s);
-    return two ? launch_blocks_t<4, false, true>(b, p, active, next, ctrl, rpt, s)
-               : launch_blocks_t<4, false, false>(b, p, active, next, ctrl, rpt, s);
+    return dispatch<BlocksLauncher>(S, c8, cm, b, p, active, next, ctrl, rpt, s);
   }
 
-  int task_occupancy(int S, bool c8, bool two, int nw)
-  {
-    if (S == 2)
-    {
-      if (c8)
-        return two ? occupancy_t<2, true, true>(nw) : occupancy_t<2, true, false>(nw);
-      return two ? occupancy_t<2, false, true>(nw) : occupancy_t<2, false, false>(nw);
-    }
-    if (c8)
-      return two ? occupancy_t<4, true, true>(nw) : occupancy_t<4, true, false>(nw);
-    return two ? occupancy_t<4, false, true>(nw) : occupancy_t<4, false, false>(nw);
-  }
+  int task_occupancy(int S, bool c8, int cm, int nw) { return dispatch<Occupancy>(S, c8, cm, nw); }
 } // namespace dtk

@@ -40,6 +40,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "dtk_internal.hpp"
 
 namespace dtk
@@ -228,11 +230,48 @@ namespace dtk
     lds_wave_sync();
   }
 
+  // ---- edge costs seen by the sweeps ------------------------------------------
+  // Per-pixel costs of a region (column layout N / NW / NE, row layout
+  // W / NW / SW), +inf on edges that leave the image or the region.
+  template <int S>
+  struct TileCosts
+  {
+    static constexpr int RH = 64 / S, SW = 64 / S;
+    float                cN_[RH], cNW_[RH], cNE_[RH];
+    float                rW_[SW + 2], rNW_[SW + 2], rSW_[SW + 2];
+    __device__ __forceinline__ float cN(int r) const { return cN_[r]; }
+    __device__ __forceinline__ float cNW(int r) const { return cNW_[r]; }
+    __device__ __forceinline__ float cNE(int r) const { return cNE_[r]; }
+    __device__ __forceinline__ float rW(int k) const { return rW_[k]; }
+    __device__ __forceinline__ float rNW(int k) const { return rNW_[k]; }
+    __device__ __forceinline__ float rSW(int k) const { return rSW_[k]; }
+  };
+
+  // lambda == 0: every edge costs its Euclidean step, so a wave keeps two
+  // scalars and four per-lane values instead of six arrays (and runs more
+  // waves per SIMD).  Only edges that would take a value across the region's
+  // edge or between row segments are +inf: the wave-shift sources that do
+  // not exist read +inf (column sweeps) or are cut by rowNW / rowSW (row
+  // sweeps), and the region's outer slots hold +inf.  Pixels outside the
+  // image are not masked: with chamfer costs a path through them is longer
+  // by at least 2 - sqrt(2) than its projection on the image border, far
+  // beyond float32 rounding, so it never wins and never reaches the stored
+  // pixels (and the halo checks that wake neighbours skip them).
+  struct ConstCosts
+  {
+    float ea, ed, colNW, colNE, rowNW, rowSW;
+    __device__ __forceinline__ float cN(int) const { return ea; }
+    __device__ __forceinline__ float cNW(int) const { return colNW; }
+    __device__ __forceinline__ float cNE(int) const { return colNE; }
+    __device__ __forceinline__ float rW(int) const { return ea; }
+    __device__ __forceinline__ float rNW(int) const { return rowNW; }
+    __device__ __forceinline__ float rSW(int) const { return rowSW; }
+  };
+
   // ---- sweeps --------------------------------------------------------------
   // Column forward: row r pulls from row r-1 (dt/src/generalised_distance_transform.cu:59-99, Forward).
-  template <int RH, bool C8, bool TWO>
-  __device__ __forceinline__ uint64_t sweep_col_fwd(float (&D)[RH], const float (&cN)[RH], const float (&cNW)[RH],
-                                                    const float (&cNE)[RH], float ea, float ed)
+  template <int RH, bool C8, bool TWO, class CC>
+  __device__ __forceinline__ uint64_t sweep_col_fwd(float (&D)[RH], const CC& cc, float ea, float ed)
   {
     uint64_t ch = 0;
     __builtin_amdgcn_wave_barrier();
@@ -240,13 +279,13 @@ namespace dtk
     for (int r = 1; r < RH; ++r)
     {
       const float up = D[r - 1];
-      const float c  = (TWO ? up + ea : up) + cN[r];
+      const float c  = (TWO ? up + ea : up) + cc.cN(r);
       float       m  = c;
       if constexpr (C8)
       {
         const float t = TWO ? up + ed : up;
-        const float a = shr_z(t) + cNW[r];
-        const float b = shl_z(t) + cNE[r];
+        const float a = shr_z(t) + cc.cNW(r);
+        const float b = shl_z(t) + cc.cNE(r);
         m             = min3f(a, b, c);
       }
       ch |= ballot(lt_nn(m, D[r]));
@@ -257,9 +296,8 @@ namespace dtk
 
   // Column backward: row r receives from row r+1.  The diagonal candidates
   // are formed by the sender (its NW / NE cost) and shifted.
-  template <int RH, bool C8, bool TWO>
-  __device__ __forceinline__ uint64_t sweep_col_bwd(float (&D)[RH], const float (&cN)[RH], const float (&cNW)[RH],
-                                                    const float (&cNE)[RH], float ea, float ed)
+  template <int RH, bool C8, bool TWO, class CC>
+  __device__ __forceinline__ uint64_t sweep_col_bwd(float (&D)[RH], const CC& cc, float ea, float ed)
   {
     uint64_t ch = 0;
     __builtin_amdgcn_wave_barrier();
@@ -267,13 +305,13 @@ namespace dtk
     for (int r = RH - 2; r >= 0; --r)
     {
       const float dn = D[r + 1];
-      const float c  = (TWO ? dn + ea : dn) + cN[r + 1];
+      const float c  = (TWO ? dn + ea : dn) + cc.cN(r + 1);
       float       m  = c;
       if constexpr (C8)
       {
         const float t   = TWO ? dn + ed : dn;
-        const float pNW = t + cNW[r + 1]; // (i, r+1) -> (i-1, r)
-        const float pNE = t + cNE[r + 1]; // (i, r+1) -> (i+1, r)
+        const float pNW = t + cc.cNW(r + 1); // (i, r+1) -> (i-1, r)
+        const float pNE = t + cc.cNE(r + 1); // (i, r+1) -> (i+1, r)
         const float a   = shl_keep(D[r], pNW);
         const float b   = shr_keep(D[r], pNE);
         m               = min3f(a, b, c);
@@ -285,10 +323,8 @@ namespace dtk
   }
 
   // Row forward: slot k pulls from slot k-1 (dt/src/generalised_distance_transform.cu:17-57, Forward).
-  template <int SW, bool C8, bool TWO>
-  __device__ __forceinline__ uint64_t sweep_row_fwd(float (&R)[SW + 2], const float (&rW)[SW + 2],
-                                                    const float (&rNW)[SW + 2], const float (&rSW)[SW + 2], float ea,
-                                                    float ed)
+  template <int SW, bool C8, bool TWO, class CC>
+  __device__ __forceinline__ uint64_t sweep_row_fwd(float (&R)[SW + 2], const CC& cc, float ea, float ed)
   {
     uint64_t ch = 0;
     __builtin_amdgcn_wave_barrier();
@@ -296,13 +332,13 @@ namespace dtk
     for (int k = 1; k <= SW; ++k)
     {
       const float lf = R[k - 1];
-      const float c  = (TWO ? lf + ea : lf) + rW[k];
+      const float c  = (TWO ? lf + ea : lf) + cc.rW(k);
       float       m  = c;
       if constexpr (C8)
       {
         const float t = TWO ? lf + ed : lf;
-        const float a = shr_z(t) + rNW[k];
-        const float b = shl_z(t) + rSW[k];
+        const float a = shr_z(t) + cc.rNW(k);
+        const float b = shl_z(t) + cc.rSW(k);
         m             = min3f(a, b, c);
       }
       ch |= ballot(lt_nn(m, R[k]));
@@ -312,10 +348,8 @@ namespace dtk
   }
 
   // Row backward: slot k receives from slot k+1.
-  template <int SW, bool C8, bool TWO>
-  __device__ __forceinline__ uint64_t sweep_row_bwd(float (&R)[SW + 2], const float (&rW)[SW + 2],
-                                                    const float (&rNW)[SW + 2], const float (&rSW)[SW + 2], float ea,
-                                                    float ed)
+  template <int SW, bool C8, bool TWO, class CC>
+  __device__ __forceinline__ uint64_t sweep_row_bwd(float (&R)[SW + 2], const CC& cc, float ea, float ed)
   {
     uint64_t ch = 0;
     __builtin_amdgcn_wave_barrier();
@@ -323,13 +357,13 @@ namespace dtk
     for (int k = SW; k >= 1; --k)
     {
       const float rt = R[k + 1];
-      const float c  = (TWO ? rt + ea : rt) + rW[k + 1];
+      const float c  = (TWO ? rt + ea : rt) + cc.rW(k + 1);
       float       m  = c;
       if constexpr (C8)
       {
         const float t   = TWO ? rt + ed : rt;
-        const float pSW = t + rSW[k + 1]; // (c+1, r) -> (c, r+1)
-        const float pNW = t + rNW[k + 1]; // (c+1, r) -> (c, r-1)
+        const float pSW = t + cc.rSW(k + 1); // (c+1, r) -> (c, r+1)
+        const float pNW = t + cc.rNW(k + 1); // (c+1, r) -> (c, r-1)
         const float a   = shr_keep(R[k], pSW);
         const float b   = shl_keep(R[k], pNW);
         m               = min3f(a, b, c);
@@ -702,7 +736,10 @@ namespace dtk
   // regions, iterates to the tile's local fixed point (or max_sweeps),
   // writes back the owned pixels and reports which neighbours saw a border
   // pixel improve (border and changed are the same in every wave).
-  template <int S, bool C8, bool TWO, int NW>
+  // CM: 0 = one add per edge with per-pixel costs (lambda 1, or lambda 0
+  // with masked costs), 1 = two adds (0 < lambda < 1), 2 = lambda 0 with
+  // ConstCosts.
+  template <int S, bool C8, int CM, int NW>
   __device__ __attribute__((always_inline)) VisitResult visit_tile(const ImgDesc& d, const KParams& p, int tx, int ty,
                                                                    float* T, float* Sv, float* X, uint32_t* F)
   {
@@ -714,27 +751,38 @@ namespace dtk
     const int     Y0   = ty * (NW * (RH - 2)) - 1 + w * (RH - 2);
     const int     gx   = X0 + lane;
     const bool    cin  = gx >= 0 && gx < d.W;
+    constexpr bool TWO  = CM == 1;
     const uint64_t clk0 = __builtin_amdgcn_s_memtime();
 
-    float cN[RH], cNW[RH], cNE[RH];
-    float rW[SW + 2], rNW[SW + 2], rSW[SW + 2];
-    if (p.zero_lambda)
-      setup_costs_const<S, C8>(d, p, X0, Y0, lane, T, cN, cNW, cNE, rW, rNW, rSW);
-    else if (d.cost)
-      setup_costs_cached<S, C8>(d, X0, Y0, lane, T, cN, cNW, cNE, rW, rNW, rSW);
-    else
-    switch (p.img_kind)
+    using Costs = std::conditional_t<CM == 2, ConstCosts, TileCosts<S>>;
+    Costs cc;
+    if constexpr (CM == 2)
     {
-    case IMG_U8:
-      setup_costs<IMG_U8, S, C8>(d, p, X0, Y0, lane, T, cN, cNW, cNE, rW, rNW, rSW);
-      break;
-    case IMG_RGB8:
-      setup_costs<IMG_RGB8, S, C8>(d, p, X0, Y0, lane, T, cN, cNW, cNE, rW, rNW, rSW);
-      break;
-    default:
-      setup_costs<IMG_F32, S, C8>(d, p, X0, Y0, lane, T, cN, cNW, cNE, rW, rNW, rSW);
-      break;
+      const int rr = lane % RH;
+      cc.ea        = p.e_axis;
+      cc.ed        = p.e_diag;
+      cc.colNW     = (C8 && lane != 0) ? p.e_diag : DTK_INF;
+      cc.colNE     = (C8 && lane != 63) ? p.e_diag : DTK_INF;
+      cc.rowNW     = (C8 && rr != 0) ? p.e_diag : DTK_INF;
+      cc.rowSW     = (C8 && rr != RH - 1) ? p.e_diag : DTK_INF;
     }
+    else if (p.zero_lambda)
+      setup_costs_const<S, C8>(d, p, X0, Y0, lane, T, cc.cN_, cc.cNW_, cc.cNE_, cc.rW_, cc.rNW_, cc.rSW_);
+    else if (d.cost)
+      setup_costs_cached<S, C8>(d, X0, Y0, lane, T, cc.cN_, cc.cNW_, cc.cNE_, cc.rW_, cc.rNW_, cc.rSW_);
+    else
+      switch (p.img_kind)
+      {
+      case IMG_U8:
+        setup_costs<IMG_U8, S, C8>(d, p, X0, Y0, lane, T, cc.cN_, cc.cNW_, cc.cNE_, cc.rW_, cc.rNW_, cc.rSW_);
+        break;
+      case IMG_RGB8:
+        setup_costs<IMG_RGB8, S, C8>(d, p, X0, Y0, lane, T, cc.cN_, cc.cNW_, cc.cNE_, cc.rW_, cc.rNW_, cc.rSW_);
+        break;
+      default:
+        setup_costs<IMG_F32, S, C8>(d, p, X0, Y0, lane, T, cc.cN_, cc.cNW_, cc.cNE_, cc.rW_, cc.rNW_, cc.rSW_);
+        break;
+      }
 
     float     Dc[RH];
     const int cx = min(max(gx, 0), d.W - 1);
@@ -772,7 +820,7 @@ namespace dtk
     {
       for (;;)
       {
-        uint64_t ch = sweep_col_fwd<RH, C8, TWO>(Dc, cN, cNW, cNE, ea, ed);
+        uint64_t ch = sweep_col_fwd<RH, C8, TWO>(Dc, cc, ea, ed);
         ++sweeps;
         any |= ch != 0;
         quiet = ch ? 0 : quiet + 1;
@@ -783,7 +831,7 @@ namespace dtk
         }
         if (sweeps >= cap)
           break;
-        ch = sweep_col_bwd<RH, C8, TWO>(Dc, cN, cNW, cNE, ea, ed);
+        ch = sweep_col_bwd<RH, C8, TWO>(Dc, cc, ea, ed);
         ++sweeps;
         any |= ch != 0;
         quiet = ch ? 0 : quiet + 1;
@@ -796,7 +844,7 @@ namespace dtk
           break;
 
         col_to_row<S>(Dc, R, T, lane);
-        const uint64_t cf = sweep_row_fwd<SW, C8, TWO>(R, rW, rNW, rSW, ea, ed);
+        const uint64_t cf = sweep_row_fwd<SW, C8, TWO>(R, cc, ea, ed);
         ++sweeps;
         any |= cf != 0;
         quiet = cf ? 0 : quiet + 1;
@@ -806,7 +854,7 @@ namespace dtk
           row_to_col<S>(R, Dc, T, lane);
           break;
         }
-        const uint64_t cb = sweep_row_bwd<SW, C8, TWO>(R, rW, rNW, rSW, ea, ed);
+        const uint64_t cb = sweep_row_bwd<SW, C8, TWO>(R, cc, ea, ed);
         ++sweeps;
         any |= cb != 0;
         quiet = (cb | cf) ? 0 : quiet + 1;
@@ -828,8 +876,8 @@ namespace dtk
       int par = 0;
       for (;;)
       {
-        uint64_t ch = sweep_col_fwd<RH, C8, TWO>(Dc, cN, cNW, cNE, ea, ed);
-        ch |= sweep_col_bwd<RH, C8, TWO>(Dc, cN, cNW, cNE, ea, ed);
+        uint64_t ch = sweep_col_fwd<RH, C8, TWO>(Dc, cc, ea, ed);
+        ch |= sweep_col_bwd<RH, C8, TWO>(Dc, cc, ea, ed);
         sweeps += 2;
         any |= ch != 0;
         bool wg = exchange_rows<RH, NW>(Dc, X, F, w, lane, par, ch != 0, any);
@@ -843,8 +891,8 @@ namespace dtk
         if (sweeps >= cap)
           break;
         col_to_row<S>(Dc, R, T, lane);
-        ch = sweep_row_fwd<SW, C8, TWO>(R, rW, rNW, rSW, ea, ed);
-        ch |= sweep_row_bwd<SW, C8, TWO>(R, rW, rNW, rSW, ea, ed);
+        ch = sweep_row_fwd<SW, C8, TWO>(R, cc, ea, ed);
+        ch |= sweep_row_bwd<SW, C8, TWO>(R, cc, ea, ed);
         row_to_col<S>(R, Dc, T, lane);
         sweeps += 2;
         any |= ch != 0;
@@ -870,16 +918,19 @@ namespace dtk
     uint32_t b    = 0;
     if (any)
     {
+      // Only pixels inside the image count: with constant costs the pixels
+      // outside it start at +inf on every visit and always decrease, which
+      // would wake the neighbours (and through them this tile) forever.
       bool colimp = false;
-      if (lane == 0 || lane == 63)
+      if ((lane == 0 || lane == 63) && cin)
       {
         const int o = (lane == 63) ? RH : 0;
 #pragma unroll
         for (int r = 1; r < RH - 1; ++r)
-          colimp |= lt_nn(Dc[r], Sv[o + r]);
+          colimp |= Y0 + r < d.H && lt_nn(Dc[r], Sv[o + r]);
       }
-      const uint64_t top  = ballot(lt_nn(Dc[0], init_top));
-      const uint64_t bot  = ballot(lt_nn(Dc[RH - 1], init_bot));
+      const uint64_t top  = ballot(cin && Y0 >= 0 && lt_nn(Dc[0], init_top));
+      const uint64_t bot  = ballot(cin && Y0 + RH - 1 < d.H && lt_nn(Dc[RH - 1], init_bot));
       const uint64_t cols = ballot(colimp);
       const uint64_t mid  = 0x7FFFFFFFFFFFFFFEull; // lanes 1..62
       b |= (uint32_t)(cols & 1u);                 // L : left halo column

@@ -94,13 +94,16 @@ def test_tile_count_closed_form(lib, W, H, rows):
     assert lib.dtk_tile_count(W, H, rows) == -(-W // 62) * -(-H // (rh - 2))
 
 
-@pytest.mark.parametrize("sched,rows,waves,owned_rows", [(2, 0, 0, 56), (2, 32, 0, 60), (2, 16, 2, 28), (2, 32, 1, 30),
-                                                         (1, 0, 0, 30), (0, 16, 0, 14)])
-def test_tile_count_for_schedules(lib, sched, rows, waves, owned_rows):
+@pytest.mark.parametrize("sched,lam,rows,waves,owned_rows", [
+    (2, 0.5, 0, 0, 56), (2, 0.0, 0, 0, 120), (2, 0.5, 32, 0, 60), (2, 0.0, 32, 0, 120), (2, 0.0, 16, 0, 56),
+    (2, 0.5, 16, 2, 28), (2, 0.0, 32, 1, 30), (1, 0.0, 0, 0, 30), (1, 0.5, 0, 0, 30), (0, 0.5, 16, 0, 14)])
+def test_tile_count_for_schedules(lib, sched, lam, rows, waves, owned_rows):
+    """Default tilings: task 16-row waves stacked four high (lambda > 0) or
+    32-row waves stacked four high (lambda 0); blocks single 32-row waves."""
     from distance_transform_benchmarks_amd import _lib
 
     p = _lib.default_params()
-    p.schedule, p.tile_rows, p.tile_waves = sched, rows, waves
+    p.schedule, p.lambda_, p.tile_rows, p.tile_waves = sched, lam, rows, waves
     W, H = 4096, 4096
     assert lib.dtk_tile_count_for(W, H, ctypes.byref(p)) == -(-W // 62) * -(-H // owned_rows)
 
