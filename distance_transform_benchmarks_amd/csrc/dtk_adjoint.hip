@@ -216,8 +216,11 @@ namespace dtk
     const float a    = shr_z(v);           // expect lane     (value of lane-1), 0 at lane 0
     const float b    = shl_z(v);           // expect lane + 2 (value of lane+1), 0 at lane 63
     const float c    = shr_keep(-1.f, v);  // lane 0 keeps -1
+    const float rl   = rol_f(v);           // value of lane (lane+1) % 64
+    const float rr   = ror_f(v);           // value of lane (lane+63) % 64
     const bool  ok   = a == (lane == 0 ? 0.f : (float)lane) && b == (lane == 63 ? 0.f : (float)(lane + 2)) &&
-                    c == (lane == 0 ? -1.f : (float)lane);
+                    c == (lane == 0 ? -1.f : (float)lane) && rl == (float)((lane + 1) % 64 + 1) &&
+                    rr == (float)((lane + 63) % 64 + 1);
     // Integer shifts feeding an integer subtract (the uint8 cost pattern).
     const uint32_t u  = (uint32_t)(lane * 37 % 256);
     const uint32_t ul = shl_u(u), ur = shr_u(u);

@@ -67,6 +67,19 @@ namespace dtk
   {
     return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x130, 0xF, 0xF, false));
   }
+  // Wave rotates by one lane: rol gives lane i the value of lane i+1 (lane
+  // 63 that of lane 0), ror that of lane i-1 (lane 0 that of lane 63).  The
+  // backward sweeps use them where the sending edge lane always holds +inf,
+  // so the wrapped value is a non-candidate, as a kept old value would be,
+  // without the copy that a keep-old shift needs.
+  __device__ __forceinline__ float rol_f(float v)
+  {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, false));
+  }
+  __device__ __forceinline__ float ror_f(float v)
+  {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x13C, 0xF, 0xF, false));
+  }
   // Integer wave shifts.  The result is copied through an opaque VALU move so
   // the DPP is never folded into an integer VOP2 instruction: measured on
   // gfx950, a folded v_subrev_u32_dpp with wave_shl:1 produced wrong values,
@@ -310,10 +323,10 @@ namespace dtk
       if constexpr (C8)
       {
         const float t   = TWO ? dn + ed : dn;
-        const float pNW = t + cc.cNW(r + 1); // (i, r+1) -> (i-1, r)
-        const float pNE = t + cc.cNE(r + 1); // (i, r+1) -> (i+1, r)
-        const float a   = shl_keep(D[r], pNW);
-        const float b   = shr_keep(D[r], pNE);
+        const float pNW = t + cc.cNW(r + 1); // (i, r+1) -> (i-1, r); +inf at lane 0
+        const float pNE = t + cc.cNE(r + 1); // (i, r+1) -> (i+1, r); +inf at lane 63
+        const float a   = rol_f(pNW);
+        const float b   = ror_f(pNE);
         m               = min3f(a, b, c);
       }
       ch |= ballot(lt_nn(m, D[r]));
@@ -362,10 +375,10 @@ namespace dtk
       if constexpr (C8)
       {
         const float t   = TWO ? rt + ed : rt;
-        const float pSW = t + cc.rSW(k + 1); // (c+1, r) -> (c, r+1)
-        const float pNW = t + cc.rNW(k + 1); // (c+1, r) -> (c, r-1)
-        const float a   = shr_keep(R[k], pSW);
-        const float b   = shl_keep(R[k], pNW);
+        const float pSW = t + cc.rSW(k + 1); // (c+1, r) -> (c, r+1); +inf in a segment's last row (lane 63)
+        const float pNW = t + cc.rNW(k + 1); // (c+1, r) -> (c, r-1); +inf in a segment's row 0 (lane 0)
+        const float a   = ror_f(pSW);
+        const float b   = rol_f(pNW);
         m               = min3f(a, b, c);
       }
       ch |= ballot(lt_nn(m, R[k]));
