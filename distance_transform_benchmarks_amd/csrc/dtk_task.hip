@@ -62,39 +62,43 @@ namespace dtk
   {
     return (uint32_t)(__hip_atomic_load(tp_word(q), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 32);
   }
-  // One wave per tile: D0 of the owned pixels, tile min / max of D0.
+  // k_init: one workgroup of kInitWaves waves per tile.  Each wave takes a
+  // contiguous share of the tile's owned rows in groups of kInitRows rows
+  // whose loads are all issued before the first use, so a tile costs a few
+  // memory latencies instead of one per row.
+  constexpr int kInitWaves = 4;
+  constexpr int kInitRows  = 8;
+
+  // D0 of rows r0 .. r0+n-1 of the tile (n <= kInitRows, wave-uniform).
   // D0 = v * (mask == 0) for a uint8 mask (mask != 0 marks a seed), or
   // nu * (1 - s) for a soft float mask s in [0, 1].
-  __global__ void __launch_bounds__(64) k_init(Batch b, KParams p, Queue q)
+  __device__ __forceinline__ void init_d0_group(const ImgDesc& d, const KParams& p, int X0, int Y0, int r0, int n,
+                                                int lane, float& mn, float& mx, uint32_t& err)
   {
-    const int t    = blockIdx.x;
-    const int lane = threadIdx.x;
-    ImgDesc   d;
-    int       tx, ty;
-    resolve_tile(b, t, d, tx, ty);
-    const int  RH  = p.nw * (p.rh - 2) + 2; // rows of the tile with its halo
-    const int  X0  = tx * kTileW - 1;
-    const int  Y0  = ty * (RH - 2) - 1;
     const int  gx  = X0 + lane;
     const bool own = lane >= 1 && lane <= kTileW && gx < d.W;
     const int  cx  = min(max(gx, 0), d.W - 1); // in-bounds address for every lane
-    float      mn = DTK_INF, mx = -DTK_INF;
-    uint32_t   err = 0;
-    // Unconditional clamped loads (all rows in flight), masked stores.
-    for (int r = 1; r < RH - 1; ++r)
+    float      m[kInitRows];
+#pragma unroll
+    for (int k = 0; k < kInitRows; ++k)
     {
-      const int  gy = Y0 + r;
-      const bool ok = own && gy < d.H;
-      const int  cy = min(max(gy, 0), d.H - 1);
+      const int cy = min(max(Y0 + r0 + k, 0), d.H - 1);
+      m[k]         = p.mask_kind == MASK_U8 ? (float)gload<unsigned char>(d.mask, (long long)cy * d.mask_pitch + cx)
+                                            : gload<float>(d.mask, (long long)cy * d.mask_pitch + 4ll * cx);
+    }
+#pragma unroll
+    for (int k = 0; k < kInitRows; ++k)
+    {
+      const int  gy = Y0 + r0 + k;
+      const bool ok = own && k < n && gy < d.H;
       float      d0;
       if (p.mask_kind == MASK_U8)
-        d0 = p.v * (float)(gload<unsigned char>(d.mask, (long long)cy * d.mask_pitch + cx) == 0);
+        d0 = p.v * (float)(m[k] == 0.f);
       else
       {
-        const float s = gload<float>(d.mask, (long long)cy * d.mask_pitch + 4ll * cx);
-        if (ok && !(s >= 0.f && s <= 1.f))
+        if (ok && !(m[k] >= 0.f && m[k] <= 1.f))
           err |= ERR_MASK_RANGE;
-        d0 = p.v * (1.f - s);
+        d0 = p.v * (1.f - m[k]);
       }
       if (ok)
       {
@@ -103,37 +107,78 @@ namespace dtk
         mx = fmaxf(mx, d0);
       }
     }
-    // Cached edge costs of the owned pixels (read back by every visit).
-    if (d.cost)
+  }
+
+  template <int KIND, bool C8>
+  __device__ void init_tile_rows(const ImgDesc& d, const KParams& p, int X0, int Y0, int r_begin, int r_end, int lane,
+                                 float& mn, float& mx, uint32_t& err)
+  {
+    for (int r0 = r_begin; r0 < r_end; r0 += kInitRows)
     {
-      switch (p.img_kind * 2 + (p.c8 ? 1 : 0))
-      {
-      case IMG_U8 * 2:
-        write_cost_rows<IMG_U8, false>(d, p, X0, Y0, RH, lane);
-        break;
-      case IMG_U8 * 2 + 1:
-        write_cost_rows<IMG_U8, true>(d, p, X0, Y0, RH, lane);
-        break;
-      case IMG_RGB8 * 2:
-        write_cost_rows<IMG_RGB8, false>(d, p, X0, Y0, RH, lane);
-        break;
-      case IMG_RGB8 * 2 + 1:
-        write_cost_rows<IMG_RGB8, true>(d, p, X0, Y0, RH, lane);
-        break;
-      case IMG_F32 * 2:
-        write_cost_rows<IMG_F32, false>(d, p, X0, Y0, RH, lane);
-        break;
-      default:
-        write_cost_rows<IMG_F32, true>(d, p, X0, Y0, RH, lane);
-        break;
-      }
+      const int n = min(kInitRows, r_end - r0);
+      init_d0_group(d, p, X0, Y0, r0, n, lane, mn, mx, err);
+      if (d.cost)
+        write_cost_group<KIND, C8, kInitRows>(d, p, X0, Y0, r0, n, lane);
+    }
+  }
+
+  // D0 and (when the call caches them) the edge costs of the owned pixels,
+  // the tile's min / max of D0, and the queue reset.
+  __global__ void __launch_bounds__(64 * kInitWaves) k_init(Batch b, KParams p, Queue q)
+  {
+    __shared__ float s_mn[kInitWaves], s_mx[kInitWaves];
+    const int        t    = blockIdx.x;
+    const int        lane = threadIdx.x & 63;
+    const int        wv   = threadIdx.x >> 6;
+    ImgDesc          d;
+    int              tx, ty;
+    resolve_tile(b, t, d, tx, ty);
+    const int owned   = p.nw * (p.rh - 2); // owned rows of the tile
+    const int X0      = tx * kTileW - 1;
+    const int Y0      = ty * owned - 1;
+    const int chunk   = (owned + kInitWaves - 1) / kInitWaves;
+    const int r_begin = 1 + wv * chunk;
+    const int r_end   = min(1 + owned, r_begin + chunk);
+    float     mn = DTK_INF, mx = -DTK_INF;
+    uint32_t  err = 0;
+    switch (p.img_kind * 2 + (p.c8 ? 1 : 0))
+    {
+    case IMG_U8 * 2:
+      init_tile_rows<IMG_U8, false>(d, p, X0, Y0, r_begin, r_end, lane, mn, mx, err);
+      break;
+    case IMG_U8 * 2 + 1:
+      init_tile_rows<IMG_U8, true>(d, p, X0, Y0, r_begin, r_end, lane, mn, mx, err);
+      break;
+    case IMG_RGB8 * 2:
+      init_tile_rows<IMG_RGB8, false>(d, p, X0, Y0, r_begin, r_end, lane, mn, mx, err);
+      break;
+    case IMG_RGB8 * 2 + 1:
+      init_tile_rows<IMG_RGB8, true>(d, p, X0, Y0, r_begin, r_end, lane, mn, mx, err);
+      break;
+    case IMG_F32 * 2:
+      init_tile_rows<IMG_F32, false>(d, p, X0, Y0, r_begin, r_end, lane, mn, mx, err);
+      break;
+    default:
+      init_tile_rows<IMG_F32, true>(d, p, X0, Y0, r_begin, r_end, lane, mn, mx, err);
+      break;
     }
     mn = wave_min(mn);
     mx = wave_max(mx);
+    if (lane == 0)
+    {
+      s_mn[wv] = mn;
+      s_mx[wv] = mx;
+    }
     if (err)
       __hip_atomic_fetch_or(&q.ctrl[CTRL_ERROR], err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (lane == 0 && q.tmin)
+    __syncthreads();
+    if (threadIdx.x == 0 && q.tmin)
     {
+      for (int k = 1; k < kInitWaves; ++k)
+      {
+        mn = fminf(mn, s_mn[k]);
+        mx = fmaxf(mx, s_mx[k]);
+      }
       q.tmin[t] = mn;
       q.tmax[t] = mx;
     }
@@ -141,9 +186,9 @@ namespace dtk
     if (q.slots)
     {
       const uint32_t cap = q.cap_mask + 1u;
-      for (uint32_t k = (uint32_t)t * 64u + lane; k < cap; k += gridDim.x * 64u)
+      for (uint32_t k = (uint32_t)t * blockDim.x + threadIdx.x; k < cap; k += gridDim.x * blockDim.x)
         q.slots[(size_t)k * kSlotStride] = (unsigned long long)(2u * k) << 32;
-      if (lane == 0)
+      if (threadIdx.x == 0)
         q.state[(size_t)t * kStateStride] = 0;
     }
   }
@@ -515,7 +560,7 @@ namespace dtk
 
   hipError_t launch_init(const Batch& b, const KParams& p, const Queue& q, hipStream_t s)
   {
-    hipLaunchKernelGGL(k_init, dim3(b.ntiles), dim3(64), 0, s, b, p, q);
+    hipLaunchKernelGGL(k_init, dim3(b.ntiles), dim3(64 * kInitWaves), 0, s, b, p, q);
     return hipGetLastError();
   }
 

@@ -508,28 +508,37 @@ namespace dtk
   typedef float f2 __attribute__((ext_vector_type(2)));
   typedef __attribute__((address_space(1))) unsigned char* gbytes_w;
 
-  // Writes the cost record of the pixels of one wave-wide row segment:
-  // column gx = X0 + lane for lanes 1..62 (lanes 0 and 63 only supply
-  // neighbour values), rows Y0+1 .. Y0+RH-2.  Record = {W, N, NW, NE} (C8)
-  // or {W, N} (C4), each the cost of the edge to that neighbour as
-  // edge_cost() defines it, +inf when the neighbour is outside the image.
-  // Edge costs are symmetric (|a - b| and the squared colour distance are),
-  // so the solver reads the cost of (p, q) from either end.
-  template <int KIND, bool C8>
-  __device__ __forceinline__ void write_cost_rows(const ImgDesc& d, const KParams& p, int X0, int Y0, int RH,
-                                                  int lane)
+  // Writes the cost records of rows r0 .. r0+n-1 of a tile (n <= NR,
+  // wave-uniform) for column gx = X0 + lane, lanes 1..62 (lanes 0 and 63
+  // only supply neighbour values).  The NR+1 image rows (with the one above)
+  // are loaded before any is used.  Record = {W, N, NW, NE} (C8) or {W, N}
+  // (C4), each the cost of the edge to that neighbour as edge_cost()
+  // defines it, +inf when the neighbour is outside the image.  Edge costs
+  // are symmetric (|a - b| and the squared colour distance are), so the
+  // solver reads the cost of (p, q) from either end.
+  template <int KIND, bool C8, int NR>
+  __device__ __forceinline__ void write_cost_group(const ImgDesc& d, const KParams& p, int X0, int Y0, int r0, int n,
+                                                   int lane)
   {
     const int  gx  = X0 + lane;
     const int  cx  = min(max(gx, 0), d.W - 1);
     const bool own = lane >= 1 && lane <= kTileW && gx < d.W;
     const bool inL = gx - 1 >= 0;
     const bool inR = gx + 1 < d.W;
-    uint32_t   up  = load_pix<KIND>(d.img + (long long)min(max(Y0, 0), d.H - 1) * d.img_pitch, cx);
-    for (int r = 1; r < RH - 1; ++r)
+    uint32_t   I[NR + 1];
+#pragma unroll
+    for (int k = 0; k <= NR; ++k)
     {
-      const int      gy   = Y0 + r;
-      const int      cy   = min(max(gy, 0), d.H - 1);
-      const uint32_t cur  = load_pix<KIND>(d.img + (long long)cy * d.img_pitch, cx);
+      const int cy = min(max(Y0 + r0 - 1 + k, 0), d.H - 1);
+      I[k]         = load_pix<KIND>(d.img + (long long)cy * d.img_pitch, cx);
+    }
+#pragma unroll
+    for (int k = 1; k <= NR; ++k)
+    {
+      if (k > n)
+        break;
+      const int      gy   = Y0 + r0 - 1 + k;
+      const uint32_t cur  = I[k], up = I[k - 1];
       const bool     rinU = gy - 1 >= 0;
       const float    cw   = edge_cost<KIND>(inL, cur, shr_u(cur), 0.f, p);
       const float    cn   = edge_cost<KIND>(rinU, cur, up, 0.f, p);
@@ -550,7 +559,6 @@ namespace dtk
         *reinterpret_cast<__attribute__((address_space(1))) f2*>((gbytes_w)d.cost + (long long)gy * d.cost_pitch +
                                                                  8ll * gx) = v;
       }
-      up = cur;
     }
   }
 
