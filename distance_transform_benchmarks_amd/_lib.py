@@ -40,7 +40,7 @@ class DtkImage(ctypes.Structure):
 class DtkParams(ctypes.Structure):
     _fields_ = [("lambda_", ctypes.c_float), ("v", ctypes.c_float), ("connectivity", ctypes.c_int),
                 ("schedule", ctypes.c_int), ("max_iterations", ctypes.c_int), ("tile_rows", ctypes.c_int),
-                ("max_sweeps", ctypes.c_int), ("waves_per_cu", ctypes.c_int), ("recompute_costs", ctypes.c_int),
+                ("max_sweeps", ctypes.c_int), ("waves_per_cu", ctypes.c_int), ("cost_plane", ctypes.c_int),
                 ("tile_waves", ctypes.c_int),
                 ("round_per_blocks", ctypes.c_void_p)]
 

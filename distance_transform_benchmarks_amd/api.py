@@ -67,7 +67,7 @@ _SCHED = {"raster": _lib.SCHED_RASTER, "blocks": _lib.SCHED_BLOCKS, "task": _lib
 
 
 def _params(lambda_, v, connectivity, schedule, max_iterations, tile_rows, max_sweeps, waves_per_cu,
-            round_per_blocks=None, recompute_costs=False, tile_waves=0) -> DtkParams:
+            round_per_blocks=None, cost_plane=False, tile_waves=0) -> DtkParams:
     p = _lib.default_params()
     p.lambda_ = float(lambda_)
     p.v = float(v)
@@ -79,7 +79,7 @@ def _params(lambda_, v, connectivity, schedule, max_iterations, tile_rows, max_s
     p.tile_rows = int(tile_rows)
     p.max_sweeps = int(max_sweeps)
     p.waves_per_cu = int(waves_per_cu)
-    p.recompute_costs = 1 if recompute_costs else 0
+    p.cost_plane = 1 if cost_plane else 0
     p.tile_waves = int(tile_waves)
     if round_per_blocks is not None:
         if round_per_blocks.dtype != torch.int32 or not round_per_blocks.is_cuda:
@@ -91,7 +91,7 @@ def _params(lambda_, v, connectivity, schedule, max_iterations, tile_rows, max_s
 def ggdt(img: torch.Tensor, mask: torch.Tensor, lambda_: float, v: float = 1e10, *, connectivity: int = 8,
          schedule: str = "task", max_iterations: int = 0, tile_rows: int = 0, max_sweeps: int = 0,
          waves_per_cu: int = 0, out: Optional[torch.Tensor] = None, stats: bool = False,
-         round_per_blocks: Optional[torch.Tensor] = None, recompute_costs: bool = False, tile_waves: int = 0):
+         round_per_blocks: Optional[torch.Tensor] = None, cost_plane: bool = False, tile_waves: int = 0):
     """GGDT of one image (any schedule).  Returns D, or (D, stats dict)."""
     H, W = mask.shape[0], mask.shape[1]
     if img.shape[0] != H or img.shape[1] != W:
@@ -99,7 +99,7 @@ def ggdt(img: torch.Tensor, mask: torch.Tensor, lambda_: float, v: float = 1e10,
     if out is None:
         out = torch.empty((H, W), dtype=torch.float32, device=img.device)
     p = _params(lambda_, v, connectivity, schedule, max_iterations, tile_rows, max_sweeps, waves_per_cu,
-                round_per_blocks, recompute_costs, tile_waves)
+                round_per_blocks, cost_plane, tile_waves)
     if round_per_blocks is not None:
         need = _lib.load().dtk_tile_count_for(W, H, ctypes.byref(p))
         if not round_per_blocks.is_contiguous() or round_per_blocks.numel() < need:
@@ -168,7 +168,7 @@ def _batch_arrays(imgs, masks, outs):
 
 def ggdt_batched(imgs: Sequence[torch.Tensor], masks: Sequence[torch.Tensor], lambda_: float, v: float = 1e10, *,
                  connectivity: int = 8, tile_rows: int = 0, max_sweeps: int = 0, waves_per_cu: int = 0,
-                 outs: Optional[List[torch.Tensor]] = None, stats: bool = False, recompute_costs: bool = False,
+                 outs: Optional[List[torch.Tensor]] = None, stats: bool = False, cost_plane: bool = False,
                  tile_waves: int = 0):
     """GGDT of many images of any sizes in one persistent launch."""
     if len(imgs) != len(masks):
@@ -176,7 +176,7 @@ def ggdt_batched(imgs: Sequence[torch.Tensor], masks: Sequence[torch.Tensor], la
     if outs is None:
         outs = [torch.empty(m.shape[:2], dtype=torch.float32, device=m.device) for m in masks]
     p = _params(lambda_, v, connectivity, "task", 0, tile_rows, max_sweeps, waves_per_cu,
-                recompute_costs=recompute_costs, tile_waves=tile_waves)
+                cost_plane=cost_plane, tile_waves=tile_waves)
     ai, am, ad = _batch_arrays(imgs, masks, outs)
     st = DtkStats()
     dev = imgs[0].device if imgs else torch.device("cuda")
@@ -186,13 +186,13 @@ def ggdt_batched(imgs: Sequence[torch.Tensor], masks: Sequence[torch.Tensor], la
     return (outs, st.as_dict()) if stats else outs
 
 
-def autotune(imgs, masks, lambda_, v=1e10, *, connectivity=8, outs=None, reps=2, recompute_costs=False):
+def autotune(imgs, masks, lambda_, v=1e10, *, connectivity=8, outs=None, reps=2, cost_plane=False):
     """Pick tile_rows / max_sweeps / waves_per_cu for a batched plan by timing
     candidate plans on this batch (dtk_batch_autotune).  Returns
     (dict of the chosen parameters, report dict).  Memoised per process."""
     if outs is None:
         outs = [torch.empty(m.shape[:2], dtype=torch.float32, device=m.device) for m in masks]
-    p = _params(lambda_, v, connectivity, "task", 0, 0, 0, 0, recompute_costs=recompute_costs)
+    p = _params(lambda_, v, connectivity, "task", 0, 0, 0, 0, cost_plane=cost_plane)
     ai, am, ad = _batch_arrays(list(imgs), list(masks), list(outs))
     best, rep = DtkParams(), _lib.DtkTuneReport()
     dev = imgs[0].device
@@ -213,7 +213,7 @@ class BatchPlan:
     """
 
     def __init__(self, imgs, masks, lambda_, v=1e10, *, connectivity=8, tile_rows=0, max_sweeps=0,
-                 waves_per_cu=0, outs=None, recompute_costs=False, tile_waves=0, tune=False):
+                 waves_per_cu=0, outs=None, cost_plane=False, tile_waves=0, tune=False):
         if outs is None:
             outs = [torch.empty(m.shape[:2], dtype=torch.float32, device=m.device) for m in masks]
         self.imgs, self.masks, self.outs = list(imgs), list(masks), list(outs)
@@ -221,14 +221,14 @@ class BatchPlan:
         self.tuning = None
         if tune:
             chosen, report = autotune(self.imgs, self.masks, lambda_, v, connectivity=connectivity, outs=self.outs,
-                                      recompute_costs=recompute_costs)
+                                      cost_plane=cost_plane)
             tile_rows, max_sweeps, waves_per_cu = chosen["tile_rows"], chosen["max_sweeps"], chosen["waves_per_cu"]
             tile_waves = chosen["tile_waves"]
             self.tuning = {**chosen, **report}
         self.params = {"tile_rows": tile_rows, "tile_waves": tile_waves, "max_sweeps": max_sweeps,
                        "waves_per_cu": waves_per_cu}
         p = _params(lambda_, v, connectivity, "task", 0, tile_rows, max_sweeps, waves_per_cu,
-                    recompute_costs=recompute_costs, tile_waves=tile_waves)
+                    cost_plane=cost_plane, tile_waves=tile_waves)
         ai, am, ad = _batch_arrays(self.imgs, self.masks, self.outs)
         self._plan = ctypes.c_void_p()
         with torch.cuda.device(self.device):

@@ -482,14 +482,23 @@ namespace dtk
     }
     if constexpr (C8)
     {
+      edge_costs<KIND, RH, E_NE>(d, p, X0, Y0, lane, I, cNE);
       {
+        // The SW edge of (i, r) is the NE edge of (i - 1, r + 1): one lane
+        // shift instead of a second cost evaluation.  Lane 0 and the last
+        // row have no SW neighbour in the region.
         float tmp[RH];
-        edge_costs<KIND, RH, E_SW>(d, p, X0, Y0, lane, I, tmp);
+#pragma unroll
+        for (int r = 0; r < RH - 1; ++r)
+        {
+          const float sw = shr_z(cNE[r + 1]);
+          tmp[r]         = lane >= 1 ? sw : DTK_INF;
+        }
+        tmp[RH - 1] = DTK_INF;
         col_to_row<S>(tmp, rSW, T, lane);
       }
       edge_costs<KIND, RH, E_NW>(d, p, X0, Y0, lane, I, cNW);
       col_to_row<S>(cNW, rNW, T, lane);
-      edge_costs<KIND, RH, E_NE>(d, p, X0, Y0, lane, I, cNE);
     }
     else
     {

@@ -85,8 +85,9 @@ typedef struct
   int   tile_rows;      /* rows of one wave's region: 32 or 16 (0 = default: 16 for task, 32 for blocks) */
   int   max_sweeps;     /* sweep cap per tile visit (0 = until the tile is stable) */
   int   waves_per_cu;   /* persistent waves per CU (0 = occupancy limit) */
-  int   recompute_costs; /* tile schedules, lambda > 0: 0 = compute edge costs once per call into a
-                            workspace plane (16 B/px C8, 8 B/px C4), 1 = recompute them on every visit */
+  int   cost_plane;      /* tile schedules, lambda > 0: 0 = compute edge costs from the image on every
+                            visit (default), 1 = compute them once per call into a workspace plane
+                            (16 B/px C8, 8 B/px C4) that every visit reads */
   int   tile_waves;      /* task schedule: waves stacked per tile, 1, 2 or 4; a tile owns 62 x tile_waves *
                             (tile_rows - 2) pixels.  0 = library default (4 with 16-row, 2 with 32-row waves) */
   int*  round_per_blocks; /* optional device int[dtk_tile_count_for()]: visits per tile (blocks and task

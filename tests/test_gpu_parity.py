@@ -119,6 +119,23 @@ def test_random_vs_oracle(dt, dev, oracle, case, sched):
         assert_bit_exact(got, want, f"{case} {sched} {kw}")
 
 
+@pytest.mark.parametrize("case", [c for c in CASES if c[5] != 0.0], ids=lambda c: "-".join(map(str, c)))
+def test_cost_plane_option(dt, dev, oracle, case):
+    """Edge costs read from a per-call plane (cost_plane=True) instead of
+    evaluated from the image on every visit: the same fixed point."""
+    ik, mk, W, H, conn, lam = case
+    seed = zlib.crc32(repr(case).encode()) & 0xFFFF
+    img, m = image(ik, W, H, seed), mask(mk, W, H, seed)
+    v = 1e10 if mk == "u8" else 1e3
+    want, _ = oracle.raster(img, m, lam, v, conn8=conn == 8)
+    for sched, kws in (("task", (dict(), dict(tile_rows=32, tile_waves=2), dict(tile_rows=16, tile_waves=1))),
+                       ("blocks", (dict(),))):
+        for kw in kws:
+            _say("cost plane", case, sched, kw)
+            got = _run(dt, dev, img, m, lam, v, sched, connectivity=conn, cost_plane=True, **kw)
+            assert_bit_exact(got, want, f"{case} {sched} {kw}")
+
+
 @pytest.mark.parametrize("lam", [1.0, 0.0])
 @pytest.mark.parametrize("max_sweeps", [1, 2, 5])
 def test_task_sweep_cap(dt, dev, oracle, max_sweeps, lam):
