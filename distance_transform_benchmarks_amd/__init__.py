@@ -13,11 +13,15 @@ from .api import (  # noqa: F401
     BatchPlan,
     autotune,
     backward,
+    colorize,
     generalised_distance_transform,
     generalised_distance_transform_blocks,
     generalised_distance_transform_task,
     ggdt,
     ggdt_batched,
+    inferno,
+    minmax,
+    normalize_u8,
     predecessors,
     round_per_blocks_buffer,
     selftest,

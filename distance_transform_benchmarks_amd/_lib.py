@@ -28,7 +28,8 @@ EXPORTS = (
     "dtk_batch_plan_create", "dtk_batch_plan_launch", "dtk_batch_plan_destroy", "dtk_ggdt_pred",
     "dtk_ggdt_backward", "dtk_tile_count", "dtk_malloc_pitch", "dtk_free", "dtk_host_to_device",
     "dtk_device_to_host", "dtk_copy2d", "dtk_selftest", "dtk_device_cu_count", "dtk_profile_begin",
-    "dtk_profile_end", "dtk_batch_autotune", "dtk_tile_count_for",
+    "dtk_profile_end", "dtk_batch_autotune", "dtk_tile_count_for", "dtk_minmax_f32", "dtk_normalize_u8",
+    "dtk_inferno", "dtk_colorize",
 )
 
 
@@ -113,6 +114,10 @@ def load() -> ctypes.CDLL:
     lib.dtk_device_cu_count.restype = ctypes.c_int
     lib.dtk_profile_begin.argtypes = []
     lib.dtk_profile_end.argtypes = [P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_int)]
+    lib.dtk_minmax_f32.argtypes = [P(DtkImage), P(ctypes.c_float), vp]
+    lib.dtk_normalize_u8.argtypes = [P(DtkImage), P(DtkImage), vp]
+    lib.dtk_inferno.argtypes = [P(DtkImage), P(DtkImage), vp]
+    lib.dtk_colorize.argtypes = [P(DtkImage), P(DtkImage), vp]
     for name in ("dtk_ggdt", "dtk_ggdt_batched", "dtk_batch_plan_create", "dtk_batch_plan_launch",
                  "dtk_batch_plan_destroy", "dtk_batch_autotune", "dtk_ggdt_pred", "dtk_ggdt_backward", "dtk_malloc_pitch", "dtk_free",
                  "dtk_host_to_device", "dtk_device_to_host", "dtk_copy2d", "dtk_selftest", "dtk_profile_begin",

@@ -36,6 +36,7 @@ __all__ = [
     "generalised_distance_transform", "generalised_distance_transform_blocks",
     "generalised_distance_transform_task", "ggdt", "ggdt_batched", "BatchPlan", "predecessors",
     "backward", "selftest", "tile_count", "round_per_blocks_buffer", "tile_grid", "visits_buffer",
+    "minmax", "normalize_u8", "inferno", "colorize",
 ]
 
 
@@ -299,3 +300,45 @@ def selftest() -> dict:
     out = (ctypes.c_int * 3)()
     check(_lib.load().dtk_selftest(out))
     return {"dpp_ok": bool(out[0]), "sqrt_mismatches": int(out[1]), "cu": int(out[2])}
+
+
+# ---- display of a distance map (the reference CLI's last steps) -------------
+def minmax(img: torch.Tensor):
+    """(min, max) of a float32 device image as dt::minmax computes them
+    (dt/include/dt/normalize.hpp:28-43): NaNs skipped, max starting at the
+    smallest positive float.  Synchronous."""
+    out = (ctypes.c_float * 2)()
+    with torch.cuda.device(img.device):
+        check(_lib.load().dtk_minmax_f32(ctypes.byref(_as_image(img, "img")), out, _stream_handle(img.device)))
+    return float(out[0]), float(out[1])
+
+
+def normalize_u8(img: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """float32 (H, W) -> uint8 (H, W): (uint8)(double(x - min) / double(max - min) * 255),
+    truncated (dt/include/dt/normalize.hpp:45-62; 0/0 gives 0)."""
+    if out is None:
+        out = torch.empty(img.shape[:2], dtype=torch.uint8, device=img.device)
+    with torch.cuda.device(img.device):
+        check(_lib.load().dtk_normalize_u8(ctypes.byref(_as_image(img, "img")), ctypes.byref(_as_image(out, "out")),
+                                           _stream_handle(img.device)))
+    return out
+
+
+def inferno(img: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """uint8 (H, W) -> uint8 (H, W, 3) through the inferno colour map (dt/src/inferno.cpp:263-274)."""
+    if out is None:
+        out = torch.empty((img.shape[0], img.shape[1], 3), dtype=torch.uint8, device=img.device)
+    with torch.cuda.device(img.device):
+        check(_lib.load().dtk_inferno(ctypes.byref(_as_image(img, "img")), ctypes.byref(_as_image(out, "out")),
+                                      _stream_handle(img.device)))
+    return out
+
+
+def colorize(img: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """float32 (H, W) -> uint8 (H, W, 3): normalize_u8 then inferno in one pass."""
+    if out is None:
+        out = torch.empty((img.shape[0], img.shape[1], 3), dtype=torch.uint8, device=img.device)
+    with torch.cuda.device(img.device):
+        check(_lib.load().dtk_colorize(ctypes.byref(_as_image(img, "img")), ctypes.byref(_as_image(out, "out")),
+                                       _stream_handle(img.device)))
+    return out

@@ -35,8 +35,11 @@ COMMON = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
-DTK_SOURCES = ["dtk_task.hip", "dtk_raster.hip", "dtk_adjoint.hip", "dtk_host.cpp"]
-DT_SOURCES = ["dt_api.cpp"]
+DTK_SOURCES = ["dtk_task.hip", "dtk_raster.hip", "dtk_adjoint.hip", "dtk_color.hip", "dtk_host.cpp"]
+DT_SOURCES = ["dt_api.cpp", "dt_color.cpp", "dt_png.cpp"]
+# Command-line tools (the reference's bin/distance_transform.cpp and
+# bin/empiric_check.cpp), linked against libdt.
+TOOLS = {"dt_distance_transform": "dt_cli_distance_transform.cpp", "dt_empiric_check": "dt_cli_empiric_check.cpp"}
 
 
 def _run(cmd):
@@ -68,16 +71,24 @@ def build(jobs: int = 8, verbose: bool = True) -> dict:
     libdt = os.path.join(LIB, "libdt.so")
     tmp = libdt + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *[objs[s] for s in DT_SOURCES], "-o", tmp,
-          f"-L{LIB}", "-ldtk", "-Wl,-rpath,$ORIGIN"])
+          f"-L{LIB}", "-ldtk", "-lz", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, libdt)
+    tools = {}
+    for name, src in TOOLS.items():
+        exe = os.path.join(LIB, name)
+        tmp = exe + ".tmp"
+        _run([HIPCC, f"--offload-arch={ARCH}", *COMMON, "-x", "hip", os.path.join(CSRC, src), "-o", tmp,
+              f"-L{LIB}", "-ldt", "-ldtk", "-Wl,-rpath,$ORIGIN"])
+        os.replace(tmp, exe)
+        tools[name] = exe
     tests = os.path.join(LIB, "dt_tests")
     tmp = tests + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", *COMMON, "-x", "hip", os.path.join(CSRC, "dt_tests.cpp"), "-o", tmp,
           f"-L{LIB}", "-ldt", "-ldtk", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, tests)
     if verbose:
-        print(f"built {libdtk}, {libdt}, {tests}")
-    return {"libdtk": libdtk, "libdt": libdt, "dt_tests": tests}
+        print(f"built {libdtk}, {libdt}, {tests}, " + ", ".join(tools.values()))
+    return {"libdtk": libdtk, "libdt": libdt, "dt_tests": tests, **tools}
 
 
 def clean():

@@ -1,6 +1,7 @@
 // Self-contained test runner of the C++ API (include/dt/), mirroring the
 // reference's gtest suites (gtest is not available here):
-//   CPU : tests/image_view.cpp, tests/image2d.cpp, tests/c4.cpp, tests/rgb8.cpp
+//   CPU : tests/image_view.cpp, tests/image2d.cpp, tests/c4.cpp, tests/rgb8.cpp,
+//         tests/io.cpp, plus normalize / inferno / invert_mask
 //   GPU : tests/transfert.cpp, tests/distance_transform.cpp (the two
 //         FastGeodis golden vectors, bit-exact, and the raster == blocks ==
 //         task cross-schedule checks), plus the extension overloads.
@@ -22,6 +23,11 @@
 #include <dt/fill.hpp>
 #include <dt/generalised_distance_transform.hpp>
 #include <dt/image2d.hpp>
+#include <dt/imread.hpp>
+#include <dt/imsave.hpp>
+#include <dt/inferno.hpp>
+#include <dt/invert_mask.hpp>
+#include <dt/normalize.hpp>
 #include <dt/neighborhood.hpp>
 #include <dt/rgb.hpp>
 #include <dt/transfert.hpp>
@@ -221,6 +227,69 @@ namespace
       dt::rgb8 c{1, 4, 3};
       EXPECT(c.r == 1 && c.g == 4 && c.b == 3 && sizeof(dt::rgb8) == 3);
     });
+    run_test("Io.uint8", [] { // tests/io.cpp:8-16 of the reference
+      const std::uint8_t data[] = {10, 3, 5, 8, 1, 3, 0, 4, 10, 4};
+      dt::image2d_view   in(data, 5, 2, 5);
+      dt::imsave("test_tmp_io.png", in);
+      auto out = dt::imread<std::uint8_t>("test_tmp_io.png");
+      expect_images_eq(in, out);
+      std::remove("test_tmp_io.png");
+    });
+    run_test("Io.pitched_uint8", [] {
+      auto img = random_image<std::uint8_t>(37, 11, 5);
+      // a view with padded rows: every other column of a wider buffer's rows is skipped by the pitch
+      std::vector<std::uint8_t> wide(64 * 11, 0);
+      for (int y = 0; y < 11; ++y)
+        for (int x = 0; x < 37; ++x)
+          wide[y * 64 + x] = img(x, y);
+      dt::image2d_view<std::uint8_t> view(wide.data(), 37, 11, 64);
+      dt::imsave("test_tmp_io2.png", view);
+      expect_images_eq(img, dt::imread<std::uint8_t>("test_tmp_io2.png"));
+      std::remove("test_tmp_io2.png");
+    });
+    run_test("Io.errors", [] {
+      bool t1 = false, t2 = false, t3 = false;
+      const std::uint8_t d[4] = {1, 2, 3, 4};
+      dt::image2d_view   v(d, 2, 2, 2);
+      try { dt::imsave("test_tmp_io.jpg", v); } catch (const std::invalid_argument&) { t1 = true; }
+      try { dt::imread<std::uint8_t>("/nonexistent/file.png"); } catch (const std::runtime_error&) { t2 = true; }
+      dt::image2d<dt::rgb8> c(3, 2);
+      dt::fill(c, dt::rgb8{1, 2, 3});
+      dt::imsave("test_tmp_rgb.png", c);
+      try { dt::imread<std::uint8_t>("test_tmp_rgb.png"); } catch (const std::invalid_argument&) { t3 = true; }
+      std::remove("test_tmp_rgb.png");
+      EXPECT(t1 && t2 && t3);
+    });
+    run_test("Normalize.known_answers", [] {
+      // [0, 1, 2] -> 0, 1/2 * 255 = 127.5 -> 127, 255; negative values; the
+      // reference's max start (smallest positive float) on an all-zero image.
+      float        a[] = {0.f, 1.f, 2.f, -1.f, 0.f, 1.f, 0.f, 0.f, 0.f};
+      std::uint8_t w[] = {0, 127, 255, 0, 127, 255, 0, 0, 0};
+      for (int k = 0; k < 3; ++k)
+      {
+        dt::image2d_view<float> in(a + 3 * k, 3, 1, 12);
+        auto                    out = dt::normalize<std::uint8_t>(in);
+        for (int x = 0; x < 3; ++x)
+          EXPECT(out(x, 0) == w[3 * k + x]);
+      }
+      float                   z[] = {0.f, 0.f};
+      dt::image2d_view<float> zv(z, 2, 1, 8);
+      EXPECT(dt::minmax(zv).first == 0.f && dt::minmax(zv).second > 0.f && dt::minmax(zv).second < 1e-37f);
+    });
+    run_test("Inferno.table_ends", [] {
+      const dt::rgb8* t = dt::inferno_table();
+      EXPECT(t[0].r == 0 && t[0].g == 0 && t[0].b == 3 && t[255].r == 252 && t[255].g == 254 && t[255].b == 164);
+      std::uint8_t            v[] = {0, 255};
+      dt::image2d_view        in(v, 2, 1, 2);
+      auto                    out = dt::inferno(in);
+      EXPECT(out(0, 0).b == 3 && out(1, 0).r == 252);
+    });
+    run_test("InvertMask.invert", [] {
+      std::uint8_t     m[] = {0, 1, 7, 0};
+      dt::image2d_view in(m, 4, 1, 4);
+      auto             out = dt::invert_mask(in);
+      EXPECT(out(0, 0) == 1 && out(1, 0) == 0 && out(2, 0) == 0 && out(3, 0) == 1);
+    });
     run_test("Abi.params_default", [] {
       dtk_params p;
       dtk_params_default(&p);
@@ -320,6 +389,28 @@ namespace
       const auto b   = dt::device_to_host(dt::generalised_distance_transform(di, dm, 0.5f, 1e3f, o));
       expect_images_eq(a, b);
       EXPECT(a(10, 10) == 0.f && a(100, 50) <= 500.f);
+    });
+    run_test("Display.device_equals_host", [] {
+      // normalize + inferno of a distance map on the device equal the host
+      // versions (the reference computes them on the host).
+      auto img  = random_image<std::uint8_t>(173, 91, 31);
+      auto mask = dt::image2d<std::uint8_t>(173, 91);
+      dt::fill(mask, std::uint8_t(0));
+      mask(17, 40) = 1;
+      mask(150, 3) = 1;
+      const auto dist   = run_sched(img, mask, 0.5f, 2);
+      const auto d_dist = dt::host_to_device(dist);
+      const auto norm   = dt::normalize<std::uint8_t>(dist);
+      const auto d_norm = dt::normalize<std::uint8_t>(d_dist);
+      expect_images_eq(norm, dt::device_to_host(d_norm));
+      const auto col   = dt::inferno(norm);
+      const auto d_col = dt::device_to_host(dt::inferno(d_norm));
+      EXPECT(col.width() == d_col.width() && col.height() == d_col.height());
+      for (int y = 0; y < col.height(); ++y)
+        for (int x = 0; x < col.width(); ++x)
+          EXPECT(col(x, y).r == d_col(x, y).r && col(x, y).g == d_col(x, y).g && col(x, y).b == d_col(x, y).b);
+      const auto mm = dt::minmax(dist), d_mm = dt::minmax(d_dist);
+      EXPECT(mm.first == d_mm.first && mm.second == d_mm.second);
     });
     run_test("Errors.InvalidLambda", [] {
       dt::image2d<std::uint8_t> img(8, 8, dt::e_memory_kind::GPU), mask(8, 8, dt::e_memory_kind::GPU);

@@ -476,6 +476,40 @@ namespace
   }
 } // namespace
 
+// ---- display of a distance map -------------------------------------------
+namespace
+{
+  using namespace dtk;
+
+  // Same-size pair check for the colour entry points.
+  dtk_status check_pair(const dtk_image* in, int in_mask, const dtk_image* out, int out_mask)
+  {
+    dtk_status st;
+    if ((st = check_image(in, "in", in_mask)) != DTK_OK || (st = check_image(out, "out", out_mask)) != DTK_OK)
+      return st;
+    if (in->width != out->width || in->height != out->height)
+      return fail(DTK_ERR_INVALID_ARGUMENT, "in and out sizes differ");
+    return DTK_OK;
+  }
+
+  // Min / max keys of a f32 image in the workspace (two u32, stream-ordered
+  // like every other use of the workspace).
+  dtk_status minmax_keys(const dtk_image* in, hipStream_t s, uint32_t** keys)
+  {
+    dtk_status st;
+    Workspace* w = nullptr;
+    if ((st = current_ws(&w, nullptr)) != DTK_OK)
+      return st;
+    std::lock_guard<std::mutex> lock(w->mu);
+    if ((st = ws_reserve(*w, 256 + sizeof(uint32_t) * kMinmaxPartialWords, 4096)) != DTK_OK)
+      return st;
+    *keys              = static_cast<uint32_t*>(w->dbuf);
+    uint32_t* partial  = reinterpret_cast<uint32_t*>(static_cast<char*>(w->dbuf) + 256);
+    DTK_HIP(launch_minmax(in->data, in->pitch, in->width, in->height, *keys, partial, w->cu, s), "launch minmax");
+    return DTK_OK;
+  }
+} // namespace
+
 extern "C" {
 
 void dtk_params_default(dtk_params* p)
@@ -1154,6 +1188,74 @@ dtk_status dtk_copy2d(void* dst, long long dst_pitch, const void* src, long long
                                                     : (kind == 2 ? "Unable to transfert device to host: "
                                                                  : "Unable to copy on device: ")) +
                                  hipGetErrorString(e));
+  return DTK_OK;
+}
+
+
+dtk_status dtk_minmax_f32(const dtk_image* img, float* min_max, void* stream)
+{
+  dtk_status st;
+  if ((st = check_image(img, "img", 1 << DTK_F32)) != DTK_OK)
+    return st;
+  if (!min_max)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "min_max is NULL");
+  hipStream_t s    = static_cast<hipStream_t>(stream);
+  uint32_t    k[2] = {0x7f7fffffu | 0x80000000u, 0x00800000u | 0x80000000u}; // keys of FLT_MAX, FLT_MIN
+  if ((long long)img->width * img->height > 0)
+  {
+    uint32_t* keys = nullptr;
+    if ((st = minmax_keys(img, s, &keys)) != DTK_OK)
+      return st;
+    DTK_HIP(hipMemcpyAsync(k, keys, sizeof(k), hipMemcpyDeviceToHost, s), "read min/max");
+    DTK_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
+  }
+  min_max[0] = key_to_float(k[0]);
+  min_max[1] = key_to_float(k[1]);
+  return DTK_OK;
+}
+
+dtk_status dtk_normalize_u8(const dtk_image* in, dtk_image* out, void* stream)
+{
+  dtk_status st;
+  if ((st = check_pair(in, 1 << DTK_F32, out, 1 << DTK_U8)) != DTK_OK)
+    return st;
+  if ((long long)in->width * in->height == 0)
+    return DTK_OK;
+  hipStream_t s    = static_cast<hipStream_t>(stream);
+  uint32_t*   keys = nullptr;
+  if ((st = minmax_keys(in, s, &keys)) != DTK_OK)
+    return st;
+  DTK_HIP(launch_normalize_u8(in->data, in->pitch, out->data, out->pitch, in->width, in->height, keys, s),
+          "launch normalize");
+  return DTK_OK;
+}
+
+dtk_status dtk_inferno(const dtk_image* in, dtk_image* out, void* stream)
+{
+  dtk_status st;
+  if ((st = check_pair(in, 1 << DTK_U8, out, 1 << DTK_RGB8)) != DTK_OK)
+    return st;
+  if ((long long)in->width * in->height == 0)
+    return DTK_OK;
+  DTK_HIP(launch_inferno(in->data, in->pitch, out->data, out->pitch, in->width, in->height,
+                         static_cast<hipStream_t>(stream)),
+          "launch inferno");
+  return DTK_OK;
+}
+
+dtk_status dtk_colorize(const dtk_image* in, dtk_image* out, void* stream)
+{
+  dtk_status st;
+  if ((st = check_pair(in, 1 << DTK_F32, out, 1 << DTK_RGB8)) != DTK_OK)
+    return st;
+  if ((long long)in->width * in->height == 0)
+    return DTK_OK;
+  hipStream_t s    = static_cast<hipStream_t>(stream);
+  uint32_t*   keys = nullptr;
+  if ((st = minmax_keys(in, s, &keys)) != DTK_OK)
+    return st;
+  DTK_HIP(launch_colorize(in->data, in->pitch, out->data, out->pitch, in->width, in->height, keys, s),
+          "launch colorize");
   return DTK_OK;
 }
 

@@ -36,4 +36,17 @@ namespace dtk
 
   // dtk_selftest.hip
   hipError_t launch_selftest(int* dpp_out, float* sqrt_out, hipStream_t s);
+
+  // dtk_color.hip: min / max into two order-preserving keys, normalisation
+  // and the inferno colour map of pitched device images.
+  constexpr int kMinmaxPartialWords = 2 * 4096; // partial keys of launch_minmax's blocks
+  hipError_t launch_minmax(const void* img, long long pitch, int W, int H, uint32_t* keys, uint32_t* partial, int cu,
+                           hipStream_t s);
+  hipError_t launch_normalize_u8(const void* in, long long in_pitch, void* out, long long out_pitch, int W, int H,
+                                 const uint32_t* keys, hipStream_t s);
+  hipError_t launch_inferno(const void* in, long long in_pitch, void* out, long long out_pitch, int W, int H,
+                            hipStream_t s);
+  hipError_t launch_colorize(const void* in, long long in_pitch, void* out, long long out_pitch, int W, int H,
+                             const uint32_t* keys, hipStream_t s);
+  float      key_to_float(uint32_t k);
 } // namespace dtk

@@ -9,6 +9,7 @@
 
 #include <cassert>
 #include <cstdint>
+#include <type_traits>
 #include <utility>
 
 #include <dt/point2d.hpp>
@@ -124,8 +125,11 @@ namespace dt
   {
   public:
     image2d_view() noexcept = default;
+    // T may be const (a view of a read-only buffer, as in the reference's
+    // tests/io.cpp:10-11); the untyped base stores a plain byte pointer.
     image2d_view(T* buffer, int width, int height, int pitch, e_memory_kind memory_kind = e_memory_kind::CPU) noexcept
-      : image2d_view<void>(reinterpret_cast<std::uint8_t*>(buffer), width, height, pitch, (int)sizeof(T), memory_kind)
+      : image2d_view<void>(reinterpret_cast<std::uint8_t*>(const_cast<std::remove_const_t<T>*>(buffer)), width, height,
+                           pitch, (int)sizeof(T), memory_kind)
     {
     }
     image2d_view(const image2d_view<T>&) noexcept            = default;

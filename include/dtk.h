@@ -227,6 +227,24 @@ dtk_status dtk_selftest(int* out3);
 /* Device properties used for launch shapes. */
 int dtk_device_cu_count(void);
 
+/* Display of a distance map on the device: the step after the transform in
+ * the reference CLI (bin/distance_transform.cpp:66-70).  All images are
+ * device images; the calls are asynchronous on stream except
+ * dtk_minmax_f32, which returns the values to the host.
+ *   dtk_minmax_f32    min / max of a f32 image as dt::minmax computes them
+ *                     (dt/include/dt/normalize.hpp:28-43: NaNs skipped, max
+ *                     starts at the smallest positive float);
+ *   dtk_normalize_u8  f32 -> u8, (uint8)(double(x - min) / double(max - min)
+ *                     * 255) truncated (dt/include/dt/normalize.hpp:45-62;
+ *                     0/0 gives 0);
+ *   dtk_inferno       u8 -> rgb8 through the inferno table
+ *                     (dt/src/inferno.cpp:263-274);
+ *   dtk_colorize      f32 -> rgb8, normalise and colour map in one pass. */
+dtk_status dtk_minmax_f32(const dtk_image* img, float* min_max, void* stream);
+dtk_status dtk_normalize_u8(const dtk_image* in, dtk_image* out, void* stream);
+dtk_status dtk_inferno(const dtk_image* in, dtk_image* out, void* stream);
+dtk_status dtk_colorize(const dtk_image* in, dtk_image* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -171,3 +171,43 @@ def fd_grad(img, smask, Wt, lam: float, nu: float, idx, eps: float = 1e-4, conn8
     _load().ora_fd_grad(_ptr(img), _ptr(smask), _ptr(Wt), W, H, lam, nu, int(conn8), eps, eps_mask, _ptr(idx), len(idx),
                         _ptr(gi), _ptr(gm))
     return gi, gm
+
+
+# ---- display of a distance map (numpy restatements) --------------------------
+# TEST INFRASTRUCTURE ONLY, like the rest of this module.  Pinned by known
+# answers in tests/test_display.py and, for the colour map, by the reference's
+# own table (tests/golden/inferno_table.json).
+
+def minmax_f32(D: np.ndarray):
+    """dt/include/dt/normalize.hpp:28-43: min starts at FLT_MAX, max at
+    numeric_limits<float>::min() (the smallest positive float); a NaN never
+    replaces the running value (std::min / std::max with it second)."""
+    v = np.asarray(D, np.float32).ravel()
+    v = v[~np.isnan(v)]
+    mn = np.float32(np.finfo(np.float32).max)
+    mx = np.float32(np.finfo(np.float32).tiny)
+    if v.size:
+        mn = min(mn, v.min())
+        mx = max(mx, v.max())
+    return np.float32(mn), np.float32(mx)
+
+
+def normalize_u8(D: np.ndarray) -> np.ndarray:
+    """dt/include/dt/normalize.hpp:45-62 with O = uint8: the subtractions in
+    float32, the quotient and the scaling by 255 in float64, truncated;
+    a NaN quotient (0/0, undefined in the reference) gives 0."""
+    D = np.asarray(D, np.float32)
+    mn, mx = minmax_f32(D)
+    with np.errstate(invalid="ignore", divide="ignore", over="ignore"):
+        num = (D - mn).astype(np.float64)
+        den = np.float64(np.float32(mx - mn))
+        v = num / den * 255.0
+    out = np.zeros(D.shape, np.uint8)
+    ok = ~np.isnan(v)
+    out[ok] = np.trunc(v[ok]).astype(np.uint8)
+    return out
+
+
+def inferno_rgb(u8: np.ndarray, table: np.ndarray) -> np.ndarray:
+    """dt/src/inferno.cpp:263-274: out = table[v] for a (256, 3) table."""
+    return np.asarray(table, np.uint8)[np.asarray(u8, np.uint8)]

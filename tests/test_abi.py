@@ -57,6 +57,22 @@ def test_cpp_api_library_exports_reference_entry_points():
         assert name in out, name
 
 
+def test_cpp_api_library_exports_display_and_io():
+    """The C++ API of the CLI's helpers (imread / imsave / normalize /
+    inferno / invert_mask) and the C ABI of include/dt_io.h."""
+    so = os.path.join(PKG, "lib", "libdt.so")
+    out = subprocess.run(["nm", "-DC", "--defined-only", so], capture_output=True, text=True, check=True).stdout
+    for name in ("dt::imread<unsigned char>(", "dt::impl::imsave<unsigned char>(", "dt::impl::imsave<dt::rgb8>(",
+                 "dt::normalize<unsigned char>(", "dt::minmax(", "dt::inferno(", "dt::invert_mask(",
+                 "dt_png_info", "dt_png_read_u8", "dt_png_write", "dt_io_last_error"):
+        assert name in out, name
+    header = open(os.path.join(ROOT, "include", "dt_io.h")).read()
+    for fn in re.findall(r"^\w[\w\s\*]*?\b(dt_\w+)\(", header, re.M):
+        assert fn in out, fn
+    for tool in ("dt_distance_transform", "dt_empiric_check"):
+        assert os.access(os.path.join(PKG, "lib", tool), os.X_OK), tool
+
+
 def test_struct_layout_matches_header(tmp_path):
     """sizeof/offsetof of the ABI structs as a C compiler sees include/dtk.h
     equal the ctypes mirrors the Python binding uses."""
