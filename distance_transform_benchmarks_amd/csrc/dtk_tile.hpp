@@ -283,8 +283,10 @@ namespace dtk
 
   // ---- sweeps --------------------------------------------------------------
   // Column forward: row r pulls from row r-1 (dt/src/generalised_distance_transform.cu:59-99, Forward).
+  // edge: changes in the rows a stacked neighbour wave also holds (1, RH-2,
+  // RH-1; row 0 is only read).
   template <int RH, bool C8, bool TWO, class CC>
-  __device__ __forceinline__ uint64_t sweep_col_fwd(float (&D)[RH], const CC& cc, float ea, float ed)
+  __device__ __forceinline__ uint64_t sweep_col_fwd(float (&D)[RH], const CC& cc, float ea, float ed, uint64_t& edge)
   {
     uint64_t ch = 0;
     __builtin_amdgcn_wave_barrier();
@@ -301,7 +303,10 @@ namespace dtk
         const float b = shl_z(t) + cc.cNE(r);
         m             = min3f(a, b, c);
       }
-      ch |= ballot(lt_nn(m, D[r]));
+      const uint64_t bal = ballot(lt_nn(m, D[r]));
+      ch |= bal;
+      if (r == 1 || r >= RH - 2)
+        edge |= bal;
       D[r] = minf_nn(m, D[r]);
     }
     return ch;
@@ -336,8 +341,11 @@ namespace dtk
   }
 
   // Row forward: slot k pulls from slot k-1 (dt/src/generalised_distance_transform.cu:17-57, Forward).
+  // edge: changes in a segment's first and last columns (slots 1 and SW),
+  // which the neighbouring segments hold as copies taken at the transpose.
   template <int SW, bool C8, bool TWO, class CC>
-  __device__ __forceinline__ uint64_t sweep_row_fwd(float (&R)[SW + 2], const CC& cc, float ea, float ed)
+  __device__ __forceinline__ uint64_t sweep_row_fwd(float (&R)[SW + 2], const CC& cc, float ea, float ed,
+                                                    uint64_t& edge)
   {
     uint64_t ch = 0;
     __builtin_amdgcn_wave_barrier();
@@ -354,10 +362,25 @@ namespace dtk
         const float b = shl_z(t) + cc.rSW(k);
         m             = min3f(a, b, c);
       }
-      ch |= ballot(lt_nn(m, R[k]));
+      const uint64_t bal = ballot(lt_nn(m, R[k]));
+      ch |= bal;
+      if (k == 1 || k == SW)
+        edge |= bal;
       R[k] = minf_nn(m, R[k]);
     }
     return ch;
+  }
+
+  // Lanes of the row layout (lane = segment * RH + row) that hold a row a
+  // stacked neighbour wave also holds: rows 0, 1, RH-2, RH-1 of every segment.
+  template <int S>
+  __device__ constexpr uint64_t shared_row_lanes()
+  {
+    constexpr int RH = 64 / S;
+    uint64_t      m  = 0;
+    for (int h = 0; h < S; ++h)
+      m |= (1ull << (h * RH)) | (1ull << (h * RH + 1)) | (1ull << (h * RH + RH - 2)) | (1ull << (h * RH + RH - 1));
+    return m;
   }
 
   // Row backward: slot k receives from slot k+1.
@@ -721,11 +744,11 @@ namespace dtk
   // rows 0 and 1 are the same image rows as wave w-1's rows RH-2 and RH-1
   // (its halo and first owned row), so after each pair of sweeps both copies
   // of those rows are merged by min through LDS (X: 2 parities x NW x 4 rows
-  // x 64 floats; F: 2 x NW change flags).  Returns whether any wave changed
-  // a pixel in the pair just swept (the same in every wave).
+  // x 64 floats; F: 2 x NW flag words).  Returns the OR over the waves of
+  // their flags for the pair just swept (the same in every wave).
   template <int RH, int NW>
-  __device__ __forceinline__ bool exchange_rows(float (&Dc)[RH], float* X, uint32_t* F, int w, int lane, int par,
-                                                bool changed, bool& any)
+  __device__ __forceinline__ uint32_t exchange_rows(float (&Dc)[RH], float* X, uint32_t* F, int w, int lane, int par,
+                                                    uint32_t flags, bool& any)
   {
     float* Xp = X + par * (NW * 4 * 64);
     Xp[(w * 4 + 0) * 64 + lane] = Dc[0];
@@ -733,7 +756,7 @@ namespace dtk
     Xp[(w * 4 + 2) * 64 + lane] = Dc[RH - 2];
     Xp[(w * 4 + 3) * 64 + lane] = Dc[RH - 1];
     if (lane == 0)
-      F[par * NW + w] = changed ? 1u : 0u;
+      F[par * NW + w] = flags;
     __syncthreads();
     bool lowered = false;
     if (w > 0)
@@ -753,11 +776,11 @@ namespace dtk
       Dc[RH - 1] = minf_nn(Dc[RH - 1], b);
     }
     any |= ballot(lowered) != 0;
-    bool wg = false;
+    uint32_t all = 0;
 #pragma unroll
     for (int k = 0; k < NW; ++k)
-      wg |= F[par * NW + k] != 0u;
-    return wg;
+      all |= F[par * NW + k];
+    return all;
   }
 
   // A tile is solved by NW stacked waves (one 64 x RH region each, RH - 2
@@ -850,7 +873,8 @@ namespace dtk
     {
       for (;;)
       {
-        uint64_t ch = sweep_col_fwd<RH, C8, TWO>(Dc, cc, ea, ed);
+        uint64_t edge_unused = 0;
+        uint64_t ch          = sweep_col_fwd<RH, C8, TWO>(Dc, cc, ea, ed, edge_unused);
         ++sweeps;
         any |= ch != 0;
         quiet = ch ? 0 : quiet + 1;
@@ -874,7 +898,7 @@ namespace dtk
           break;
 
         col_to_row<S>(Dc, R, T, lane);
-        const uint64_t cf = sweep_row_fwd<SW, C8, TWO>(R, cc, ea, ed);
+        const uint64_t cf = sweep_row_fwd<SW, C8, TWO>(R, cc, ea, ed, edge_unused);
         ++sweeps;
         any |= cf != 0;
         quiet = cf ? 0 : quiet + 1;
@@ -901,39 +925,52 @@ namespace dtk
     else
     {
       // Sweep pairs (column down+up, row right+left) in every wave at once;
-      // the shared rows are merged after each pair, and the tile is stable
-      // after two consecutive pairs without a change in any wave.
-      int par = 0;
+      // the shared rows are merged after each pair.  A forward sweep leaves
+      // its direction's edges satisfied except where it read a copy that it
+      // then changed: rows a neighbour wave also holds, and (row layout) a
+      // segment's first and last columns.  So a pair whose backward sweep
+      // changed nothing and whose forward sweep did not touch those leaves
+      // the tile stable in its direction ("clean"), and one pair of the other
+      // direction without any change then certifies the tile.  Before the
+      // first pair the tile counts as not clean, which makes an untouched
+      // tile take two quiet pairs.
+      int  par        = 0;
+      bool prev_clean = false;
       for (;;)
       {
-        uint64_t ch = sweep_col_fwd<RH, C8, TWO>(Dc, cc, ea, ed);
-        ch |= sweep_col_bwd<RH, C8, TWO>(Dc, cc, ea, ed);
+        uint64_t       edge = 0;
+        const uint64_t cf   = sweep_col_fwd<RH, C8, TWO>(Dc, cc, ea, ed, edge);
+        const uint64_t cb   = sweep_col_bwd<RH, C8, TWO>(Dc, cc, ea, ed);
         sweeps += 2;
-        any |= ch != 0;
-        bool wg = exchange_rows<RH, NW>(Dc, X, F, w, lane, par, ch != 0, any);
+        any |= (cf | cb) != 0;
+        uint32_t all = exchange_rows<RH, NW>(
+          Dc, X, F, w, lane, par, ((cf | cb) != 0 ? 1u : 0u) | ((cb | edge) != 0 ? 2u : 0u), any);
         par ^= 1;
-        quiet = wg ? 0 : quiet + 2;
-        if (quiet >= 4)
+        if (!(all & 1u) && prev_clean)
         {
           conv = true;
           break;
         }
+        prev_clean = !(all & 2u);
         if (sweeps >= cap)
           break;
         col_to_row<S>(Dc, R, T, lane);
-        ch = sweep_row_fwd<SW, C8, TWO>(R, cc, ea, ed);
-        ch |= sweep_row_bwd<SW, C8, TWO>(R, cc, ea, ed);
+        edge              = 0;
+        const uint64_t rf = sweep_row_fwd<SW, C8, TWO>(R, cc, ea, ed, edge);
+        const uint64_t rb = sweep_row_bwd<SW, C8, TWO>(R, cc, ea, ed);
         row_to_col<S>(R, Dc, T, lane);
         sweeps += 2;
-        any |= ch != 0;
-        wg = exchange_rows<RH, NW>(Dc, X, F, w, lane, par, ch != 0, any);
+        any |= (rf | rb) != 0;
+        edge |= rf & shared_row_lanes<S>();
+        all = exchange_rows<RH, NW>(Dc, X, F, w, lane, par,
+                                    ((rf | rb) != 0 ? 1u : 0u) | ((rb | edge) != 0 ? 2u : 0u), any);
         par ^= 1;
-        quiet = wg ? 0 : quiet + 2;
-        if (quiet >= 4)
+        if (!(all & 1u) && prev_clean)
         {
           conv = true;
           break;
         }
+        prev_clean = !(all & 2u);
         if (sweeps >= cap)
           break;
       }
