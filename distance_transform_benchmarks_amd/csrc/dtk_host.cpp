@@ -574,21 +574,33 @@ dtk_status dtk_ggdt(const dtk_image* img, const dtk_image* mask, dtk_image* dist
   if (p->schedule == DTK_SCHED_RASTER)
   {
     // Strips of a pass wait on each other, so all of them must be resident.
-    const size_t words = raster_progress_words(d);
-    if ((long long)words > (long long)w->cu * raster_occupancy(kp, c8))
+    if ((long long)raster_strips(d) > (long long)w->cu * raster_occupancy(kp, c8))
       return fail(DTK_ERR_UNSUPPORTED, "image too large for the resident raster pipeline");
-    if ((st = ws_reserve(*w, 4096 + words * 4, 4096)) != DTK_OK)
+    // Chunk numbers (16 steps each) must stay below 2^16 in the stamps.
+    if (d.W >= (1 << 19) || d.H >= (1 << 19))
+      return fail(DTK_ERR_UNSUPPORTED, "raster schedule: at most 2^19 - 1 pixels per side");
+    const size_t mail_bytes = raster_mail_words(d) * sizeof(unsigned long long);
+    if ((st = ws_reserve(*w, 4096 + mail_bytes, 4096)) != DTK_OK)
       return st;
-    uint32_t* changed  = static_cast<uint32_t*>(w->dbuf);
-    uint32_t* progress = changed + 64;
-    uint32_t* h        = static_cast<uint32_t*>(w->hbuf);
+    uint32_t*           changed = static_cast<uint32_t*>(w->dbuf);
+    unsigned long long* mail    = reinterpret_cast<unsigned long long*>(static_cast<char*>(w->dbuf) + 4096);
+    uint32_t*           h       = static_cast<uint32_t*>(w->hbuf);
     DTK_HIP(hipMemsetAsync(changed, 0, 2 * sizeof(uint32_t), s), "memset");
     DTK_HIP(launch_raster_init(d, kp, changed + 1, s), "launch raster init");
-    int iters = 0;
+    int      iters = 0;
+    uint32_t pass0 = 0xffffffffu; // pass index stamped into the mailbox words
     for (;;)
     {
+      // Stamps are (pass << 16 | chunk); clear the mailbox before the pass
+      // index would wrap, so no word of an earlier pass can match.
+      if (pass0 > 65531u)
+      {
+        DTK_HIP(hipMemsetAsync(mail, 0, mail_bytes, s), "memset mailbox");
+        pass0 = 1;
+      }
       DTK_HIP(hipMemsetAsync(changed, 0, sizeof(uint32_t), s), "memset");
-      DTK_HIP(launch_raster_iteration(d, kp, c8, changed, progress, s), "launch raster pass");
+      DTK_HIP(launch_raster_iteration(d, kp, c8, changed, mail, pass0, s), "launch raster pass");
+      pass0 += 4;
       DTK_HIP(hipMemcpyAsync(h, changed, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read flag");
       DTK_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
       ++iters;

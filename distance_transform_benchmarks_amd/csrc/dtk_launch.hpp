@@ -22,8 +22,11 @@ namespace dtk
   // reference schedule; `changed` (device u32) is set when a pixel decreased.
   hipError_t launch_raster_init(const ImgDesc& d, const KParams& p, uint32_t* err, hipStream_t s);
   hipError_t launch_raster_iteration(const ImgDesc& d, const KParams& p, bool c8, uint32_t* changed,
-                                     uint32_t* progress, hipStream_t s);
-  size_t     raster_progress_words(const ImgDesc& d);
+                                     unsigned long long* mail, uint32_t pass0, hipStream_t s);
+  // Strips of the longest pass (all resident at once) and the mailbox words
+  // of their chunk handoff.
+  size_t raster_strips(const ImgDesc& d);
+  size_t raster_mail_words(const ImgDesc& d);
   // Strips (one 64-lane workgroup each) of one raster pass kernel that can be
   // resident on one CU at once; every strip of a pass must be resident.
   int        raster_occupancy(const KParams& p, bool c8);

@@ -12,10 +12,15 @@
 // lines plus G = 16 ghost lines on each side, and advances in chunks of
 // C = G steps: at step j of a chunk, lane l is exact when lanes l-1..l+1
 // were exact at step j-1, so after C steps the 32 owned lanes are exact even
-// though the outermost ghost lanes had no neighbours.  Before a chunk, a
-// strip waits until both neighbour strips have published the previous chunk
-// (a release/acquire progress flag), so the ghost lines it reloads are
-// final up to the chunk's first step.  Strips form a wavefront pipeline;
+// though the outermost ghost lanes had no neighbours.  A chunk needs, for
+// its ghost lanes, the neighbour strips' final values of the step before it
+// (their other values give the same result whenever they are read).  So at
+// the end of every chunk a strip publishes the final values of its first
+// and last G owned lines to a mailbox, as 64-bit (stamp << 32 | value)
+// words, and the neighbours poll those words: one memory round trip per
+// chunk, no flag and no fence.  The mailbox is double-buffered by chunk
+// parity (a strip is never more than one chunk ahead of its neighbours) and
+// stamped with (pass << 16 | chunk + 1).  Strips form a wavefront pipeline;
 // all strips of a pass are resident at once (checked by the host).
 //
 // Edge costs are formed on the fly from the image values of the current and
@@ -70,23 +75,17 @@ namespace dtk
     return reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(d.D) + (long long)y * d.D_pitch)[x];
   }
 
-  __device__ __forceinline__ void wait_progress(uint32_t* progress, int strip, uint32_t c, uint32_t* changed)
+  // Mailbox word of strip s: side 0 = its first G owned lines, side 1 = its
+  // last G; par = chunk parity; k = line within the side.
+  __device__ __forceinline__ size_t mail_index(int strip, int side, int par, int k)
   {
-    for (uint32_t spin = 0; __hip_atomic_load(&progress[strip], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < c; ++spin)
-    {
-      if (spin >= kSpinLimit)
-      {
-        // changed[1] is the error word of a raster call.
-        __hip_atomic_fetch_or(&changed[1], (uint32_t)ERR_WATCHDOG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return;
-      }
-      __builtin_amdgcn_s_sleep(1);
-    }
+    return ((size_t)(strip * 2 + side) * 2 + par) * kGhost + k;
   }
 
   // ROWS: row pass (steps along x, lanes are rows); else column pass.
   template <int KIND, bool ROWS, bool FWD, bool C8, bool TWO>
-  __global__ void __launch_bounds__(64) k_raster_pass(ImgDesc d, KParams p, uint32_t* changed, uint32_t* progress)
+  __global__ void __launch_bounds__(64) k_raster_pass(ImgDesc d, KParams p, uint32_t* changed,
+                                                      unsigned long long* mail, uint32_t epoch)
   {
     const int lane   = threadIdx.x;
     const int strip  = blockIdx.x;
@@ -113,59 +112,81 @@ namespace dtk
     float    Dprev = DTK_INF;
     uint32_t Iprev = 0;
 
+    // The chunk's own steps are loaded before waiting: no other strip
+    // stores an owned line's pixels, and a ghost line read before or after
+    // its owner stored this chunk gives the same result (the owner's value
+    // is min(old value, the relaxations recomputed here)).  (Loading them a
+    // chunk ahead is slower: loads complete in order, so the mailbox poll
+    // would wait for the whole next chunk's loads.)
+    const int cl = min(max(line, 0), nlines - 1);
     for (int c = 0; c < nchunks; ++c)
     {
       const int k0 = c * kGhost;
       const int kn = min(kGhost, len - k0);
 
-      // The chunk's own steps are loaded before waiting: no other strip
-      // stores an owned line's pixels, and a ghost line read before or after
-      // its owner stored this chunk gives the same result (the owner's value
-      // is min(old value, the relaxations recomputed here)).
       float    Dv[kGhost];
       uint32_t Iv[kGhost];
-      {
-        const int cl = min(max(line, 0), nlines - 1);
 #pragma unroll
-        for (int j = 0; j < kGhost; ++j)
-        {
-          const int      s  = start + dir * (k0 + min(j, kn - 1));
-          const int      x = ROWS ? s : cl, y = ROWS ? cl : s;
-          const float    dv = gload<float>(d.D, (long long)y * d.D_pitch + 4ll * x);
-          const uint32_t iv = img_at<KIND>(d, x, y);
-          Dv[j]             = (j < kn && lin) ? dv : DTK_INF;
-          Iv[j]             = (j < kn && lin) ? iv : 0u;
-        }
-      }
-
-      // Wait for the neighbour strips to finish chunk c-1: the previous
-      // step's ghost values below must be final.
-      if (c > 0)
+      for (int j = 0; j < kGhost; ++j)
       {
-        if (lane == 0)
-        {
-          if (strip > 0)
-            wait_progress(progress, strip - 1, (uint32_t)c, changed);
-          if (strip + 1 < nstrips)
-            wait_progress(progress, strip + 1, (uint32_t)c, changed);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        __syncthreads();
+        const int      s  = start + dir * (k0 + min(j, kn - 1));
+        const int      x = ROWS ? s : cl, y = ROWS ? cl : s;
+        const float    dv = gload<float>(d.D, (long long)y * d.D_pitch + 4ll * x);
+        const uint32_t iv = img_at<KIND>(d, x, y);
+        Dv[j]             = (j < kn && lin) ? dv : DTK_INF;
+        Iv[j]             = (j < kn && lin) ? iv : 0u;
       }
 
-      // Previous step: owned lanes keep their registers, the others reload.
+      // Previous step: from memory in the first chunk.  Afterwards every
+      // lane keeps its image value (the last step's), owned lanes keep their
+      // distance, and ghost lanes take the neighbour's final distance of the
+      // step before this chunk from its mailbox.
+      if (c == 0)
       {
         const int s = start + dir * (k0 - 1);
         // Clamped unconditional loads, masked afterwards.
-        const int   cl = min(max(line, 0), nlines - 1);
-        const int   x = ROWS ? s : cl, y = ROWS ? cl : s;
-        const float dv = gload<float>(d.D, (long long)y * d.D_pitch + 4ll * x);
+        const int      x = ROWS ? s : cl, y = ROWS ? cl : s;
+        const float    dv = gload<float>(d.D, (long long)y * d.D_pitch + 4ll * x);
         const uint32_t iv = img_at<KIND>(d, x, y);
-        if (c == 0 || !own)
+        Dprev             = lin ? dv : DTK_INF;
+        Iprev             = lin ? iv : 0u;
+      }
+      else
+      {
+        // Lanes 0..G-1 are the last G owned lines of strip - 1 (its side 1),
+        // lanes R+G..63 the first G owned lines of strip + 1 (its side 0).
+        const bool lower = lane < kGhost && strip > 0;
+        const bool upper = lane >= kGhost + kStripR && strip + 1 < nstrips;
+        const bool need  = (lower || upper) && lin;
+        const size_t   at = lower ? mail_index(strip - 1, 1, c & 1, lane)
+                                  : mail_index(strip + 1, 0, c & 1, max(lane - kGhost - kStripR, 0));
+        const uint32_t want = epoch | (uint32_t)c;
+        bool           have = !need;
+        float          got  = DTK_INF;
+        for (uint32_t spin = 0;; ++spin)
         {
-          Dprev = lin ? dv : DTK_INF;
-          Iprev = lin ? iv : 0u;
+          if (!have)
+          {
+            const unsigned long long w = __hip_atomic_load(&mail[at], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((uint32_t)(w >> 32) == want)
+            {
+              have = true;
+              got  = __uint_as_float((uint32_t)w);
+            }
+          }
+          if (__ballot(!have) == 0)
+            break;
+          if (spin >= kSpinLimit)
+          {
+            // changed[1] is the error word of a raster call.
+            if (lane == 0)
+              __hip_atomic_fetch_or(&changed[1], (uint32_t)ERR_WATCHDOG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
         }
+        if (need)
+          Dprev = got;
       }
 #pragma unroll
       for (int j = 0; j < kGhost; ++j)
@@ -223,47 +244,51 @@ namespace dtk
           }
         }
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      __syncthreads();
-      if (lane == 0)
-        __hip_atomic_store(&progress[strip], (uint32_t)(c + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // Publish the chunk's final values of the first and last G owned
+      // lines (Dprev holds the last step's value).
+      // First G owned lanes: [G, 2G) -> side 0; last G: [R, R + G) -> side 1.
+      const bool first = lane >= kGhost && lane < 2 * kGhost;
+      const bool last  = lane >= kStripR && lane < kStripR + kGhost;
+      if (c + 1 < nchunks && (first || last))
+      {
+        const unsigned long long w = ((unsigned long long)(epoch | (uint32_t)(c + 1)) << 32) | __float_as_uint(Dprev);
+        __hip_atomic_store(&mail[mail_index(strip, first ? 0 : 1, (c + 1) & 1, first ? lane - kGhost : lane - kStripR)],
+                           w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
     }
     if (__ballot(any) != 0 && lane == 0)
       __hip_atomic_store(changed, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 
+  // pass0: index of the iteration's first pass within the call (1-based,
+  // pass0 + 3 < 65536); the passes stamp their mailbox words with it.
   template <int KIND, bool C8, bool TWO>
-  static hipError_t raster_iter_t(const ImgDesc& d, const KParams& p, uint32_t* changed, uint32_t* progress,
-                                  hipStream_t s)
+  static hipError_t raster_iter_t(const ImgDesc& d, const KParams& p, uint32_t* changed, unsigned long long* mail,
+                                  uint32_t pass0, hipStream_t s)
   {
     const int rs = (d.H + kStripR - 1) / kStripR; // row-pass strips
     const int cs = (d.W + kStripR - 1) / kStripR; // column-pass strips
-    hipError_t e;
-    if ((e = hipMemsetAsync(progress, 0, sizeof(uint32_t) * rs, s)) != hipSuccess)
-      return e;
-    hipLaunchKernelGGL((k_raster_pass<KIND, true, true, C8, TWO>), dim3(rs), dim3(64), 0, s, d, p, changed, progress);
-    if ((e = hipMemsetAsync(progress, 0, sizeof(uint32_t) * rs, s)) != hipSuccess)
-      return e;
-    hipLaunchKernelGGL((k_raster_pass<KIND, true, false, C8, TWO>), dim3(rs), dim3(64), 0, s, d, p, changed, progress);
-    if ((e = hipMemsetAsync(progress, 0, sizeof(uint32_t) * cs, s)) != hipSuccess)
-      return e;
-    hipLaunchKernelGGL((k_raster_pass<KIND, false, true, C8, TWO>), dim3(cs), dim3(64), 0, s, d, p, changed, progress);
-    if ((e = hipMemsetAsync(progress, 0, sizeof(uint32_t) * cs, s)) != hipSuccess)
-      return e;
-    hipLaunchKernelGGL((k_raster_pass<KIND, false, false, C8, TWO>), dim3(cs), dim3(64), 0, s, d, p, changed, progress);
+    hipLaunchKernelGGL((k_raster_pass<KIND, true, true, C8, TWO>), dim3(rs), dim3(64), 0, s, d, p, changed, mail,
+                       (pass0 + 0u) << 16);
+    hipLaunchKernelGGL((k_raster_pass<KIND, true, false, C8, TWO>), dim3(rs), dim3(64), 0, s, d, p, changed, mail,
+                       (pass0 + 1u) << 16);
+    hipLaunchKernelGGL((k_raster_pass<KIND, false, true, C8, TWO>), dim3(cs), dim3(64), 0, s, d, p, changed, mail,
+                       (pass0 + 2u) << 16);
+    hipLaunchKernelGGL((k_raster_pass<KIND, false, false, C8, TWO>), dim3(cs), dim3(64), 0, s, d, p, changed, mail,
+                       (pass0 + 3u) << 16);
     return hipGetLastError();
   }
 
   template <int KIND>
-  static hipError_t raster_iter_k(const ImgDesc& d, const KParams& p, bool c8, uint32_t* changed, uint32_t* progress,
-                                  hipStream_t s)
+  static hipError_t raster_iter_k(const ImgDesc& d, const KParams& p, bool c8, uint32_t* changed,
+                                  unsigned long long* mail, uint32_t pass0, hipStream_t s)
   {
     const bool two = !(p.zero_lambda || p.l_grad == 1.f);
     if (c8)
-      return two ? raster_iter_t<KIND, true, true>(d, p, changed, progress, s)
-                 : raster_iter_t<KIND, true, false>(d, p, changed, progress, s);
-    return two ? raster_iter_t<KIND, false, true>(d, p, changed, progress, s)
-               : raster_iter_t<KIND, false, false>(d, p, changed, progress, s);
+      return two ? raster_iter_t<KIND, true, true>(d, p, changed, mail, pass0, s)
+                 : raster_iter_t<KIND, true, false>(d, p, changed, mail, pass0, s);
+    return two ? raster_iter_t<KIND, false, true>(d, p, changed, mail, pass0, s)
+               : raster_iter_t<KIND, false, false>(d, p, changed, mail, pass0, s);
   }
 
   hipError_t launch_raster_init(const ImgDesc& d, const KParams& p, uint32_t* err, hipStream_t s)
@@ -275,16 +300,16 @@ namespace dtk
   }
 
   hipError_t launch_raster_iteration(const ImgDesc& d, const KParams& p, bool c8, uint32_t* changed,
-                                     uint32_t* progress, hipStream_t s)
+                                     unsigned long long* mail, uint32_t pass0, hipStream_t s)
   {
     switch (p.img_kind)
     {
     case IMG_U8:
-      return raster_iter_k<IMG_U8>(d, p, c8, changed, progress, s);
+      return raster_iter_k<IMG_U8>(d, p, c8, changed, mail, pass0, s);
     case IMG_RGB8:
-      return raster_iter_k<IMG_RGB8>(d, p, c8, changed, progress, s);
+      return raster_iter_k<IMG_RGB8>(d, p, c8, changed, mail, pass0, s);
     default:
-      return raster_iter_k<IMG_F32>(d, p, c8, changed, progress, s);
+      return raster_iter_k<IMG_F32>(d, p, c8, changed, mail, pass0, s);
     }
   }
 
@@ -325,10 +350,12 @@ namespace dtk
     }
   }
 
-  size_t raster_progress_words(const ImgDesc& d)
+  size_t raster_strips(const ImgDesc& d)
   {
     const int rs = (d.H + kStripR - 1) / kStripR;
     const int cs = (d.W + kStripR - 1) / kStripR;
     return (size_t)(rs > cs ? rs : cs);
   }
+
+  size_t raster_mail_words(const ImgDesc& d) { return raster_strips(d) * 2 * 2 * kGhost; }
 } // namespace dtk

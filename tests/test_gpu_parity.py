@@ -161,7 +161,8 @@ def test_fixed_iterations(dt, dev, oracle, iters, conn):
 
 
 @pytest.mark.parametrize("shape", [(1, 1), (1, 9), (9, 1), (62, 14), (63, 15), (64, 16), (2, 70), (70, 2),
-                                   (62, 60), (63, 61), (64, 62), (70, 121), (125, 31), (130, 241)])
+                                   (62, 60), (63, 61), (64, 62), (70, 121), (125, 31), (130, 241), (1100, 70),
+                                   (70, 1100)])
 @pytest.mark.parametrize("sched", ["raster", "blocks", "task"])
 @pytest.mark.parametrize("lam", [0.5, 0.0])
 def test_edge_shapes(dt, dev, oracle, shape, sched, lam):
