@@ -493,8 +493,11 @@ namespace
   }
 
   // Min / max keys of a f32 image in the workspace (two u32, stream-ordered
-  // like every other use of the workspace).
-  dtk_status minmax_keys(const dtk_image* in, hipStream_t s, uint32_t** keys)
+  // like every other use of the workspace), then `then(keys)` enqueues what
+  // reads them, all under the workspace lock (another call may grow, and so
+  // reallocate, the workspace once the lock is released).
+  template <typename Then>
+  dtk_status with_minmax_keys(const dtk_image* in, hipStream_t s, Then&& then)
   {
     dtk_status st;
     Workspace* w = nullptr;
@@ -503,10 +506,10 @@ namespace
     std::lock_guard<std::mutex> lock(w->mu);
     if ((st = ws_reserve(*w, 256 + sizeof(uint32_t) * kMinmaxPartialWords, 4096)) != DTK_OK)
       return st;
-    *keys              = static_cast<uint32_t*>(w->dbuf);
-    uint32_t* partial  = reinterpret_cast<uint32_t*>(static_cast<char*>(w->dbuf) + 256);
-    DTK_HIP(launch_minmax(in->data, in->pitch, in->width, in->height, *keys, partial, w->cu, s), "launch minmax");
-    return DTK_OK;
+    uint32_t* keys    = static_cast<uint32_t*>(w->dbuf);
+    uint32_t* partial = reinterpret_cast<uint32_t*>(static_cast<char*>(w->dbuf) + 256);
+    DTK_HIP(launch_minmax(in->data, in->pitch, in->width, in->height, keys, partial, w->cu, s), "launch minmax");
+    return then(keys);
   }
 } // namespace
 
@@ -1215,11 +1218,13 @@ dtk_status dtk_minmax_f32(const dtk_image* img, float* min_max, void* stream)
   uint32_t    k[2] = {0x7f7fffffu | 0x80000000u, 0x00800000u | 0x80000000u}; // keys of FLT_MAX, FLT_MIN
   if ((long long)img->width * img->height > 0)
   {
-    uint32_t* keys = nullptr;
-    if ((st = minmax_keys(img, s, &keys)) != DTK_OK)
+    st = with_minmax_keys(img, s, [&](uint32_t* keys) {
+      DTK_HIP(hipMemcpyAsync(k, keys, sizeof(k), hipMemcpyDeviceToHost, s), "read min/max");
+      DTK_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
+      return DTK_OK;
+    });
+    if (st != DTK_OK)
       return st;
-    DTK_HIP(hipMemcpyAsync(k, keys, sizeof(k), hipMemcpyDeviceToHost, s), "read min/max");
-    DTK_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
   }
   min_max[0] = key_to_float(k[0]);
   min_max[1] = key_to_float(k[1]);
@@ -1233,13 +1238,12 @@ dtk_status dtk_normalize_u8(const dtk_image* in, dtk_image* out, void* stream)
     return st;
   if ((long long)in->width * in->height == 0)
     return DTK_OK;
-  hipStream_t s    = static_cast<hipStream_t>(stream);
-  uint32_t*   keys = nullptr;
-  if ((st = minmax_keys(in, s, &keys)) != DTK_OK)
-    return st;
-  DTK_HIP(launch_normalize_u8(in->data, in->pitch, out->data, out->pitch, in->width, in->height, keys, s),
-          "launch normalize");
-  return DTK_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return with_minmax_keys(in, s, [&](uint32_t* keys) {
+    DTK_HIP(launch_normalize_u8(in->data, in->pitch, out->data, out->pitch, in->width, in->height, keys, s),
+            "launch normalize");
+    return DTK_OK;
+  });
 }
 
 dtk_status dtk_inferno(const dtk_image* in, dtk_image* out, void* stream)
@@ -1262,13 +1266,12 @@ dtk_status dtk_colorize(const dtk_image* in, dtk_image* out, void* stream)
     return st;
   if ((long long)in->width * in->height == 0)
     return DTK_OK;
-  hipStream_t s    = static_cast<hipStream_t>(stream);
-  uint32_t*   keys = nullptr;
-  if ((st = minmax_keys(in, s, &keys)) != DTK_OK)
-    return st;
-  DTK_HIP(launch_colorize(in->data, in->pitch, out->data, out->pitch, in->width, in->height, keys, s),
-          "launch colorize");
-  return DTK_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return with_minmax_keys(in, s, [&](uint32_t* keys) {
+    DTK_HIP(launch_colorize(in->data, in->pitch, out->data, out->pitch, in->width, in->height, keys, s),
+            "launch colorize");
+    return DTK_OK;
+  });
 }
 
 dtk_status dtk_selftest(int* out3)
