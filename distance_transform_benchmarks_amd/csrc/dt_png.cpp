@@ -171,7 +171,8 @@ namespace
         h.interlace = data[12];
         if (data[10] != 0 || data[11] != 0 || h.interlace > 1 || !valid_depth(h.color, h.depth))
           return "unsupported IHDR";
-        if (h.width == 0 || h.height == 0 || h.width > (1u << 24) || h.height > (1u << 24))
+        if (h.width == 0 || h.height == 0 || h.width > (1u << 24) || h.height > (1u << 24) ||
+            (unsigned long long)h.width * h.height > (1ull << 31))
           return "bad image size";
         have_header = true;
       }
@@ -365,7 +366,16 @@ namespace dt
   {
     std::vector<unsigned char> bytes;
     Decoded                    d;
-    if (!read_file(filename, bytes) || !decode(bytes, d).empty())
+    bool                       ok = false;
+    try
+    {
+      ok = read_file(filename, bytes) && decode(bytes, d).empty();
+    }
+    catch (const std::bad_alloc&)
+    {
+      ok = false;
+    }
+    if (!ok)
       throw std::runtime_error(std::string("Error while loading ") + filename);
     if (d.components != 1)
       throw std::invalid_argument("Input image must have 1 component (grayscale) of 8 bits");
@@ -401,15 +411,21 @@ extern "C"
   {
     std::vector<unsigned char> bytes;
     Decoded                    d;
-    if (!path || !read_file(path, bytes))
+    std::string                why;
+    try
     {
-      g_io_err = std::string("Error while loading ") + (path ? path : "(null)");
-      return DT_IO_ERR_READ;
+      if (!path || !read_file(path, bytes))
+        why = "cannot read the file";
+      else
+        why = decode(bytes, d);
     }
-    const std::string why = decode(bytes, d);
+    catch (const std::exception& e)
+    {
+      why = e.what();
+    }
     if (!why.empty())
     {
-      g_io_err = std::string("Error while loading ") + path + ": " + why;
+      g_io_err = std::string("Error while loading ") + (path ? path : "(null)") + ": " + why;
       return DT_IO_ERR_READ;
     }
     if (width)
