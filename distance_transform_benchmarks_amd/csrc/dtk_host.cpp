@@ -643,27 +643,39 @@ dtk_status dtk_ggdt(const dtk_image* img, const dtk_image* mask, dtk_image* dist
     DTK_HIP(launch_reset_ctrl(q.ctrl, s), "launch reset");
     DTK_HIP(launch_init(b, kb, q, s), "launch init");
     DTK_HIP(launch_blocks_seed(b, q, a0, s), "launch seed");
-    uint32_t* h      = static_cast<uint32_t*>(w->hbuf);
-    int       rounds = 0;
+    uint32_t* h = static_cast<uint32_t*>(w->hbuf);
+    // Rounds go out in batches with one read-back each: a round after
+    // convergence finds no active tile and costs one empty launch, while a
+    // read-back costs a host round trip.  Round r of the call stores r in
+    // CTRL_CHANGED when it wakes a tile.  The loop ends once the last waking
+    // round is not the batch's final one (the round after it ran and woke
+    // nothing), and the call took (last waking round + 1) rounds.
+    constexpr int kRoundsPerReadback = 8;
+    uint32_t      round = 0, last_wake = 0;
     for (;;)
     {
       DTK_HIP(hipMemsetAsync(q.ctrl + CTRL_CHANGED, 0, sizeof(uint32_t), s), "memset");
-      DTK_HIP(launch_blocks(S, c8, cm, b, kb, a0, a1, q.ctrl, p->round_per_blocks, s), "launch blocks");
+      for (int k = 0; k < kRoundsPerReadback; ++k)
+      {
+        ++round;
+        DTK_HIP(launch_blocks(S, c8, cm, b, kb, a0, a1, q.ctrl, p->round_per_blocks, round, s), "launch blocks");
+        std::swap(a0, a1);
+      }
       DTK_HIP(hipMemcpyAsync(h, q.ctrl, CTRL_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read flag");
       DTK_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
-      ++rounds;
       if (h[CTRL_ERROR] & ERR_MASK_RANGE)
         return fail(DTK_ERR_INVALID_ARGUMENT, "soft mask values must be in [0, 1]");
-      if (!h[CTRL_CHANGED])
+      if (h[CTRL_CHANGED])
+        last_wake = h[CTRL_CHANGED];
+      if (last_wake < round)
         break;
       // Each round with a wake lowers some stored pixel, so the loop ends;
       // this bound (far above any measured round count) turns a protocol
       // fault into an error instead of an endless loop.
-      if (rounds > 1000 + 64ll * ntiles)
-        return fail(DTK_ERR_NOT_CONVERGED, "blocks schedule: no convergence after " + std::to_string(rounds) + " rounds");
-      std::swap(a0, a1);
+      if (round > 1000 + 64ull * ntiles)
+        return fail(DTK_ERR_NOT_CONVERGED, "blocks schedule: no convergence after " + std::to_string(round) + " rounds");
     }
-    return fill_stats(stats, h, rounds, ntiles, rh, 0);
+    return fill_stats(stats, h, (int)last_wake + 1, ntiles, rh, 0);
   }
 
   // Task schedule.

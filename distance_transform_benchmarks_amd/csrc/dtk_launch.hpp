@@ -15,7 +15,7 @@ namespace dtk
   hipError_t launch_task(int S, bool c8, int cm, const Batch& b, const KParams& p, const Queue& q, int workers,
                          hipStream_t s);
   hipError_t launch_blocks(int S, bool c8, int cm, const Batch& b, const KParams& p, uint8_t* active, uint8_t* next,
-                           uint32_t* ctrl, int* rpt, hipStream_t s);
+                           uint32_t* ctrl, int* rpt, uint32_t round, hipStream_t s);
   int        task_occupancy(int S, bool c8, int cm, int nw);
 
   // dtk_raster.hip: one full raster iteration (four passes) with the exact

@@ -453,10 +453,12 @@ namespace dtk
   }
 
   // One host-driven round: every active tile solves to its local fixed point
-  // and activates the neighbours whose halo it changed.
+  // and activates the neighbours whose halo it changed.  A round that woke a
+  // tile stores its number (>= 1) in CTRL_CHANGED, so the host can enqueue
+  // several rounds and read back once which was the last to wake anything.
   template <int S, bool C8, int CM>
   __global__ void __launch_bounds__(64) k_blocks(Batch b, KParams p, uint8_t* active, uint8_t* next, uint32_t* ctrl,
-                                                 int* rounds_per_tile)
+                                                 int* rounds_per_tile, uint32_t round)
   {
     __shared__ float T[(64 / S) * 65];
     __shared__ float Sv[2 * (64 / S)];
@@ -491,7 +493,7 @@ namespace dtk
         woke                                = true;
       }
       if (woke)
-        __hip_atomic_store(&ctrl[CTRL_CHANGED], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctrl[CTRL_CHANGED], round, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 
@@ -535,9 +537,9 @@ namespace dtk
 
   template <int S, bool C8, int CM>
   static hipError_t launch_blocks_t(const Batch& b, const KParams& p, uint8_t* active, uint8_t* next, uint32_t* ctrl,
-                                    int* rpt, hipStream_t s)
+                                    int* rpt, uint32_t round, hipStream_t s)
   {
-    hipLaunchKernelGGL((k_blocks<S, C8, CM>), dim3(b.ntiles), dim3(64), 0, s, b, p, active, next, ctrl, rpt);
+    hipLaunchKernelGGL((k_blocks<S, C8, CM>), dim3(b.ntiles), dim3(64), 0, s, b, p, active, next, ctrl, rpt, round);
     return hipGetLastError();
   }
 
@@ -603,9 +605,9 @@ namespace dtk
   struct BlocksLauncher
   {
     static hipError_t run(const Batch& b, const KParams& p, uint8_t* active, uint8_t* next, uint32_t* ctrl, int* rpt,
-                          hipStream_t s)
+                          uint32_t round, hipStream_t s)
     {
-      return launch_blocks_t<S, C8, CM>(b, p, active, next, ctrl, rpt, s);
+      return launch_blocks_t<S, C8, CM>(b, p, active, next, ctrl, rpt, round, s);
     }
   };
   template <int S, bool C8, int CM>
@@ -621,9 +623,9 @@ namespace dtk
   }
 
   hipError_t launch_blocks(int S, bool c8, int cm, const Batch& b, const KParams& p, uint8_t* active, uint8_t* next,
-                           uint32_t* ctrl, int* rpt, hipStream_t s)
+                           uint32_t* ctrl, int* rpt, uint32_t round, hipStream_t s)
   {
-    return dispatch<BlocksLauncher>(S, c8, cm, b, p, active, next, ctrl, rpt, s);
+    return dispatch<BlocksLauncher>(S, c8, cm, b, p, active, next, ctrl, rpt, round, s);
   }
 
   int task_occupancy(int S, bool c8, int cm, int nw) { return dispatch<Occupancy>(S, c8, cm, nw); }
