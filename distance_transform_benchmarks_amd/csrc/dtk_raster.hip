@@ -82,35 +82,115 @@ namespace dtk
     return ((size_t)(strip * 2 + side) * 2 + par) * kGhost + k;
   }
 
+  // Image value of one pixel as the raster passes hold it: uint8 as a float
+  // (differences of small integers are exact in float, so |a - b| is the
+  // same value as the integer difference), float as float, RGB8 packed.
+  template <int KIND>
+  struct RasterPix
+  {
+    using T = float;
+  };
+  template <>
+  struct RasterPix<IMG_RGB8>
+  {
+    using T = uint32_t;
+  };
+
+  template <int KIND>
+  __device__ __forceinline__ typename RasterPix<KIND>::T raster_img(const ImgDesc& d, int x, int y)
+  {
+    const unsigned char* r = d.img + (long long)y * d.img_pitch;
+    if constexpr (KIND == IMG_U8)
+      return (float)gload<unsigned char>(r, x);
+    else if constexpr (KIND == IMG_F32)
+      return gload<float>(r, 4ll * x);
+    else
+      return load_pix<KIND>(r, x);
+  }
+
+  // Wave shifts of image values, copied through an opaque move so the DPP
+  // is never folded into the subtract that uses it (see opaque_u).
+  template <class T>
+  __device__ __forceinline__ T shr_img(T v)
+  {
+    if constexpr (sizeof(T) == 4 && !__is_same(T, float))
+      return shr_u(v);
+    else
+      return __uint_as_float(opaque_u(__float_as_uint(shr_z(v))));
+  }
+  template <class T>
+  __device__ __forceinline__ T shl_img(T v)
+  {
+    if constexpr (sizeof(T) == 4 && !__is_same(T, float))
+      return shl_u(v);
+    else
+      return __uint_as_float(opaque_u(__float_as_uint(shl_z(v))));
+  }
+
+  // Cost term of an edge: CM 2: l_grad * |I(p) - I(q)|; CM 1 (l_grad = 1):
+  // |I(p) - I(q)|.  A NaN (float images only) becomes +inf, which never wins
+  // a strict compare, as a NaN would not.
+  template <int KIND, int CM, class T>
+  __device__ __forceinline__ float raster_cost(T a, T b, float l_grad)
+  {
+    float g;
+    if constexpr (KIND == IMG_RGB8)
+      g = pix_dist<IMG_RGB8>(a, b);
+    else
+      g = fabsf(a - b);
+    if constexpr (CM == 2)
+      g = l_grad * g;
+    if constexpr (KIND == IMG_F32)
+      g = (g == g) ? g : DTK_INF;
+    return g;
+  }
+
   // ROWS: row pass (steps along x, lanes are rows); else column pass.
-  template <int KIND, bool ROWS, bool FWD, bool C8, bool TWO>
+  // CM: 0 = lambda 0 (chamfer costs e_axis / e_diag, the image is not read),
+  // 1 = lambda 1 (cur = D(q) + g), 2 = otherwise (cur = (D(q) + e) + g).
+  //
+  // A chunk runs in three phases.  Its image values, its distances and the
+  // first mailbox poll are issued first; the edge costs of all its steps
+  // (which depend on the image only) are formed while the distances and the
+  // poll are in flight; the step chain then needs two adds per candidate and
+  // two minima per step, without branches.  Lanes outside the image, and
+  // the steps of the pass's last chunk past its end, get +inf costs (+inf
+  // chamfer steps at lambda 0), so they leave D at +inf or unchanged.  Ghost
+  // lanes take no further masks: the outermost lanes (0 and 63) read a zero
+  // from the wave shift, but a wrong value on a ghost lane moves in by one
+  // lane per step and a chunk has G steps, so owned lanes never see it.
+  template <int KIND, bool ROWS, bool FWD, bool C8, int CM>
   __global__ void __launch_bounds__(64) k_raster_pass(ImgDesc d, KParams p, uint32_t* changed,
                                                       unsigned long long* mail, uint32_t epoch)
   {
-    const int lane   = threadIdx.x;
-    const int strip  = blockIdx.x;
-    const int nlines = ROWS ? d.H : d.W; // lines (lanes) dimension
-    const int nsteps = ROWS ? d.W : d.H; // step dimension
-    const int line   = strip * kStripR - kGhost + lane;
-    const bool lin   = line >= 0 && line < nlines;
-    const bool own   = lane >= kGhost && lane < kGhost + kStripR && lin;
-    const bool linL  = line - 1 >= 0 && line - 1 < nlines;
-    const bool linR  = line + 1 >= 0 && line + 1 < nlines;
+    using T            = typename RasterPix<KIND>::T;
+    const int lane     = threadIdx.x;
+    const int strip    = blockIdx.x;
+    const int nlines   = ROWS ? d.H : d.W; // lines (lanes) dimension
+    const int nsteps   = ROWS ? d.W : d.H; // step dimension
+    const int line     = strip * kStripR - kGhost + lane;
+    const bool lin     = line >= 0 && line < nlines;
+    const bool own     = lane >= kGhost && lane < kGhost + kStripR && lin;
     const int  nstrips = gridDim.x;
 
-    const int len   = nsteps - 1;         // steps of the pass
+    const int len   = nsteps - 1; // steps of the pass
     const int start = FWD ? 1 : nsteps - 2;
     const int dir   = FWD ? 1 : -1;
     if (len <= 0)
       return;
     const int nchunks = (len + kGhost - 1) / kGhost;
 
-    const float ea = p.e_axis, ed = p.e_diag;
-    bool        any = false;
+    const float ea = p.e_axis;
+    // e_diag in a vector register, so the shifted distance and e_diag can
+    // form one DPP add (DPP takes no scalar operand).
+    float ed;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(ed) : "s"(p.e_diag));
+    const uint64_t own_bits = __ballot(own);
+    uint64_t       ch       = 0;
 
     // Values of the previous step (the step before the chunk's first).
-    float    Dprev = DTK_INF;
-    uint32_t Iprev = 0;
+    float Dprev = DTK_INF;
+    T     Iprev = T(0);
 
     // The chunk's own steps are loaded before waiting: no other strip
     // stores an owned line's pixels, and a ghost line read before or after
@@ -124,58 +204,92 @@ namespace dtk
       const int k0 = c * kGhost;
       const int kn = min(kGhost, len - k0);
 
-      float    Dv[kGhost];
-      uint32_t Iv[kGhost];
+      T     Iv[kGhost];
+      float Dv[kGhost];
+      if constexpr (CM != 0)
+      {
+#pragma unroll
+        for (int j = 0; j < kGhost; ++j)
+        {
+          const int s = start + dir * (k0 + min(j, kn - 1));
+          Iv[j]       = raster_img<KIND>(d, ROWS ? s : cl, ROWS ? cl : s);
+        }
+      }
 #pragma unroll
       for (int j = 0; j < kGhost; ++j)
       {
-        const int      s  = start + dir * (k0 + min(j, kn - 1));
-        const int      x = ROWS ? s : cl, y = ROWS ? cl : s;
-        const float    dv = gload<float>(d.D, (long long)y * d.D_pitch + 4ll * x);
-        const uint32_t iv = img_at<KIND>(d, x, y);
-        Dv[j]             = (j < kn && lin) ? dv : DTK_INF;
-        Iv[j]             = (j < kn && lin) ? iv : 0u;
+        const int   s  = start + dir * (k0 + min(j, kn - 1));
+        const int   x = ROWS ? s : cl, y = ROWS ? cl : s;
+        const float dv = gload<float>(d.D, (long long)y * d.D_pitch + 4ll * x);
+        Dv[j]          = (j < kn && lin) ? dv : DTK_INF;
       }
 
       // Previous step: from memory in the first chunk.  Afterwards every
       // lane keeps its image value (the last step's), owned lanes keep their
       // distance, and ghost lanes take the neighbour's final distance of the
       // step before this chunk from its mailbox.
+      // Lanes 0..G-1 are the last G owned lines of strip - 1 (its side 1),
+      // lanes R+G..63 the first G owned lines of strip + 1 (its side 0).
+      const bool   lower = lane < kGhost && strip > 0;
+      const bool   upper = lane >= kGhost + kStripR && strip + 1 < nstrips;
+      const bool   need  = c > 0 && (lower || upper) && lin;
+      const size_t at    = lower ? mail_index(strip - 1, 1, c & 1, lane)
+                                 : mail_index(strip + 1, 0, c & 1, max(lane - kGhost - kStripR, 0));
+      const uint32_t     want = epoch | (uint32_t)c;
+      unsigned long long w    = 0;
       if (c == 0)
       {
         const int s = start + dir * (k0 - 1);
         // Clamped unconditional loads, masked afterwards.
-        const int      x = ROWS ? s : cl, y = ROWS ? cl : s;
-        const float    dv = gload<float>(d.D, (long long)y * d.D_pitch + 4ll * x);
-        const uint32_t iv = img_at<KIND>(d, x, y);
-        Dprev             = lin ? dv : DTK_INF;
-        Iprev             = lin ? iv : 0u;
+        const int   x = ROWS ? s : cl, y = ROWS ? cl : s;
+        const float dv = gload<float>(d.D, (long long)y * d.D_pitch + 4ll * x);
+        Dprev          = lin ? dv : DTK_INF;
+        if constexpr (CM != 0)
+          Iprev = raster_img<KIND>(d, x, y);
       }
-      else
+      else if (need)
+        w = __hip_atomic_load(&mail[at], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+
+      // Edge costs of the chunk: centre, left (line - 1) and right
+      // (line + 1) neighbour of the previous step; at lambda 0 the chamfer
+      // steps.  +inf on lanes outside the image and past the pass's end.
+      float Gc[kGhost], Gl[kGhost], Gr[kGhost];
       {
-        // Lanes 0..G-1 are the last G owned lines of strip - 1 (its side 1),
-        // lanes R+G..63 the first G owned lines of strip + 1 (its side 0).
-        const bool lower = lane < kGhost && strip > 0;
-        const bool upper = lane >= kGhost + kStripR && strip + 1 < nstrips;
-        const bool need  = (lower || upper) && lin;
-        const size_t   at = lower ? mail_index(strip - 1, 1, c & 1, lane)
-                                  : mail_index(strip + 1, 0, c & 1, max(lane - kGhost - kStripR, 0));
-        const uint32_t want = epoch | (uint32_t)c;
-        bool           have = !need;
-        float          got  = DTK_INF;
-        for (uint32_t spin = 0;; ++spin)
+        T ip = Iprev;
+#pragma unroll
+        for (int j = 0; j < kGhost; ++j)
         {
-          if (!have)
+          const bool v = lin && j < kn;
+          if constexpr (CM == 0)
           {
-            const unsigned long long w = __hip_atomic_load(&mail[at], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if ((uint32_t)(w >> 32) == want)
-            {
-              have = true;
-              got  = __uint_as_float((uint32_t)w);
-            }
+            Gc[j] = v ? ea : DTK_INF;
+            Gl[j] = Gr[j] = v ? p.e_diag : DTK_INF;
           }
-          if (__ballot(!have) == 0)
-            break;
+          else
+          {
+            const T iv = Iv[j];
+            Gc[j]      = v ? raster_cost<KIND, CM>(iv, ip, p.l_grad) : DTK_INF;
+            if constexpr (C8)
+            {
+              // Shifted outside the selects: an opaque move cannot be
+              // evaluated speculatively, so inside them it would branch.
+              const T il = shr_img(ip), ir = shl_img(ip);
+              Gl[j]      = v ? raster_cost<KIND, CM>(iv, il, p.l_grad) : DTK_INF;
+              Gr[j]      = v ? raster_cost<KIND, CM>(iv, ir, p.l_grad) : DTK_INF;
+            }
+            ip = iv;
+          }
+        }
+        if constexpr (CM != 0)
+          Iprev = Iv[kGhost - 1];
+      }
+
+      if (c > 0)
+      {
+        bool  have = !need || (uint32_t)(w >> 32) == want;
+        float got  = __uint_as_float((uint32_t)w);
+        for (uint32_t spin = 0; __ballot(!have) != 0; ++spin)
+        {
           if (spin >= kSpinLimit)
           {
             // changed[1] is the error word of a raster call.
@@ -184,48 +298,33 @@ namespace dtk
             break;
           }
           __builtin_amdgcn_s_sleep(1);
+          if (!have)
+          {
+            w = __hip_atomic_load(&mail[at], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((uint32_t)(w >> 32) == want)
+            {
+              have = true;
+              got  = __uint_as_float((uint32_t)w);
+            }
+          }
         }
         if (need)
           Dprev = got;
       }
+
 #pragma unroll
       for (int j = 0; j < kGhost; ++j)
       {
-        if (j < kn)
+        // CM 0 and 1 add one term (the chamfer step, or g with e = 0).
+        float nd = minf_nn(Dv[j], CM == 2 ? (Dprev + ea) + Gc[j] : Dprev + Gc[j]);
+        if constexpr (C8)
         {
-          const uint32_t ip = Iv[j];
-          float          nd = Dv[j];
-          {
-            float g = p.l_grad * pix_dist<KIND>(ip, Iprev);
-            g       = (g == g) ? g : DTK_INF;
-            const float cost = p.zero_lambda ? ea : g;
-            const float cur  = TWO ? (Dprev + ea) + cost : Dprev + cost;
-            nd               = minf_nn(nd, cur);
-          }
-          if constexpr (C8)
-          {
-            const float    dl = shr_z(Dprev), dr = shl_z(Dprev);
-            const uint32_t il = shr_u(Iprev), ir = shl_u(Iprev);
-            float          gl = p.l_grad * pix_dist<KIND>(ip, il);
-            float          gr = p.l_grad * pix_dist<KIND>(ip, ir);
-            gl                = (gl == gl) ? gl : DTK_INF;
-            gr                = (gr == gr) ? gr : DTK_INF;
-            const float cl    = p.zero_lambda ? ed : gl;
-            const float cr    = p.zero_lambda ? ed : gr;
-            float       curl  = TWO ? (dl + ed) + cl : dl + cl;
-            float       curr  = TWO ? (dr + ed) + cr : dr + cr;
-            curl              = (linL && lane > 0) ? curl : DTK_INF;
-            curr              = (linR && lane < 63) ? curr : DTK_INF;
-            nd                = min3f(nd, curl, curr);
-          }
-          if (lin && lt_nn(nd, Dv[j]))
-          {
-            Dv[j] = nd;
-            any |= own;
-          }
-          Dprev = Dv[j];
-          Iprev = ip;
+          const float dl = shr_z(Dprev), dr = shl_z(Dprev);
+          nd = min3f(nd, CM == 2 ? (dl + ed) + Gl[j] : dl + Gl[j], CM == 2 ? (dr + ed) + Gr[j] : dr + Gr[j]);
         }
+        ch |= __ballot(lt_nn(nd, Dv[j]));
+        Dv[j] = nd;
+        Dprev = nd;
       }
 
       // Store the owned lines of the chunk and publish it.
@@ -245,50 +344,65 @@ namespace dtk
         }
       }
       // Publish the chunk's final values of the first and last G owned
-      // lines (Dprev holds the last step's value).
+      // lines (Dprev holds the last step's value; every chunk but the last
+      // has G steps).
       // First G owned lanes: [G, 2G) -> side 0; last G: [R, R + G) -> side 1.
       const bool first = lane >= kGhost && lane < 2 * kGhost;
       const bool last  = lane >= kStripR && lane < kStripR + kGhost;
       if (c + 1 < nchunks && (first || last))
       {
-        const unsigned long long w = ((unsigned long long)(epoch | (uint32_t)(c + 1)) << 32) | __float_as_uint(Dprev);
+        const unsigned long long wv = ((unsigned long long)(epoch | (uint32_t)(c + 1)) << 32) | __float_as_uint(Dprev);
         __hip_atomic_store(&mail[mail_index(strip, first ? 0 : 1, (c + 1) & 1, first ? lane - kGhost : lane - kStripR)],
-                           w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                           wv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
-    if (__ballot(any) != 0 && lane == 0)
+    if ((ch & own_bits) != 0 && lane == 0)
       __hip_atomic_store(changed, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 
   // pass0: index of the iteration's first pass within the call (1-based,
   // pass0 + 3 < 65536); the passes stamp their mailbox words with it.
-  template <int KIND, bool C8, bool TWO>
+  template <int KIND, bool C8, int CM>
   static hipError_t raster_iter_t(const ImgDesc& d, const KParams& p, uint32_t* changed, unsigned long long* mail,
                                   uint32_t pass0, hipStream_t s)
   {
     const int rs = (d.H + kStripR - 1) / kStripR; // row-pass strips
     const int cs = (d.W + kStripR - 1) / kStripR; // column-pass strips
-    hipLaunchKernelGGL((k_raster_pass<KIND, true, true, C8, TWO>), dim3(rs), dim3(64), 0, s, d, p, changed, mail,
+    hipLaunchKernelGGL((k_raster_pass<KIND, true, true, C8, CM>), dim3(rs), dim3(64), 0, s, d, p, changed, mail,
                        (pass0 + 0u) << 16);
-    hipLaunchKernelGGL((k_raster_pass<KIND, true, false, C8, TWO>), dim3(rs), dim3(64), 0, s, d, p, changed, mail,
+    hipLaunchKernelGGL((k_raster_pass<KIND, true, false, C8, CM>), dim3(rs), dim3(64), 0, s, d, p, changed, mail,
                        (pass0 + 1u) << 16);
-    hipLaunchKernelGGL((k_raster_pass<KIND, false, true, C8, TWO>), dim3(cs), dim3(64), 0, s, d, p, changed, mail,
+    hipLaunchKernelGGL((k_raster_pass<KIND, false, true, C8, CM>), dim3(cs), dim3(64), 0, s, d, p, changed, mail,
                        (pass0 + 2u) << 16);
-    hipLaunchKernelGGL((k_raster_pass<KIND, false, false, C8, TWO>), dim3(cs), dim3(64), 0, s, d, p, changed, mail,
+    hipLaunchKernelGGL((k_raster_pass<KIND, false, false, C8, CM>), dim3(cs), dim3(64), 0, s, d, p, changed, mail,
                        (pass0 + 3u) << 16);
     return hipGetLastError();
+  }
+
+  // Cost mode of a raster pass: 0 = lambda 0, 1 = lambda 1, 2 = otherwise.
+  static int raster_cm(const KParams& p) { return p.zero_lambda ? 0 : p.l_grad == 1.f ? 1 : 2; }
+
+  template <int KIND, bool C8>
+  static hipError_t raster_iter_c(const ImgDesc& d, const KParams& p, uint32_t* changed, unsigned long long* mail,
+                                  uint32_t pass0, hipStream_t s)
+  {
+    switch (raster_cm(p))
+    {
+    case 0:
+      return raster_iter_t<KIND, C8, 0>(d, p, changed, mail, pass0, s);
+    case 1:
+      return raster_iter_t<KIND, C8, 1>(d, p, changed, mail, pass0, s);
+    default:
+      return raster_iter_t<KIND, C8, 2>(d, p, changed, mail, pass0, s);
+    }
   }
 
   template <int KIND>
   static hipError_t raster_iter_k(const ImgDesc& d, const KParams& p, bool c8, uint32_t* changed,
                                   unsigned long long* mail, uint32_t pass0, hipStream_t s)
   {
-    const bool two = !(p.zero_lambda || p.l_grad == 1.f);
-    if (c8)
-      return two ? raster_iter_t<KIND, true, true>(d, p, changed, mail, pass0, s)
-                 : raster_iter_t<KIND, true, false>(d, p, changed, mail, pass0, s);
-    return two ? raster_iter_t<KIND, false, true>(d, p, changed, mail, pass0, s)
-               : raster_iter_t<KIND, false, false>(d, p, changed, mail, pass0, s);
+    return c8 ? raster_iter_c<KIND, true>(d, p, changed, mail, pass0, s)
+              : raster_iter_c<KIND, false>(d, p, changed, mail, pass0, s);
   }
 
   hipError_t launch_raster_init(const ImgDesc& d, const KParams& p, uint32_t* err, hipStream_t s)
@@ -313,40 +427,44 @@ namespace dtk
     }
   }
 
-  template <int KIND, bool C8, bool TWO>
+  template <int KIND, bool C8, int CM>
   static int raster_occ_t()
   {
     int n = 1 << 30, k = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, k_raster_pass<KIND, true, true, C8, TWO>, 64, 0) == hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, k_raster_pass<KIND, true, true, C8, CM>, 64, 0) == hipSuccess)
       n = min(n, k);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, k_raster_pass<KIND, true, false, C8, TWO>, 64, 0) == hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, k_raster_pass<KIND, true, false, C8, CM>, 64, 0) == hipSuccess)
       n = min(n, k);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, k_raster_pass<KIND, false, true, C8, TWO>, 64, 0) == hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, k_raster_pass<KIND, false, true, C8, CM>, 64, 0) == hipSuccess)
       n = min(n, k);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, k_raster_pass<KIND, false, false, C8, TWO>, 64, 0) == hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, k_raster_pass<KIND, false, false, C8, CM>, 64, 0) == hipSuccess)
       n = min(n, k);
     return n == (1 << 30) ? 0 : n;
   }
 
-  template <int KIND>
-  static int raster_occ_k(bool c8, bool two)
+  template <int KIND, bool C8>
+  static int raster_occ_c(int cm)
   {
-    if (c8)
-      return two ? raster_occ_t<KIND, true, true>() : raster_occ_t<KIND, true, false>();
-    return two ? raster_occ_t<KIND, false, true>() : raster_occ_t<KIND, false, false>();
+    return cm == 0 ? raster_occ_t<KIND, C8, 0>() : cm == 1 ? raster_occ_t<KIND, C8, 1>() : raster_occ_t<KIND, C8, 2>();
+  }
+
+  template <int KIND>
+  static int raster_occ_k(bool c8, int cm)
+  {
+    return c8 ? raster_occ_c<KIND, true>(cm) : raster_occ_c<KIND, false>(cm);
   }
 
   int raster_occupancy(const KParams& p, bool c8)
   {
-    const bool two = !(p.zero_lambda || p.l_grad == 1.f);
+    const int cm = raster_cm(p);
     switch (p.img_kind)
     {
     case IMG_U8:
-      return raster_occ_k<IMG_U8>(c8, two);
+      return raster_occ_k<IMG_U8>(c8, cm);
     case IMG_RGB8:
-      return raster_occ_k<IMG_RGB8>(c8, two);
+      return raster_occ_k<IMG_RGB8>(c8, cm);
     default:
-      return raster_occ_k<IMG_F32>(c8, two);
+      return raster_occ_k<IMG_F32>(c8, cm);
     }
   }
 

@@ -160,6 +160,36 @@ def test_fixed_iterations(dt, dev, oracle, iters, conn):
     assert_bit_exact(got, want)
 
 
+FIXED_CASES = [
+    # (img kind, mask kind, W, H, conn, lambda): every cost mode of the raster
+    # passes (lambda 0, 1, other), each image kind, C4 and C8; 177 columns
+    # give row passes of 176 steps (whole 16-step chunks), 97 rows column
+    # passes of 96, the other sizes a partial last chunk.
+    ("u8", "u8", 177, 97, 8, 0.0),
+    ("u8", "u8", 177, 97, 8, 1.0),
+    ("u8", "u8", 177, 97, 4, 0.5),
+    ("rgb8", "f32", 150, 70, 8, 0.5),
+    ("rgb8", "u8", 150, 70, 4, 1.0),
+    ("f32", "f32", 90, 140, 8, 0.3),
+    ("f32", "u8", 90, 140, 4, 0.0),
+]
+
+
+@pytest.mark.parametrize("iters", [1, 3])
+@pytest.mark.parametrize("case", FIXED_CASES, ids=lambda c: "-".join(map(str, c)))
+def test_fixed_iterations_kinds(dt, dev, oracle, case, iters):
+    """Intermediate states of the raster schedule (after 1 and 3 iterations)
+    match the CPU schedule bit for bit for every image kind and cost mode."""
+    ik, mk, W, H, conn, lam = case
+    seed = zlib.crc32(repr(case).encode()) & 0xFFFF
+    img, m = image(ik, W, H, seed, smooth=True), mask(mk, W, H, seed, 2)
+    v = 1e10 if mk == "u8" else 1e3
+    want, it = oracle.raster(img, m, lam, v, conn8=conn == 8, max_iters=iters)
+    _say("fixed kinds", case, iters)
+    got = _run(dt, dev, img, m, lam, v, "raster", connectivity=conn, max_iterations=iters)
+    assert_bit_exact(got, want, f"{case} iters={iters} (oracle ran {it})")
+
+
 @pytest.mark.parametrize("shape", [(1, 1), (1, 9), (9, 1), (62, 14), (63, 15), (64, 16), (2, 70), (70, 2),
                                    (62, 60), (63, 61), (64, 62), (70, 121), (125, 31), (130, 241), (1100, 70),
                                    (70, 1100)])
