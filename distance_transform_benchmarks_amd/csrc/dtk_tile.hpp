@@ -165,15 +165,17 @@ namespace dtk
       return fabsf(__uint_as_float(a) - __uint_as_float(b));
   }
 
-  // Cost stored for an edge: e (lambda == 0, single add), else g = l_grad*dist.
-  // A NaN g is stored as +inf: a NaN candidate never wins a strict compare,
-  // and neither does +inf, so both leave D unchanged.
+  // Cost term of an edge at lambda > 0: g = l_grad * dist, +inf when not
+  // valid.  (Lambda 0 takes the chamfer costs of setup_costs_const.)  A NaN
+  // g (float images only; lambda is finite, so integer kinds give finite g)
+  // is stored as +inf: a NaN candidate never wins a strict compare, and
+  // neither does +inf, so both leave D unchanged.
   template <int KIND>
-  __device__ __forceinline__ float edge_cost(bool valid, uint32_t a, uint32_t b, float e_dir, const KParams& p)
+  __device__ __forceinline__ float edge_cost(bool valid, uint32_t a, uint32_t b, const KParams& p)
   {
     float g = p.l_grad * pix_dist<KIND>(a, b);
-    g       = (g == g) ? g : DTK_INF;
-    g       = p.zero_lambda ? e_dir : g;
+    if constexpr (KIND == IMG_F32)
+      g = (g == g) ? g : DTK_INF;
     return valid ? g : DTK_INF;
   }
 
@@ -450,20 +452,19 @@ namespace dtk
     const bool cin = gx >= 0 && gx < d.W;
     const bool inL = lane >= 1 && gx - 1 >= 0 && gx - 1 < d.W;
     const bool inR = lane <= 62 && gx + 1 >= 0 && gx + 1 < d.W;
-    const float e  = (DIR == E_W || DIR == E_N) ? p.e_axis : p.e_diag;
 #pragma unroll
     for (int r = 0; r < RH; ++r)
     {
       const int  gy   = Y0 + r;
       const bool here = cin && gy >= 0 && gy < d.H;
       if constexpr (DIR == E_W)
-        out[r] = edge_cost<KIND>(here && inL, I[r], shr_u(I[r]), e, p);
+        out[r] = edge_cost<KIND>(here && inL, I[r], shr_u(I[r]), p);
       else if constexpr (DIR == E_SW)
       {
         if (r <= RH - 2)
         {
           const bool rinD = gy + 1 >= 0 && gy + 1 < d.H;
-          out[r]          = edge_cost<KIND>(here && inL && rinD, I[r], shr_u(I[r + 1]), e, p);
+          out[r]          = edge_cost<KIND>(here && inL && rinD, I[r], shr_u(I[r + 1]), p);
         }
         else
           out[r] = DTK_INF;
@@ -475,11 +476,11 @@ namespace dtk
           const bool     rinU = gy - 1 >= 0 && gy - 1 < d.H;
           const uint32_t u    = I[r - 1];
           if constexpr (DIR == E_N)
-            out[r] = edge_cost<KIND>(here && rinU, I[r], u, e, p);
+            out[r] = edge_cost<KIND>(here && rinU, I[r], u, p);
           else if constexpr (DIR == E_NW)
-            out[r] = edge_cost<KIND>(here && rinU && inL, I[r], shr_u(u), e, p);
+            out[r] = edge_cost<KIND>(here && rinU && inL, I[r], shr_u(u), p);
           else
-            out[r] = edge_cost<KIND>(here && rinU && inR, I[r], shl_u(u), e, p);
+            out[r] = edge_cost<KIND>(here && rinU && inR, I[r], shl_u(u), p);
         }
         else
           out[r] = DTK_INF;
@@ -572,12 +573,12 @@ namespace dtk
       const int      gy   = Y0 + r0 - 1 + k;
       const uint32_t cur  = I[k], up = I[k - 1];
       const bool     rinU = gy - 1 >= 0;
-      const float    cw   = edge_cost<KIND>(inL, cur, shr_u(cur), 0.f, p);
-      const float    cn   = edge_cost<KIND>(rinU, cur, up, 0.f, p);
+      const float    cw   = edge_cost<KIND>(inL, cur, shr_u(cur), p);
+      const float    cn   = edge_cost<KIND>(rinU, cur, up, p);
       if constexpr (C8)
       {
-        const float cnw = edge_cost<KIND>(rinU && inL, cur, shr_u(up), 0.f, p);
-        const float cne = edge_cost<KIND>(rinU && inR, cur, shl_u(up), 0.f, p);
+        const float cnw = edge_cost<KIND>(rinU && inL, cur, shr_u(up), p);
+        const float cne = edge_cost<KIND>(rinU && inR, cur, shl_u(up), p);
         if (own && gy < d.H)
         {
           f4 v = {cw, cn, cnw, cne};
