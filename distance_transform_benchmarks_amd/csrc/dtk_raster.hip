@@ -145,6 +145,64 @@ namespace dtk
     return g;
   }
 
+  // Sixteen consecutive pixels of one row, for a whole chunk of a row pass
+  // (a lane is a row there, so every scalar access of a step touches its own
+  // cache line).  Float rows are only 4-byte aligned, and 16-byte loads and
+  // stores need only dword alignment.
+  typedef float f4d __attribute__((ext_vector_type(4), aligned(4)));
+  typedef const __attribute__((address_space(1))) f4d* gf4d_r;
+  typedef __attribute__((address_space(1))) f4d*       gf4d_w;
+
+  __device__ __forceinline__ void load_f32_row16(const unsigned char* base, long long pitch, int y, int x0,
+                                                 float (&v)[16])
+  {
+    const gf4d_r q = reinterpret_cast<gf4d_r>((gbytes)base + (long long)y * pitch + 4ll * x0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+      const f4d t = q[k];
+      v[4 * k]     = t.x;
+      v[4 * k + 1] = t.y;
+      v[4 * k + 2] = t.z;
+      v[4 * k + 3] = t.w;
+    }
+  }
+
+  __device__ __forceinline__ void store_f32_row16(void* base, long long pitch, int y, int x0, const float (&v)[16])
+  {
+    const gf4d_w q = reinterpret_cast<gf4d_w>((gbytes_w)base + (long long)y * pitch + 4ll * x0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      q[k] = f4d{v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]};
+  }
+
+  // Bytes x0 .. x0+15 of a uint8 row as floats: the four or five aligned
+  // dwords that hold them, realigned with v_alignbyte.  The fifth dword is
+  // read only when x0 is not 4-aligned, and then it holds byte x0+15, so no
+  // dword without a pixel of the row is read.
+  __device__ __forceinline__ void load_u8_row16(const ImgDesc& d, int y, int x0, float (&v)[16])
+  {
+    const uintptr_t a  = reinterpret_cast<uintptr_t>(d.img + (long long)y * d.img_pitch + x0);
+    const uint32_t  sh = (uint32_t)(a & 3u);
+    const auto*     q  = reinterpret_cast<const __attribute__((address_space(1))) uint32_t*>(a - sh);
+    uint32_t        w[5];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      w[k] = q[k];
+    w[4] = 0u;
+    if (sh != 0u)
+      w[4] = q[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+      const uint32_t b = __builtin_amdgcn_alignbyte(w[k + 1], w[k], sh);
+      v[4 * k]         = (float)(b & 255u);
+      v[4 * k + 1]     = (float)((b >> 8) & 255u);
+      v[4 * k + 2]     = (float)((b >> 16) & 255u);
+      v[4 * k + 3]     = (float)(b >> 24);
+    }
+  }
+
   // ROWS: row pass (steps along x, lanes are rows); else column pass.
   // CM: 0 = lambda 0 (chamfer costs e_axis / e_diag, the image is not read),
   // 1 = lambda 1 (cur = D(q) + g), 2 = otherwise (cur = (D(q) + e) + g).
@@ -206,22 +264,52 @@ namespace dtk
 
       T     Iv[kGhost];
       float Dv[kGhost];
+      // A whole chunk of a row pass covers pixels xlo .. xlo+15 of each
+      // lane's row, in step order for a forward pass and reversed for a
+      // backward one; it is loaded (and stored) as row vectors.
+      const bool rowvec = ROWS && kn == kGhost;
+      const int  xlo    = FWD ? start + k0 : start - k0 - (kGhost - 1);
       if constexpr (CM != 0)
+      {
+        if (ROWS && KIND != IMG_RGB8 && rowvec)
+        {
+          float t[kGhost];
+          if constexpr (KIND == IMG_U8)
+            load_u8_row16(d, cl, xlo, t);
+          else
+            load_f32_row16(d.img, d.img_pitch, cl, xlo, t);
+#pragma unroll
+          for (int j = 0; j < kGhost; ++j)
+            Iv[j] = (T)t[FWD ? j : kGhost - 1 - j];
+        }
+        else
+        {
+#pragma unroll
+          for (int j = 0; j < kGhost; ++j)
+          {
+            const int s = start + dir * (k0 + min(j, kn - 1));
+            Iv[j]       = raster_img<KIND>(d, ROWS ? s : cl, ROWS ? cl : s);
+          }
+        }
+      }
+      if (rowvec)
+      {
+        float t[kGhost];
+        load_f32_row16(reinterpret_cast<const unsigned char*>(d.D), d.D_pitch, cl, xlo, t);
+#pragma unroll
+        for (int j = 0; j < kGhost; ++j)
+          Dv[j] = lin ? t[FWD ? j : kGhost - 1 - j] : DTK_INF;
+      }
+      else
       {
 #pragma unroll
         for (int j = 0; j < kGhost; ++j)
         {
-          const int s = start + dir * (k0 + min(j, kn - 1));
-          Iv[j]       = raster_img<KIND>(d, ROWS ? s : cl, ROWS ? cl : s);
+          const int   s  = start + dir * (k0 + min(j, kn - 1));
+          const int   x = ROWS ? s : cl, y = ROWS ? cl : s;
+          const float dv = gload<float>(d.D, (long long)y * d.D_pitch + 4ll * x);
+          Dv[j]          = (j < kn && lin) ? dv : DTK_INF;
         }
-      }
-#pragma unroll
-      for (int j = 0; j < kGhost; ++j)
-      {
-        const int   s  = start + dir * (k0 + min(j, kn - 1));
-        const int   x = ROWS ? s : cl, y = ROWS ? cl : s;
-        const float dv = gload<float>(d.D, (long long)y * d.D_pitch + 4ll * x);
-        Dv[j]          = (j < kn && lin) ? dv : DTK_INF;
       }
 
       // Previous step: from memory in the first chunk.  Afterwards every
@@ -328,7 +416,15 @@ namespace dtk
       }
 
       // Store the owned lines of the chunk and publish it.
-      if (own)
+      if (own && rowvec)
+      {
+        float t[kGhost];
+#pragma unroll
+        for (int j = 0; j < kGhost; ++j)
+          t[FWD ? j : kGhost - 1 - j] = Dv[j];
+        store_f32_row16(d.D, d.D_pitch, line, xlo, t);
+      }
+      else if (own)
       {
 #pragma unroll
         for (int j = 0; j < kGhost; ++j)
