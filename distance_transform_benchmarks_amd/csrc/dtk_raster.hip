@@ -433,9 +433,10 @@ namespace dtk
           {
             const int s = start + dir * (k0 + j);
             const int x = ROWS ? s : line, y = ROWS ? line : s;
-            // No other strip reads an owned pixel during the pass (ghost
-            // lanes take the mailbox values), so any store order works; a
-            // device-scope store and a plain one measured the same.
+            // Neighbouring strips load these pixels as ghost lines, and the
+            // value before or after this store gives them the same result,
+            // so no ordering is needed; a device-scope store and a plain one
+            // measured the same.
             __hip_atomic_store(&d_at(d, x, y), Dv[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           }
         }
