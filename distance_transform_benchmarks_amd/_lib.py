@@ -29,7 +29,7 @@ EXPORTS = (
     "dtk_ggdt_backward", "dtk_tile_count", "dtk_malloc_pitch", "dtk_free", "dtk_host_to_device",
     "dtk_device_to_host", "dtk_copy2d", "dtk_selftest", "dtk_device_cu_count", "dtk_profile_begin",
     "dtk_profile_end", "dtk_batch_autotune", "dtk_tile_count_for", "dtk_minmax_f32", "dtk_normalize_u8",
-    "dtk_inferno", "dtk_colorize",
+    "dtk_inferno", "dtk_colorize", "dtk_block_priorities",
 )
 
 
@@ -118,6 +118,7 @@ def load() -> ctypes.CDLL:
     lib.dtk_normalize_u8.argtypes = [P(DtkImage), P(DtkImage), vp]
     lib.dtk_inferno.argtypes = [P(DtkImage), P(DtkImage), vp]
     lib.dtk_colorize.argtypes = [P(DtkImage), P(DtkImage), vp]
+    lib.dtk_block_priorities.argtypes = [P(DtkImage), vp, vp, P(ctypes.c_uint32), vp]
     for name in ("dtk_ggdt", "dtk_ggdt_batched", "dtk_batch_plan_create", "dtk_batch_plan_launch",
                  "dtk_batch_plan_destroy", "dtk_batch_autotune", "dtk_ggdt_pred", "dtk_ggdt_backward", "dtk_malloc_pitch", "dtk_free",
                  "dtk_host_to_device", "dtk_device_to_host", "dtk_copy2d", "dtk_selftest", "dtk_profile_begin",

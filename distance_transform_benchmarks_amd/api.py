@@ -334,6 +334,28 @@ def inferno(img: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tens
     return out
 
 
+def block_priorities(mask: torch.Tensor):
+    """Tile priorities of the reference's task queue, dt::compute_block_priorities
+    (dt/include/dt/block_priority.hpp:7-8, dt/src/block_priority.cu:74-155), for
+    a uint8 or float32 device mask (nonzero = seed) cut into 32 x 32 tiles.
+
+    Returns (priorities, offsets, level0): priorities uint8 (ceil(H/32),
+    ceil(W/32)) = cdf[d] * 64 // (N + 1), with d the city-block distance in
+    tiles to the nearest seed tile and cdf[d] the number of tiles at distance
+    <= d; offsets int32 (64,), the number of tiles of priority < k; level0
+    the number of seed tiles.  Synchronous."""
+    H, W = mask.shape[0], mask.shape[1]
+    gh, gw = (H + 31) // 32, (W + 31) // 32
+    prio = torch.empty((gh, gw), dtype=torch.uint8, device=mask.device)
+    offsets = torch.zeros(64, dtype=torch.int32, device=mask.device)
+    level0 = ctypes.c_uint32(0)
+    with torch.cuda.device(mask.device):
+        check(_lib.load().dtk_block_priorities(ctypes.byref(_as_image(mask, "mask")), ctypes.c_void_p(prio.data_ptr()),
+                                               ctypes.c_void_p(offsets.data_ptr()), ctypes.byref(level0),
+                                               _stream_handle(mask.device)))
+    return prio, offsets, int(level0.value)
+
+
 def colorize(img: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
     """float32 (H, W) -> uint8 (H, W, 3): normalize_u8 then inferno in one pass."""
     if out is None:

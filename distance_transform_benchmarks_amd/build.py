@@ -35,7 +35,7 @@ COMMON = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
-DTK_SOURCES = ["dtk_task.hip", "dtk_raster.hip", "dtk_adjoint.hip", "dtk_color.hip", "dtk_host.cpp"]
+DTK_SOURCES = ["dtk_task.hip", "dtk_raster.hip", "dtk_adjoint.hip", "dtk_color.hip", "dtk_priority.hip", "dtk_host.cpp"]
 DT_SOURCES = ["dt_api.cpp", "dt_color.cpp", "dt_png.cpp"]
 # Command-line tools (the reference's bin/distance_transform.cpp and
 # bin/empiric_check.cpp), linked against libdt.

@@ -11,6 +11,7 @@
 #include <string>
 #include <type_traits>
 
+#include <dt/block_priority.hpp>
 #include <dt/generalised_distance_transform.hpp>
 #include <dt/transfert.hpp>
 #include <dtk.h>
@@ -255,4 +256,14 @@ namespace dt
   DT_INSTANTIATE(rgb8, float)
   DT_INSTANTIATE(float, std::uint8_t)
   DT_INSTANTIATE(float, float)
+  std::uint32_t compute_block_priorities(const image2d_view<std::uint8_t>& mask, std::uint8_t* priorities,
+                                         std::uint32_t* cdf)
+  {
+    if (mask.memory_kind() != e_memory_kind::GPU)
+      throw std::invalid_argument("compute_block_priorities: the mask must be a GPU image");
+    dtk_image     dm     = to_dtk(mask);
+    std::uint32_t level0 = 0;
+    check(dtk_block_priorities(&dm, priorities, cdf, &level0, nullptr), "compute_block_priorities");
+    return level0;
+  }
 } // namespace dt

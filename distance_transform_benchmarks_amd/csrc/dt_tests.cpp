@@ -20,6 +20,7 @@
 #include <string>
 #include <vector>
 
+#include <dt/block_priority.hpp>
 #include <dt/fill.hpp>
 #include <dt/generalised_distance_transform.hpp>
 #include <dt/image2d.hpp>
@@ -411,6 +412,40 @@ namespace
           EXPECT(col(x, y).r == d_col(x, y).r && col(x, y).g == d_col(x, y).g && col(x, y).b == d_col(x, y).b);
       const auto mm = dt::minmax(dist), d_mm = dt::minmax(d_dist);
       EXPECT(mm.first == d_mm.first && mm.second == d_mm.second);
+    });
+    run_test("BlockPriority.single_seed", [] {
+      // One seed pixel in tile (1, 1) of a 3 x 4 grid of 32 x 32 tiles:
+      // city-block distances 0..3 hold 1, 4, 5 and 2 tiles, so cdf is 1, 5,
+      // 10, 12 and the priorities cdf * 64 / 13 are 4, 24, 49, 59
+      // (dt/src/block_priority.cu:109-121).
+      dt::image2d<std::uint8_t> mask(70, 100);
+      dt::fill(mask, std::uint8_t(0));
+      mask(50, 40) = 1;
+      const auto     dm = dt::host_to_device(mask);
+      std::uint8_t*  prio = nullptr;
+      std::uint32_t* cdf  = nullptr;
+      EXPECT(hipMalloc(&prio, 12) == hipSuccess && hipMalloc(&cdf, 64 * sizeof(std::uint32_t)) == hipSuccess);
+      const std::uint32_t level0 = dt::compute_block_priorities(dm, prio, cdf);
+      std::uint8_t        hp[12];
+      std::uint32_t       hc[64];
+      EXPECT(hipMemcpy(hp, prio, 12, hipMemcpyDeviceToHost) == hipSuccess);
+      EXPECT(hipMemcpy(hc, cdf, sizeof(hc), hipMemcpyDeviceToHost) == hipSuccess);
+      hipFree(prio);
+      hipFree(cdf);
+      const std::uint8_t want[12] = {49, 24, 49, 24, 4, 24, 49, 24, 49, 59, 49, 59};
+      EXPECT(level0 == 1);
+      EXPECT(std::memcmp(hp, want, 12) == 0);
+      EXPECT(hc[0] == 0 && hc[5] == 1 && hc[25] == 5 && hc[50] == 10 && hc[60] == 12 && hc[63] == 12);
+      bool thrown = false;
+      try
+      {
+        dt::compute_block_priorities(mask, nullptr, nullptr);
+      }
+      catch (const std::invalid_argument&)
+      {
+        thrown = true;
+      }
+      EXPECT(thrown);
     });
     run_test("Errors.InvalidLambda", [] {
       dt::image2d<std::uint8_t> img(8, 8, dt::e_memory_kind::GPU), mask(8, 8, dt::e_memory_kind::GPU);

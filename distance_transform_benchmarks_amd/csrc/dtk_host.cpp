@@ -1219,6 +1219,47 @@ dtk_status dtk_copy2d(void* dst, long long dst_pitch, const void* src, long long
 }
 
 
+dtk_status dtk_block_priorities(const dtk_image* mask, uint8_t* priorities, uint32_t* offsets, uint32_t* level0,
+                                void* stream)
+{
+  dtk_status st;
+  if ((st = check_image(mask, "mask", (1 << DTK_U8) | (1 << DTK_F32))) != DTK_OK)
+    return st;
+  if (!level0)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "level0 is NULL");
+  *level0 = 0;
+  if ((long long)mask->width * mask->height == 0)
+    return DTK_OK;
+  if (!priorities || !offsets)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "priorities / offsets is NULL");
+  int gw, gh;
+  prio_grid(mask->width, mask->height, &gw, &gh);
+  if (gh > 65535)
+    return fail(DTK_ERR_UNSUPPORTED, "block priorities: at most 65535 tile rows");
+  const size_t n = (size_t)gw * gh;
+  Workspace*   w = nullptr;
+  if ((st = current_ws(&w, nullptr)) != DTK_OK)
+    return st;
+  std::lock_guard<std::mutex> lock(w->mu);
+  const size_t dist_bytes = align_up(n * sizeof(uint32_t), 256);
+  const size_t hist_bytes = align_up(((size_t)gw + gh) * sizeof(uint32_t), 256);
+  if ((st = ws_reserve(*w, 256 + dist_bytes + hist_bytes, 4096)) != DTK_OK)
+    return st;
+  char*       base = static_cast<char*>(w->dbuf);
+  uint32_t*   l0   = reinterpret_cast<uint32_t*>(base);
+  uint32_t*   dist = reinterpret_cast<uint32_t*>(base + 256);
+  uint32_t*   hist = reinterpret_cast<uint32_t*>(base + 256 + dist_bytes);
+  hipStream_t s    = static_cast<hipStream_t>(stream);
+  DTK_HIP(launch_block_priorities(mask->data, mask->pitch, mask->dtype == DTK_F32, mask->width, mask->height, dist,
+                                  hist, priorities, offsets, l0, s),
+          "launch block priorities");
+  uint32_t* h = static_cast<uint32_t*>(w->hbuf);
+  DTK_HIP(hipMemcpyAsync(h, l0, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read level0");
+  DTK_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
+  *level0 = h[0];
+  return DTK_OK;
+}
+
 dtk_status dtk_minmax_f32(const dtk_image* img, float* min_max, void* stream)
 {
   dtk_status st;

@@ -52,4 +52,11 @@ namespace dtk
   hipError_t launch_colorize(const void* in, long long in_pitch, void* out, long long out_pitch, int W, int H,
                              const uint32_t* keys, hipStream_t s);
   float      key_to_float(uint32_t k);
+
+  // dtk_priority.hip: tile priorities of 32 x 32 tiles (gw x gh of them).
+  // dist: gw * gh words; hist: gw + gh words; all device buffers.
+  void       prio_grid(int W, int H, int* gw, int* gh);
+  hipError_t launch_block_priorities(const void* mask, long long pitch, bool f32, int W, int H, uint32_t* dist,
+                                     uint32_t* hist, uint8_t* prio, uint32_t* offsets, uint32_t* level0,
+                                     hipStream_t s);
 } // namespace dtk

@@ -245,6 +245,23 @@ dtk_status dtk_normalize_u8(const dtk_image* in, dtk_image* out, void* stream);
 dtk_status dtk_inferno(const dtk_image* in, dtk_image* out, void* stream);
 dtk_status dtk_colorize(const dtk_image* in, dtk_image* out, void* stream);
 
+/* Tile priorities of the reference's task queue:
+ * dt::compute_block_priorities (dt/include/dt/block_priority.hpp:7-8,
+ * dt/src/block_priority.cu:74-155).  The image is cut into 32 x 32 tiles,
+ * gw = ceil(W / 32) by gh = ceil(H / 32), N = gw * gh, row-major.
+ *   mask        u8 (nonzero = seed) or f32 (nonzero = seed), on the device
+ *   priorities  device array of N bytes: cdf[d] * 64 / (N + 1), where d is
+ *               the tile's city-block distance (in tiles) to the nearest
+ *               tile holding a seed and cdf[d] the number of tiles at
+ *               distance <= d
+ *   offsets     device array of 64 words: offsets[k] = number of tiles with
+ *               priority < k (the reference's `cdf` argument)
+ *   level0      host word: the number of tiles holding a seed
+ * Synchronous on `stream` (it reads level0 back).  With no seed at all,
+ * every priority is N * 64 / (N + 1) and level0 is 0. */
+dtk_status dtk_block_priorities(const dtk_image* mask, uint8_t* priorities, uint32_t* offsets, uint32_t* level0,
+                                void* stream);
+
 #ifdef __cplusplus
 }
 #endif

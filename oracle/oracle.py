@@ -211,3 +211,50 @@ def normalize_u8(D: np.ndarray) -> np.ndarray:
 def inferno_rgb(u8: np.ndarray, table: np.ndarray) -> np.ndarray:
     """dt/src/inferno.cpp:263-274: out = table[v] for a (256, 3) table."""
     return np.asarray(table, np.uint8)[np.asarray(u8, np.uint8)]
+
+
+def block_priorities(mask: np.ndarray, return_distance: bool = False):
+    """dt/src/block_priority.cu:74-155 (dt::compute_block_priorities) on the
+    CPU.  32 x 32 tiles (dt/src/utils.cuh:11), gh x gw of them, n = gh * gw:
+    - distance 0 for a tile holding a nonzero mask pixel, else uint32 max / 2
+      (:84-91, initialize_distance :12-35);
+    - the city-block distance on the tile grid: every row forward then
+      backward, then every column forward then backward, d = min(d(prev) + 1, d)
+      (:37-71, :94-106);
+    - cdf[d] = number of tiles at distance <= d (:109-115), level0 = cdf[0];
+    - priority = cdf[d] * 64 / (n + 1) in uint32 arithmetic (:118-121);
+    - offsets[k] = number of tiles of priority < k (:125-148).
+    With no seed tile every distance stays far and every cdf value is n.
+    Returns (priorities (gh, gw) uint8, offsets (64,) uint32, level0[, d])."""
+    m = np.asarray(mask)
+    H, W = m.shape[0], m.shape[1]
+    gh, gw = (H + 31) // 32, (W + 31) // 32
+    n = gh * gw
+    far = 0x7FFFFFFF
+    if n == 0:
+        out = (np.zeros((gh, gw), np.uint8), np.zeros(64, np.uint32), 0)
+        return out + (np.zeros((gh, gw), np.int64),) if return_distance else out
+    seed_px = np.zeros((gh * 32, gw * 32), bool)
+    seed_px[:H, :W] = m != 0
+    seed = seed_px.reshape(gh, 32, gw, 32).any(axis=(1, 3))
+    d = np.where(seed, 0, far).astype(np.int64)
+    for x in range(1, gw):
+        d[:, x] = np.minimum(d[:, x - 1] + 1, d[:, x])
+    for x in range(gw - 2, -1, -1):
+        d[:, x] = np.minimum(d[:, x + 1] + 1, d[:, x])
+    for y in range(1, gh):
+        d[y] = np.minimum(d[y - 1] + 1, d[y])
+    for y in range(gh - 2, -1, -1):
+        d[y] = np.minimum(d[y + 1] + 1, d[y])
+    if d.max() >= far:
+        cdf_of = np.full(d.shape, n, np.uint64)
+        level0 = 0
+    else:
+        cdf = np.cumsum(np.bincount(d.ravel(), minlength=int(d.max()) + 1))
+        cdf_of = cdf[d].astype(np.uint64)
+        level0 = int(cdf[0])
+    prio = (((cdf_of * np.uint64(64)) % np.uint64(1 << 32)) // np.uint64(n + 1)).astype(np.uint8)
+    counts = np.bincount(prio.ravel(), minlength=64)[:64]
+    offsets = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.uint32)
+    out = (prio, offsets, level0)
+    return out + (d,) if return_distance else out
