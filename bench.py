@@ -79,6 +79,7 @@ def main(argv=None) -> int:
     ap.add_argument("--tile-waves", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched comparison")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable (adjoint) leg")
     ap.add_argument("--batch-count", type=int, default=256)
     a = ap.parse_args(argv)
 
@@ -171,6 +172,9 @@ def main(argv=None) -> int:
     if not a.no_batched:
         result["batched"] = bench_batched(dt, synth, torch, dev, a.batch_count)
 
+    if not a.no_adjoint:
+        result["adjoint"] = bench_adjoint(dt, synth, torch, dev)
+
     if not a.no_cpu:
         from oracle import oracle as O
 
@@ -210,6 +214,47 @@ def bench_display(dt, torch, D, npx, reps=20):
     return {"workload": "dtk_colorize of the lambda=0.5 map (min/max, normalise to uint8, inferno)", "ms": round(ms, 4),
             "mpix_s": round(npx / ms / 1e3, 1), "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes": 11 * npx}
+
+
+def bench_adjoint(dt, synth, torch, dev, count=32, size=1024, reps=3):
+    """The differentiable workload: 32 float32 1024^2 blob fields with soft
+    masks, C8, lambda=1, loss = sum(W * D).  Forward = batched converged
+    solve + predecessor codes; backward = the adjoint kernels, which read
+    pred 1 + dL/dD 4 + image 4 B/px and write dL/dimage 4 + dL/dmask 4 B/px."""
+    t = time.time()
+    imgs, masks, wts = synth.diff32(count, size)
+    ti = torch.from_numpy(imgs).to(dev).requires_grad_(True)
+    tm = torch.from_numpy(masks).to(dev).requires_grad_(True)
+    tw = torch.from_numpy(wts).to(dev)
+    log(f"adjoint inputs: {count} x {size}^2, generated in {time.time() - t:.1f}s")
+    fwd, bwd = [], []
+    for r in range(reps + 1):
+        ti.grad = None
+        tm.grad = None
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        torch.cuda.synchronize()
+        e[0].record()
+        D = dt.ggdt_differentiable(ti, tm, 1.0, 1e3)
+        loss = (D * tw).sum()
+        e[1].record()
+        loss.backward()
+        e[2].record()
+        torch.cuda.synchronize()
+        if r:  # the first pass warms up
+            fwd.append(e[0].elapsed_time(e[1]))
+            bwd.append(e[1].elapsed_time(e[2]))
+    f_ms, b_ms = min(fwd), min(bwd)
+    npx = count * size * size
+    gbs = 17 * npx / (b_ms * 1e-3) / 1e9
+    finite = bool(torch.isfinite(ti.grad).all() and torch.isfinite(tm.grad).all())
+    log(f"adjoint: forward {f_ms:.2f} ms, backward {b_ms:.2f} ms ({gbs:.0f} GB/s algorithmic), finite {finite}")
+    return {"workload": f"diff32: {count} float32 {size}x{size} blob fields + N(0, 1e-3) noise, soft masks, C8, "
+                        f"lambda=1, loss = sum(W * D), forward + backward through torch.autograd",
+            "pixels": npx, "forward_ms": round(f_ms, 3), "backward_ms": round(b_ms, 3),
+            "imgs_s": round(count / (f_ms + b_ms) * 1e3, 1),
+            "backward_achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "backward_frac": round(gbs / HBM_PEAK_GBS, 4), "backward_algorithmic_bytes": 17 * npx,
+            "grads_finite": finite}
 
 
 def bench_batched(dt, synth, torch, dev, count):

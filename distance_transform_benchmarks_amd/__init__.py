@@ -13,6 +13,7 @@ from .api import (  # noqa: F401
     BatchPlan,
     autotune,
     backward,
+    backward_batched,
     block_priorities,
     colorize,
     generalised_distance_transform,

@@ -281,6 +281,26 @@ def backward(img, mask, dist, pred, grad_dist, lambda_, v=1e10, *, connectivity=
     return gi, gm
 
 
+def backward_batched(imgs, masks, dists, preds, grad_dists, lambda_, v=1e10, *, connectivity=8):
+    """Adjoint of sum(grad_dists[i] * D_i) for a batch, one launch of each
+    adjoint kernel for all images (their predecessor chains climb together).
+    Sequences of equal length; returns (list of dL/dimg, list of dL/dmask)."""
+    n = len(imgs)
+    if not (len(masks) == len(dists) == len(preds) == len(grad_dists) == n):
+        raise ValueError("backward_batched: sequences of different lengths")
+    gis = [torch.empty_like(t, dtype=torch.float32) for t in imgs]
+    gms = [torch.empty(m.shape, dtype=torch.float32, device=m.device) for m in masks]
+    gds = [g.contiguous() for g in grad_dists]
+    p = _params(lambda_, v, connectivity, "task", 0, 0, 0, 0)
+    arr = lambda ts, name: (DtkImage * max(1, n))(*[_as_image(t, name) for t in ts])  # noqa: E731
+    A = [arr(imgs, "image"), arr(masks, "mask"), arr(dists, "dist"), arr(preds, "pred"), arr(gds, "grad_dist"),
+         arr(gis, "grad_img"), arr(gms, "grad_mask")]
+    if n:
+        with torch.cuda.device(imgs[0].device):
+            check(_lib.load().dtk_ggdt_backward_batched(n, *A, ctypes.byref(p), _stream_handle(imgs[0].device)))
+    return gis, gms
+
+
 class KernelTimer:
     """Sums the device time of the task-schedule kernels of every call made
     inside the ``with`` block (HIP events recorded on each call's stream)."""

@@ -2,8 +2,9 @@
 
 forward : converged GGDT (task schedule) of float32 images with soft float
           masks, then the argmin predecessor codes (saved for backward).
-backward: the adjoint kernel of libdtk (dtk_ggdt_backward) scatters dL/dD
-          along the predecessor forest into dL/dimage and dL/dmask.
+backward: the adjoint kernels of libdtk (dtk_ggdt_backward_batched, one
+          launch for the whole batch) scatter dL/dD along the predecessor
+          forest into dL/dimage and dL/dmask.
 Distances stay float32; rounding is treated as exact in the adjoint.
 No reference counterpart (the reference GGDT is forward-only).
 """
@@ -41,11 +42,10 @@ class GGDTFunction(torch.autograd.Function):
         imgs, masks, D, P = ctx.saved_tensors
         g = grad_D.unsqueeze(0) if ctx.squeeze else grad_D
         g = g.contiguous().float()
-        gi = torch.empty_like(imgs)
-        gm = torch.empty_like(masks)
-        for b in range(imgs.shape[0]):
-            gi[b], gm[b] = api.backward(imgs[b], masks[b], D[b], P[b], g[b], ctx.lambda_, ctx.nu,
+        gis, gms = api.backward_batched(list(imgs), list(masks), list(D), list(P), list(g), ctx.lambda_, ctx.nu,
                                         connectivity=ctx.connectivity)
+        gi = torch.stack(gis) if gis else torch.empty_like(imgs)
+        gm = torch.stack(gms) if gms else torch.empty_like(masks)
         if ctx.squeeze:
             gi, gm = gi[0], gm[0]
         return gi, gm, None, None, None

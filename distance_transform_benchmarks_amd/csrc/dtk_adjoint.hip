@@ -12,8 +12,14 @@
 //   root r      : dL/ds(r) = -nu * G(r)               (D0 = nu * (1 - s))
 // Subtree sums without a sort: every pixel counts its children; one thread
 // starts at each leaf and climbs while it is the last child to arrive at
-// the parent (atomic counter), so each pixel is finished exactly once,
-// after all of its children.
+// the parent, so each pixel is finished exactly once, after all of its
+// children.  A pixel's count of children still to arrive and the float sum
+// of their subtree sums share one 64-bit word, updated by a relaxed
+// compare-and-swap: the last child reads the complete sum from the value it
+// wrote itself, so no fence orders two addresses (an acquire-release
+// counter beside a separate sum costs an L2 write-back and invalidate per
+// step).  The kernels take a table of images (blockIdx.z), so the chains of
+// a whole batch climb in one launch.
 #include <hip/hip_runtime.h>
 
 #include "dtk_launch.hpp"
@@ -76,10 +82,12 @@ namespace dtk
   }
 
   // Children count of every pixel (gather, no atomics).
-  __global__ void k_child_count(ImgDesc d, uint32_t* counts, float* acc)
+  __global__ void k_child_count(const BackJob* jobs)
   {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
+    const BackJob& j = jobs[blockIdx.z];
+    const ImgDesc& d = j.d;
+    const int      x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int      y = blockIdx.y;
     if (x >= d.W || y >= d.H)
       return;
     uint32_t n = 0;
@@ -90,16 +98,20 @@ namespace dtk
         continue;
       n += d.pred[(long long)cy * d.pred_pitch + cx] == k;
     }
-    counts[(long long)y * d.W + x] = n;
-    acc[(long long)y * d.W + x]    = 0.f;
+    j.cw[(long long)y * d.W + x] = (unsigned long long)n << 32; // sum +0.0f
   }
 
   // Leaf-started climbs computing G.
-  __global__ void k_subtree(ImgDesc d, const unsigned char* gd, long long gd_pitch, uint32_t* counts, float* acc,
-                            float* G)
+  __global__ void k_subtree(const BackJob* jobs)
   {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    int y = blockIdx.y;
+    const BackJob&       j        = jobs[blockIdx.z];
+    const ImgDesc&       d        = j.d;
+    const unsigned char* gd       = j.gd;
+    const long long      gd_pitch = j.gd_pitch;
+    unsigned long long*  cw       = j.cw;
+    float*               G        = j.G;
+    int                  x        = blockIdx.x * blockDim.x + threadIdx.x;
+    int                  y        = blockIdx.y;
     if (x >= d.W || y >= d.H)
       return;
     long long id = (long long)y * d.W + x;
@@ -113,11 +125,10 @@ namespace dtk
       if (d.pred[(long long)cy * d.pred_pitch + cx] == k)
         return;
     }
+    float g = rd_f(gd, gd_pitch, x, y); // a leaf: G = W
     for (;;)
     {
-      const float a = __hip_atomic_load(&acc[id], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const float g = rd_f(gd, gd_pitch, x, y) + a;
-      G[id]         = g;
+      G[id]          = g;
       const int code = d.pred[(long long)y * d.pred_pitch + x];
       if (code >= 8 || x + kOffX[code] < 0 || y + kOffY[code] < 0 || x + kOffX[code] >= d.W ||
           y + kOffY[code] >= d.H)
@@ -125,11 +136,23 @@ namespace dtk
       x += kOffX[code];
       y += kOffY[code];
       id = (long long)y * d.W + x;
-      __hip_atomic_fetch_add(&acc[id], g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      // The last child to arrive continues with the parent; acq_rel orders
-      // every sibling's acc add before the parent's read of acc.
-      if (__hip_atomic_fetch_sub(&counts[id], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) != 1u)
-        return;
+      // One fewer child to wait for, g added to the sum.  Every failed
+      // exchange means a sibling's succeeded, and a pixel has at most eight
+      // children, so the loop ends.
+      unsigned long long old = __hip_atomic_load(&cw[id], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      unsigned long long nw;
+      for (;;)
+      {
+        const uint32_t left = (uint32_t)(old >> 32) - 1u;
+        const float    sum  = __uint_as_float((uint32_t)old) + g;
+        nw                  = ((unsigned long long)left << 32) | __float_as_uint(sum);
+        if (__hip_atomic_compare_exchange_strong(&cw[id], &old, nw, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+          break;
+      }
+      if ((uint32_t)(nw >> 32) != 0u)
+        return; // a sibling is still climbing: the last one continues
+      g = rd_f(gd, gd_pitch, x, y) + __uint_as_float((uint32_t)nw);
     }
   }
 
@@ -141,11 +164,17 @@ namespace dtk
     return __uint_as_float(load_pix<IMG_F32>(d.img + (long long)y * d.img_pitch, x));
   }
 
-  __global__ void k_grad(ImgDesc d, KParams p, const float* G, unsigned char* gi, long long gi_pitch,
-                         unsigned char* gm, long long gm_pitch)
+  __global__ void k_grad(const BackJob* jobs, KParams p)
   {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
+    const BackJob&  j        = jobs[blockIdx.z];
+    const ImgDesc&  d        = j.d;
+    const float*    G        = j.G;
+    unsigned char*  gi       = j.gi;
+    const long long gi_pitch = j.gi_pitch;
+    unsigned char*  gm       = j.gm;
+    const long long gm_pitch = j.gm_pitch;
+    const int       x        = blockIdx.x * blockDim.x + threadIdx.x;
+    const int       y        = blockIdx.y;
     if (x >= d.W || y >= d.H)
       return;
     const long long id   = (long long)y * d.W + x;
@@ -194,17 +223,16 @@ namespace dtk
     return hipGetLastError();
   }
 
-  hipError_t launch_backward(const ImgDesc& d, const KParams& p, bool c8, const unsigned char* grad_dist,
-                             long long gd_pitch, unsigned char* grad_img, long long gi_pitch, unsigned char* grad_mask,
-                             long long gm_pitch, float* G, uint32_t* counts, float* acc, hipStream_t s)
+  // jobs: n device entries; the grid covers the widest and tallest image
+  // (maxW x maxH), and every block of a smaller image past its edge returns.
+  hipError_t launch_backward(const BackJob* jobs, int n, int maxW, int maxH, const KParams& p, hipStream_t s)
   {
-    (void)c8;
-    if (d.W <= 0 || d.H <= 0)
+    if (n <= 0 || maxW <= 0 || maxH <= 0)
       return hipSuccess;
-    const dim3 grid((d.W + 255) / 256, d.H), blk(256);
-    hipLaunchKernelGGL(k_child_count, grid, blk, 0, s, d, counts, acc);
-    hipLaunchKernelGGL(k_subtree, grid, blk, 0, s, d, grad_dist, gd_pitch, counts, acc, G);
-    hipLaunchKernelGGL(k_grad, grid, blk, 0, s, d, p, G, grad_img, gi_pitch, grad_mask, gm_pitch);
+    const dim3 grid((maxW + 255) / 256, maxH, n), blk(256);
+    hipLaunchKernelGGL(k_child_count, grid, blk, 0, s, jobs);
+    hipLaunchKernelGGL(k_subtree, grid, blk, 0, s, jobs);
+    hipLaunchKernelGGL(k_grad, grid, blk, 0, s, jobs, p);
     return hipGetLastError();
   }
 

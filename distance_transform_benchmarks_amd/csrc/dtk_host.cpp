@@ -511,6 +511,85 @@ namespace
     DTK_HIP(launch_minmax(in->data, in->pitch, in->width, in->height, keys, partial, w->cu, s), "launch minmax");
     return then(keys);
   }
+  // The adjoint of n images in one launch of each kernel (see BackJob).
+  // With have_gm, grad_masks[i] receives dL/dmask (zeros for u8 masks).
+  dtk_status backward_impl(int n, const dtk_image* imgs, const dtk_image* masks, const dtk_image* dists,
+                           const dtk_image* preds, const dtk_image* gds, dtk_image* gis, dtk_image* gms, bool have_gm,
+                           const dtk_params* p, void* stream)
+  {
+    dtk_status st;
+    if ((st = check_params(p)) != DTK_OK)
+      return st;
+    size_t dbytes = 0;
+    int    maxW = 0, maxH = 0;
+    for (int i = 0; i < n; ++i)
+    {
+      if ((st = check_triplet(&imgs[i], &masks[i], dists ? &dists[i] : nullptr)) != DTK_OK)
+        return st;
+      if (imgs[i].dtype != DTK_F32)
+        return fail(DTK_ERR_UNSUPPORTED, "the adjoint needs a float32 image");
+      if (masks[i].dtype != masks[0].dtype)
+        return fail(DTK_ERR_INVALID_ARGUMENT, "the masks of a batch must share a dtype");
+      if ((st = check_image(&preds[i], "pred", 1 << DTK_U8)) != DTK_OK ||
+          (st = check_image(&gds[i], "grad_dist", 1 << DTK_F32)) != DTK_OK ||
+          (st = check_image(&gis[i], "grad_img", 1 << DTK_F32)) != DTK_OK)
+        return st;
+      if (have_gm && (st = check_image(&gms[i], "grad_mask", 1 << DTK_F32)) != DTK_OK)
+        return st;
+      const int W = imgs[i].width, H = imgs[i].height;
+      if (preds[i].width != W || preds[i].height != H || gds[i].width != W || gds[i].height != H ||
+          gis[i].width != W || gis[i].height != H || (have_gm && (gms[i].width != W || gms[i].height != H)))
+        return fail(DTK_ERR_INVALID_ARGUMENT, "gradient image sizes differ");
+      if (H > 65535)
+        return fail(DTK_ERR_UNSUPPORTED, "the adjoint takes at most 65535 rows");
+      maxW = std::max(maxW, W);
+      maxH = std::max(maxH, H);
+      dbytes += align_up((size_t)W * H * 4, 256) + align_up((size_t)W * H * 8, 256);
+    }
+    if (n == 0 || maxW == 0 || maxH == 0)
+      return DTK_OK;
+    Workspace* w = nullptr;
+    if ((st = current_ws(&w, nullptr)) != DTK_OK)
+      return st;
+    std::lock_guard<std::mutex> lock(w->mu);
+    const size_t jbytes = align_up((size_t)n * sizeof(BackJob), 256);
+    if ((st = ws_reserve(*w, dbytes + jbytes + 1024, std::max<size_t>(4096, jbytes))) != DTK_OK)
+      return st;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    Carver      c(w->dbuf);
+    BackJob*    djobs = c.take<BackJob>(n);
+    BackJob*    hjobs = static_cast<BackJob*>(w->hbuf);
+    const bool  soft  = masks[0].dtype == DTK_F32;
+    for (int i = 0; i < n; ++i)
+    {
+      const size_t npx = (size_t)imgs[i].width * imgs[i].height;
+      dtk_image    dc  = dists ? dists[i] : dtk_image{};
+      BackJob      j{};
+      j.d            = make_desc(&imgs[i], &masks[i], &dc, 32, 1);
+      j.d.pred       = static_cast<unsigned char*>(preds[i].data);
+      j.d.pred_pitch = preds[i].pitch;
+      j.gd           = static_cast<const unsigned char*>(gds[i].data);
+      j.gd_pitch     = gds[i].pitch;
+      j.gi           = static_cast<unsigned char*>(gis[i].data);
+      j.gi_pitch     = gis[i].pitch;
+      j.gm           = have_gm && soft ? static_cast<unsigned char*>(gms[i].data) : nullptr;
+      j.gm_pitch     = have_gm ? gms[i].pitch : 0;
+      j.G            = c.take<float>(npx);
+      j.cw           = c.take<unsigned long long>(npx);
+      hjobs[i]       = j;
+    }
+    DTK_HIP(hipMemcpyAsync(djobs, hjobs, (size_t)n * sizeof(BackJob), hipMemcpyHostToDevice, s), "upload jobs");
+    // The staging buffer is reused by the next call: let the copy finish.
+    DTK_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
+    const KParams kp = make_kparams(p, DTK_F32, masks[0].dtype, 32);
+    DTK_HIP(launch_backward(djobs, n, maxW, maxH, kp, s), "launch backward");
+    if (have_gm && !soft)
+      for (int i = 0; i < n; ++i)
+        if ((size_t)imgs[i].width * imgs[i].height > 0)
+          DTK_HIP(hipMemset2DAsync(gms[i].data, gms[i].pitch, 0, (size_t)imgs[i].width * 4, imgs[i].height, s),
+                  "zero grad_mask");
+    return DTK_OK;
+  }
 } // namespace
 
 extern "C" {
@@ -1069,52 +1148,23 @@ dtk_status dtk_ggdt_backward(const dtk_image* img, const dtk_image* mask, const 
                              const dtk_image* pred, const dtk_image* grad_dist, dtk_image* grad_img,
                              dtk_image* grad_mask, const dtk_params* p, void* stream)
 {
-  dtk_status st;
-  if ((st = check_params(p)) != DTK_OK)
-    return st;
+  if (!img || !mask || !pred || !grad_dist || !grad_img)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "NULL image");
   dtk_image dcopy = dist ? *dist : dtk_image{};
-  if ((st = check_triplet(img, mask, dist ? &dcopy : nullptr)) != DTK_OK)
-    return st;
-  if (img->dtype != DTK_F32)
-    return fail(DTK_ERR_UNSUPPORTED, "the adjoint needs a float32 image");
-  if ((st = check_image(pred, "pred", 1 << DTK_U8)) != DTK_OK ||
-      (st = check_image(grad_dist, "grad_dist", 1 << DTK_F32)) != DTK_OK ||
-      (st = check_image(grad_img, "grad_img", 1 << DTK_F32)) != DTK_OK)
-    return st;
-  if (grad_mask && (st = check_image(grad_mask, "grad_mask", 1 << DTK_F32)) != DTK_OK)
-    return st;
-  const int W = img->width, H = img->height;
-  if (pred->width != W || pred->height != H || grad_dist->width != W || grad_dist->height != H ||
-      grad_img->width != W || grad_img->height != H ||
-      (grad_mask && (grad_mask->width != W || grad_mask->height != H)))
-    return fail(DTK_ERR_INVALID_ARGUMENT, "gradient image sizes differ");
-  if (W == 0 || H == 0)
-    return DTK_OK;
-  Workspace* w = nullptr;
-  if ((st = current_ws(&w, nullptr)) != DTK_OK)
-    return st;
-  std::lock_guard<std::mutex> lock(w->mu);
-  const size_t npx = (size_t)W * H;
-  if ((st = ws_reserve(*w, 3 * align_up(npx * 4, 256) + 1024, 4096)) != DTK_OK)
-    return st;
-  Carver    c(w->dbuf);
-  float*    G      = c.take<float>(npx);
-  uint32_t* counts = c.take<uint32_t>(npx);
-  float*    acc    = c.take<float>(npx);
-  ImgDesc   d      = make_desc(img, mask, &dcopy, 32, 1);
-  d.pred           = static_cast<unsigned char*>(pred->data);
-  d.pred_pitch     = pred->pitch;
-  KParams kp       = make_kparams(p, img->dtype, mask->dtype, 32);
-  const bool soft  = mask->dtype == DTK_F32;
-  DTK_HIP(launch_backward(d, kp, p->connectivity == DTK_C8, static_cast<const unsigned char*>(grad_dist->data),
-                          grad_dist->pitch, static_cast<unsigned char*>(grad_img->data), grad_img->pitch,
-                          soft && grad_mask ? static_cast<unsigned char*>(grad_mask->data) : nullptr,
-                          grad_mask ? grad_mask->pitch : 0, G, counts, acc, static_cast<hipStream_t>(stream)),
-          "launch backward");
-  if (grad_mask && !soft)
-    DTK_HIP(hipMemset2DAsync(grad_mask->data, grad_mask->pitch, 0, (size_t)W * 4, H, static_cast<hipStream_t>(stream)),
-            "zero grad_mask");
-  return DTK_OK;
+  return backward_impl(1, img, mask, dist ? &dcopy : nullptr, pred, grad_dist, grad_img, grad_mask,
+                       grad_mask != nullptr, p, stream);
+}
+
+dtk_status dtk_ggdt_backward_batched(int n, const dtk_image* imgs, const dtk_image* masks, const dtk_image* dists,
+                                     const dtk_image* preds, const dtk_image* grad_dists, dtk_image* grad_imgs,
+                                     dtk_image* grad_masks, const dtk_params* p, void* stream)
+{
+  if (n < 0)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "negative image count");
+  if (n > 0 && (!imgs || !masks || !dists || !preds || !grad_dists || !grad_imgs))
+    return fail(DTK_ERR_INVALID_ARGUMENT, "NULL image array");
+  return backward_impl(n, imgs, masks, dists, preds, grad_dists, grad_imgs, grad_masks, grad_masks != nullptr, p,
+                       stream);
 }
 
 dtk_status dtk_malloc_pitch(void** ptr, long long* pitch, long long width_bytes, int height)

@@ -50,6 +50,24 @@ namespace dtk
     int                  pad_;
   };
 
+  // One image of an adjoint launch (dtk_ggdt_backward[_batched]): its
+  // inputs (d: image, mask, D, predecessor codes), dL/dD, the outputs and
+  // its scratch (subtree sums G; per pixel a word holding the count of
+  // children still to arrive (high 32 bits) and the float sum of their
+  // subtree sums so far (low 32 bits)), all device.
+  struct BackJob
+  {
+    ImgDesc              d;
+    const unsigned char* gd;
+    long long            gd_pitch;
+    unsigned char*       gi;
+    long long            gi_pitch;
+    unsigned char*       gm; // NULL: no dL/dmask
+    long long            gm_pitch;
+    float*               G;
+    unsigned long long*  cw;
+  };
+
   // A call: one image (d0) or a batch described by a device table.
   struct Batch
   {

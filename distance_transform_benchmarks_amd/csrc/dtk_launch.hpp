@@ -33,9 +33,7 @@ namespace dtk
 
   // dtk_adjoint.hip
   hipError_t launch_pred(const ImgDesc& d, const KParams& p, bool c8, hipStream_t s);
-  hipError_t launch_backward(const ImgDesc& d, const KParams& p, bool c8, const unsigned char* grad_dist,
-                             long long gd_pitch, unsigned char* grad_img, long long gi_pitch, unsigned char* grad_mask,
-                             long long gm_pitch, float* G, uint32_t* counts, float* acc, hipStream_t s);
+  hipError_t launch_backward(const BackJob* jobs, int n, int maxW, int maxH, const KParams& p, hipStream_t s);
 
   // dtk_selftest.hip
   hipError_t launch_selftest(int* dpp_out, float* sqrt_out, hipStream_t s);

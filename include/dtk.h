@@ -183,6 +183,16 @@ dtk_status dtk_ggdt_backward(const dtk_image* img, const dtk_image* mask, const 
                              const dtk_image* pred, const dtk_image* grad_dist, dtk_image* grad_img,
                              dtk_image* grad_mask, const dtk_params* p, void* stream);
 
+/* The adjoint of n images in one launch of each kernel: arrays of n
+ * descriptors, as for dtk_ggdt_backward (grad_masks may be NULL; every mask
+ * of the batch has the same dtype).  The subtree sums
+ * climb predecessor chains with two atomics per step, so an image costs as
+ * long as its longest chain; in one launch the chains of all images climb
+ * together.  No reference counterpart. */
+dtk_status dtk_ggdt_backward_batched(int n, const dtk_image* imgs, const dtk_image* masks, const dtk_image* dists,
+                                     const dtk_image* preds, const dtk_image* grad_dists, dtk_image* grad_imgs,
+                                     dtk_image* grad_masks, const dtk_params* p, void* stream);
+
 /* Tiles of the blocks schedule on a width x height image with single-wave
  * regions of tile_rows rows (0 = 32): the size of round_per_blocks for
  * dt::generalised_distance_transform_blocks

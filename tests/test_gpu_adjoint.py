@@ -121,3 +121,38 @@ def test_autograd_equals_direct_adjoint(dt, dev):
     D2.backward(_t(g[0], dev))
     assert_rel(t2i.grad.cpu().numpy(), ti.grad[0].cpu().numpy(), 1e-4, "2-D input")
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("mk", ["f32", "u8"])
+def test_backward_batched_equals_per_image(dt, dev, mk):
+    """dtk_ggdt_backward_batched (one launch per adjoint kernel for a batch of
+    uneven images) gives the per-image dtk_ggdt_backward results.  The
+    subtree sums add children in arrival order (float atomics), so the two
+    agree to 1e-5 of the largest gradient rather than bit for bit; u8 masks
+    give zero dL/dmask."""
+    import torch
+
+    rng = np.random.default_rng(41)
+    shapes = ((80, 64), (50, 120), (33, 33), (1, 17))
+    imgs, masks, Ds, Ps, Ws = [], [], [], [], []
+    for i, (H, W) in enumerate(shapes):
+        I = rng.random((H, W)).astype(np.float32) + 1e-3 * rng.normal(size=(H, W)).astype(np.float32)
+        M = mask(mk, W, H, 100 + i, 2)
+        di, dm = _t(I, dev), _t(M, dev)
+        nu = 1e3 if mk == "f32" else 1e10
+        D = dt.ggdt(di, dm, 1.0, nu)
+        imgs.append(di)
+        masks.append(dm)
+        Ds.append(D)
+        Ps.append(dt.predecessors(di, dm, D, 1.0, nu))
+        Ws.append(_t(rng.random((H, W)).astype(np.float32), dev))
+    _say("backward batched", mk, shapes)
+    gis, gms = dt.backward_batched(imgs, masks, Ds, Ps, Ws, 1.0, nu)
+    for i in range(len(shapes)):
+        gi, gm = dt.backward(imgs[i], masks[i], Ds[i], Ps[i], Ws[i], 1.0, nu)
+        torch.cuda.synchronize()
+        for got, want in ((gis[i], gi), (gms[i], gm)):
+            a, b = got.cpu().numpy(), want.cpu().numpy()
+            assert np.abs(a - b).max() <= 1e-5 * max(1.0, np.abs(b).max()), (i, np.abs(a - b).max())
+        if mk == "u8":
+            assert not gms[i].any()
