@@ -18,8 +18,12 @@
 // compare-and-swap: the last child reads the complete sum from the value it
 // wrote itself, so no fence orders two addresses (an acquire-release
 // counter beside a separate sum costs an L2 write-back and invalidate per
-// step).  The kernels take a table of images (blockIdx.z), so the chains of
-// a whole batch climb in one launch.
+// step).  Most pixels have a single child, and a single child is always
+// the last to arrive: k_child_count also writes an immutable byte per pixel
+// (child count << 4 | predecessor code), and a climb through a pixel with
+// one child reads only that byte (a plain load) and moves on.  The kernels
+// take a table of images (blockIdx.z), so the chains of a whole batch climb
+// in one launch.
 #include <hip/hip_runtime.h>
 
 #include "dtk_launch.hpp"
@@ -99,6 +103,10 @@ namespace dtk
       n += d.pred[(long long)cy * d.pred_pitch + cx] == k;
     }
     j.cw[(long long)y * d.W + x] = (unsigned long long)n << 32; // sum +0.0f
+    int code = d.pred[(long long)y * d.pred_pitch + x];
+    if (code < 8 && (x + kOffX[code] < 0 || y + kOffY[code] < 0 || x + kOffX[code] >= d.W || y + kOffY[code] >= d.H))
+      code = 8; // invalid code: treated as a root
+    j.meta[(long long)y * d.W + x] = (unsigned char)(n << 4 | (uint32_t)code);
   }
 
   // Leaf-started climbs computing G.
@@ -114,28 +122,28 @@ namespace dtk
     int                  y        = blockIdx.y;
     if (x >= d.W || y >= d.H)
       return;
-    long long id = (long long)y * d.W + x;
-    // Leaves are decided from the (immutable) predecessor map: the counters
-    // are being decremented by climbing threads while this kernel runs.
-    for (int k = 0; k < 8; ++k)
-    {
-      const int cx = x - kOffX[k], cy = y - kOffY[k];
-      if (cx < 0 || cy < 0 || cx >= d.W || cy >= d.H)
-        continue;
-      if (d.pred[(long long)cy * d.pred_pitch + cx] == k)
-        return;
-    }
+    long long            id   = (long long)y * d.W + x;
+    const unsigned char* meta = j.meta; // immutable here: the words in cw change
+    uint32_t             m    = meta[id];
+    if ((m >> 4) != 0u)
+      return; // not a leaf
     float g = rd_f(gd, gd_pitch, x, y); // a leaf: G = W
     for (;;)
     {
       G[id]          = g;
-      const int code = d.pred[(long long)y * d.pred_pitch + x];
-      if (code >= 8 || x + kOffX[code] < 0 || y + kOffY[code] < 0 || x + kOffX[code] >= d.W ||
-          y + kOffY[code] >= d.H)
-        return; // root (or an invalid code, treated as a root)
+      const int code = (int)(m & 15u);
+      if (code >= 8)
+        return; // root (k_child_count marked invalid codes as roots)
       x += kOffX[code];
       y += kOffY[code];
       id = (long long)y * d.W + x;
+      m  = meta[id];
+      if ((m >> 4) == 1u)
+      {
+        // The only child: the last to arrive, and the sum is g alone.
+        g = rd_f(gd, gd_pitch, x, y) + g;
+        continue;
+      }
       // One fewer child to wait for, g added to the sum.  Every failed
       // exchange means a sibling's succeeded, and a pixel has at most eight
       // children, so the loop ends.

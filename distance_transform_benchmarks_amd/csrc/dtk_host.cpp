@@ -544,7 +544,7 @@ namespace
         return fail(DTK_ERR_UNSUPPORTED, "the adjoint takes at most 65535 rows");
       maxW = std::max(maxW, W);
       maxH = std::max(maxH, H);
-      dbytes += align_up((size_t)W * H * 4, 256) + align_up((size_t)W * H * 8, 256);
+      dbytes += align_up((size_t)W * H * 4, 256) + align_up((size_t)W * H * 8, 256) + align_up((size_t)W * H, 256);
     }
     if (n == 0 || maxW == 0 || maxH == 0)
       return DTK_OK;
@@ -576,6 +576,7 @@ namespace
       j.gm_pitch     = have_gm ? gms[i].pitch : 0;
       j.G            = c.take<float>(npx);
       j.cw           = c.take<unsigned long long>(npx);
+      j.meta         = c.take<unsigned char>(npx);
       hjobs[i]       = j;
     }
     DTK_HIP(hipMemcpyAsync(djobs, hjobs, (size_t)n * sizeof(BackJob), hipMemcpyHostToDevice, s), "upload jobs");

@@ -66,6 +66,7 @@ namespace dtk
     long long            gm_pitch;
     float*               G;
     unsigned long long*  cw;
+    unsigned char*       meta; // per pixel: child count << 4 | predecessor code
   };
 
   // A call: one image (d0) or a batch described by a device table.
