@@ -56,30 +56,34 @@ namespace dtk
     unsigned char code = 8;
     if (!(Dp == d0))
     {
-      const uint32_t    ip  = load_pix<KIND>(d.img + (long long)y * d.img_pitch, x);
-      const long long   pid = (long long)y * d.W + x;
+      // Every neighbour's D and pixel are loaded first, from clamped (in
+      // bounds) addresses, so the loads are in flight together; then the
+      // neighbours are tested in dt::c8 order and the first match wins.
+      const uint32_t  ip  = load_pix<KIND>(d.img + (long long)y * d.img_pitch, x);
+      const long long pid = (long long)y * d.W + x;
+      float           Dq[8];
+      uint32_t        iq[8];
+#pragma unroll
       for (int k = 0; k < 8; ++k)
       {
+        const int nx = min(max(x + kOffX[k], 0), d.W - 1), ny = min(max(y + kOffY[k], 0), d.H - 1);
+        Dq[k]        = rd_f(Db, d.D_pitch, nx, ny);
+        iq[k]        = load_pix<KIND>(d.img + (long long)ny * d.img_pitch, nx);
+      }
+#pragma unroll
+      for (int k = 7; k >= 0; --k)
+      {
         const bool diag = kOffX[k] != 0 && kOffY[k] != 0;
-        if (diag && !c8)
-          continue;
-        const int nx = x + kOffX[k], ny = y + kOffY[k];
-        if (nx < 0 || ny < 0 || nx >= d.W || ny >= d.H)
-          continue;
-        const float     Dq  = rd_f(Db, d.D_pitch, nx, ny);
+        const int  nx = x + kOffX[k], ny = y + kOffY[k];
+        const bool in = (!diag || c8) && nx >= 0 && ny >= 0 && nx < d.W && ny < d.H;
         const long long qid = (long long)ny * d.W + nx;
-        if (!(Dq < Dp || (Dq == Dp && qid < pid)))
-          continue;
-        const uint32_t iq = load_pix<KIND>(d.img + (long long)ny * d.img_pitch, nx);
-        float          g  = p.l_grad * pix_dist<KIND>(ip, iq);
-        g                 = (g == g) ? g : DTK_INF;
-        float cur         = Dq + (diag ? p.e_diag : p.e_axis);
-        cur               = cur + g;
-        if (cur == Dp)
-        {
+        float           g   = p.l_grad * pix_dist<KIND>(ip, iq[k]);
+        g                   = (g == g) ? g : DTK_INF;
+        float cur           = Dq[k] + (diag ? p.e_diag : p.e_axis);
+        cur                 = cur + g;
+        // Walking k downwards, the last match kept is the first in c8 order.
+        if (in && (Dq[k] < Dp || (Dq[k] == Dp && qid < pid)) && cur == Dp)
           code = (unsigned char)k;
-          break;
-        }
       }
     }
     d.pred[(long long)y * d.pred_pitch + x] = code;
