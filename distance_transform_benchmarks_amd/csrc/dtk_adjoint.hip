@@ -98,13 +98,21 @@ namespace dtk
     const int      y = blockIdx.y;
     if (x >= d.W || y >= d.H)
       return;
-    uint32_t n = 0;
+    // Child c of p has c + off[k] == p and pred(c) == k; the eight codes
+    // are loaded first, from clamped addresses.
+    unsigned char pc[8];
+#pragma unroll
     for (int k = 0; k < 8; ++k)
     {
-      const int cx = x - kOffX[k], cy = y - kOffY[k]; // child c with c + off[k] == p
-      if (cx < 0 || cy < 0 || cx >= d.W || cy >= d.H)
-        continue;
-      n += d.pred[(long long)cy * d.pred_pitch + cx] == k;
+      const int cx = min(max(x - kOffX[k], 0), d.W - 1), cy = min(max(y - kOffY[k], 0), d.H - 1);
+      pc[k]        = d.pred[(long long)cy * d.pred_pitch + cx];
+    }
+    uint32_t n = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+    {
+      const int cx = x - kOffX[k], cy = y - kOffY[k];
+      n += (cx >= 0 && cy >= 0 && cx < d.W && cy < d.H && pc[k] == k) ? 1u : 0u;
     }
     j.cw[(long long)y * d.W + x] = (unsigned long long)n << 32; // sum +0.0f
     int code = d.pred[(long long)y * d.pred_pitch + x];
@@ -200,15 +208,24 @@ namespace dtk
       const float iq = img_val<IMG_F32>(d, x + kOffX[code], y + kOffY[code]);
       acc += G[id] * p.l_grad * sgnf(ip - iq);
     }
+    // The eight possible children: codes, G and pixels loaded first from
+    // clamped addresses, then summed in k order for the real children.
+    unsigned char pc[8];
+    float         gc[8], ic[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+    {
+      const int cx = min(max(x - kOffX[k], 0), d.W - 1), cy = min(max(y - kOffY[k], 0), d.H - 1);
+      pc[k]        = d.pred[(long long)cy * d.pred_pitch + cx];
+      gc[k]        = G[(long long)cy * d.W + cx];
+      ic[k]        = img_val<IMG_F32>(d, cx, cy);
+    }
+#pragma unroll
     for (int k = 0; k < 8; ++k)
     {
       const int cx = x - kOffX[k], cy = y - kOffY[k];
-      if (cx < 0 || cy < 0 || cx >= d.W || cy >= d.H)
-        continue;
-      if (d.pred[(long long)cy * d.pred_pitch + cx] != k)
-        continue;
-      const float ic = img_val<IMG_F32>(d, cx, cy);
-      acc -= G[(long long)cy * d.W + cx] * p.l_grad * sgnf(ic - ip);
+      if (cx >= 0 && cy >= 0 && cx < d.W && cy < d.H && pc[k] == k)
+        acc -= gc[k] * p.l_grad * sgnf(ic[k] - ip);
     }
     reinterpret_cast<float*>(gi + (long long)y * gi_pitch)[x] = acc;
     if (gm)
