@@ -176,6 +176,52 @@ def test_invalid_arguments_rejected_on_host(lib, case, msg):
         _lib.check(st)
 
 
+def test_priority_and_batched_backward_reject_bad_arguments_on_host(lib):
+    """dtk_block_priorities and dtk_ggdt_backward_batched check their
+    arguments before the first HIP call."""
+    from distance_transform_benchmarks_amd import _lib
+
+    lv = ctypes.c_uint32(7)
+    rgb = _img(8, 8, 1)
+    assert lib.dtk_block_priorities(ctypes.byref(rgb), None, None, ctypes.byref(lv), None) == _lib.DTK_ERR_INVALID_ARGUMENT
+    assert "unsupported dtype" in lib.dtk_last_error().decode()
+    m = _img(8, 8, 0)
+    assert lib.dtk_block_priorities(ctypes.byref(m), None, None, None, None) == _lib.DTK_ERR_INVALID_ARGUMENT
+    assert "level0" in lib.dtk_last_error().decode()
+    assert lib.dtk_block_priorities(ctypes.byref(m), None, None, ctypes.byref(lv), None) == _lib.DTK_ERR_INVALID_ARGUMENT
+    assert "NULL" in lib.dtk_last_error().decode()
+    empty = _img(0, 5, 0)
+    assert lib.dtk_block_priorities(ctypes.byref(empty), None, None, ctypes.byref(lv), None) == _lib.DTK_OK
+    assert lv.value == 0
+
+    p = _lib.default_params()
+    p.lambda_ = 1.0
+
+    def arr(*ims):
+        return (_lib.DtkImage * len(ims))(*ims)
+
+    f32 = lambda w=8, h=8: _img(w, h, 2)  # noqa: E731
+    u8 = lambda w=8, h=8: _img(w, h, 0)  # noqa: E731
+    assert lib.dtk_ggdt_backward_batched(-1, None, None, None, None, None, None, None, ctypes.byref(p), None) == \
+        _lib.DTK_ERR_INVALID_ARGUMENT
+    assert lib.dtk_ggdt_backward_batched(1, None, None, None, None, None, None, None, ctypes.byref(p), None) == \
+        _lib.DTK_ERR_INVALID_ARGUMENT
+    assert "NULL" in lib.dtk_last_error().decode()
+    assert lib.dtk_ggdt_backward_batched(0, None, None, None, None, None, None, None, ctypes.byref(p), None) == \
+        _lib.DTK_OK
+    cases = [
+        (arr(u8(), u8()), arr(f32(), f32()), "float32 image", _lib.DTK_ERR_UNSUPPORTED),
+        (arr(f32(), f32()), arr(f32(), u8()), "share a dtype", _lib.DTK_ERR_INVALID_ARGUMENT),
+    ]
+    for imgs, masks, msg, code in cases:
+        st = lib.dtk_ggdt_backward_batched(2, imgs, masks, arr(f32(), f32()), arr(u8(), u8()), arr(f32(), f32()),
+                                           arr(f32(), f32()), None, ctypes.byref(p), None)
+        assert st == code and msg in lib.dtk_last_error().decode(), (msg, lib.dtk_last_error())
+    st = lib.dtk_ggdt_backward_batched(2, arr(f32(), f32()), arr(f32(), f32()), arr(f32(), f32()), arr(u8(), u8()),
+                                       arr(f32(), f32(9, 8)), arr(f32(), f32()), None, ctypes.byref(p), None)
+    assert st == _lib.DTK_ERR_INVALID_ARGUMENT and "sizes differ" in lib.dtk_last_error().decode()
+
+
 def test_cpp_suite_cpu_part():
     """The C++ API test runner (mirrors the reference's gtest cases) without --gpu."""
     exe = os.path.join(PKG, "lib", "dt_tests")
