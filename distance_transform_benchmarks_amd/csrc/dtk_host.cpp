@@ -234,13 +234,14 @@ namespace
   // calls keep the masked per-pixel costs.
   bool const_costs(const dtk_params* p) { return p->lambda == 0.f && p->max_sweeps == 0; }
 
-  // Stacked waves per tile: the task schedule stacks four 16-row regions
-  // (62 x 56 owned pixels), four 32-row regions with constant costs
-  // (62 x 120) and two 32-row regions when 32 rows are asked for otherwise
-  // (62 x 60), unless told otherwise; other schedules use one.
+  // Stacked waves per tile, for the task and blocks schedules alike: four
+  // 16-row regions (62 x 56 owned pixels), four 32-row regions with constant
+  // costs (62 x 120) and two 32-row regions when 32 rows are asked for
+  // otherwise (62 x 60), unless told otherwise.  The raster schedule has no
+  // tiles.
   int tile_waves(const dtk_params* p, int rh)
   {
-    if (p->schedule != DTK_SCHED_TASK)
+    if (p->schedule == DTK_SCHED_RASTER)
       return 1;
     if (p->tile_waves != 0)
       return p->tile_waves;
@@ -274,16 +275,17 @@ namespace
     return p->lambda == 0.f || p->lambda == 1.f ? 0 : 1;
   }
 
-  // Region rows per wave: the task schedule defaults to 16-row waves (the
-  // per-pixel cost arrays leave room for two waves per SIMD) and to 32-row
-  // waves with constant costs (105 VGPRs); the blocks schedule to single
-  // 32-row waves (62 x 30).  Measured on 4096^2 RGB8: 16 x 4 is fastest at
-  // lambda 0.5 and 1, 32 x 4 at lambda 0.
+  // Region rows per wave: 16-row waves (the per-pixel cost arrays leave
+  // room for two waves per SIMD), and 32-row waves with constant costs
+  // (105 VGPRs).  Measured on 4096^2 RGB8 (task) and on 8.5-9.6 Mpx u8
+  // images with one seed (task and blocks): 16 x 4 is fastest at lambda 0.5
+  // and 1, 32 x 4 at lambda 0; single-wave blocks tiles (62 x 30) took
+  // 1.5-2.1x as long.
   int region_rows(const dtk_params* p)
   {
     if (p->tile_rows != 0)
       return p->tile_rows;
-    if (p->schedule != DTK_SCHED_TASK)
+    if (p->schedule == DTK_SCHED_RASTER)
       return 32;
     return const_costs(p) ? 32 : kDefaultTaskRows;
   }

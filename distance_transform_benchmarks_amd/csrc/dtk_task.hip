@@ -456,13 +456,16 @@ namespace dtk
   // and activates the neighbours whose halo it changed.  A round that woke a
   // tile stores its number (>= 1) in CTRL_CHANGED, so the host can enqueue
   // several rounds and read back once which was the last to wake anything.
-  template <int S, bool C8, int CM>
-  __global__ void __launch_bounds__(64) k_blocks(Batch b, KParams p, uint8_t* active, uint8_t* next, uint32_t* ctrl,
-                                                 int* rounds_per_tile, uint32_t round)
+  template <int S, bool C8, int CM, int NW>
+  __global__ void __launch_bounds__(64 * NW) k_blocks(Batch b, KParams p, uint8_t* active, uint8_t* next,
+                                                      uint32_t* ctrl, int* rounds_per_tile, uint32_t round)
   {
-    __shared__ float T[(64 / S) * 65];
-    __shared__ float Sv[2 * (64 / S)];
-    const int        t = blockIdx.x;
+    __shared__ float    T[NW][(64 / S) * 65];
+    __shared__ float    Sv[NW][2 * (64 / S)];
+    __shared__ float    X[NW > 1 ? 2 * NW * 4 * 64 : 1];
+    __shared__ uint32_t F[3 * NW];
+    const int           w = NW > 1 ? (int)(threadIdx.x >> 6) : 0;
+    const int           t = blockIdx.x;
     if (!active[t])
       return;
     __syncthreads();
@@ -475,7 +478,7 @@ namespace dtk
     ImgDesc d;
     int     tx, ty;
     resolve_tile(b, t, d, tx, ty);
-    const VisitResult vr = visit_tile<S, C8, CM, 1>(d, p, tx, ty, T, Sv, nullptr, nullptr);
+    const VisitResult vr = visit_tile<S, C8, CM, NW>(d, p, tx, ty, T[w], Sv[w], X, F);
     if (threadIdx.x == 0)
     {
       __hip_atomic_fetch_add(&ctrl[CTRL_VISITS], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -539,7 +542,15 @@ namespace dtk
   static hipError_t launch_blocks_t(const Batch& b, const KParams& p, uint8_t* active, uint8_t* next, uint32_t* ctrl,
                                     int* rpt, uint32_t round, hipStream_t s)
   {
-    hipLaunchKernelGGL((k_blocks<S, C8, CM>), dim3(b.ntiles), dim3(64), 0, s, b, p, active, next, ctrl, rpt, round);
+    if (p.nw == 4)
+      hipLaunchKernelGGL((k_blocks<S, C8, CM, 4>), dim3(b.ntiles), dim3(256), 0, s, b, p, active, next, ctrl, rpt,
+                         round);
+    else if (p.nw == 2)
+      hipLaunchKernelGGL((k_blocks<S, C8, CM, 2>), dim3(b.ntiles), dim3(128), 0, s, b, p, active, next, ctrl, rpt,
+                         round);
+    else
+      hipLaunchKernelGGL((k_blocks<S, C8, CM, 1>), dim3(b.ntiles), dim3(64), 0, s, b, p, active, next, ctrl, rpt,
+                         round);
     return hipGetLastError();
   }
 

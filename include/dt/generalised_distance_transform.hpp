@@ -29,9 +29,11 @@ namespace dt
   image2d<float> generalised_distance_transform(const image2d_view<std::uint8_t>& img,
                                                 const image2d_view<std::uint8_t>& mask, float lambda, float v = 1e10);
 
-  // [reference] blocks schedule.  round_per_blocks (device int array of
-  // dtk_tile_count(W, H, 0) entries, or nullptr) counts visits per tile of
-  // this library's tiling (62 x 30 owned pixels by default).
+  // [reference] blocks schedule.  round_per_blocks (device int array, or
+  // nullptr) counts visits per tile of this library's tiling, row-major:
+  // 62 x 56 owned pixels by default (62 x 120 at lambda 0), so
+  // dtk_tile_count_for() entries; dtk_tile_count(W, H, 0) entries (62 x 30
+  // tiles) are enough for any default tiling.
   void generalised_distance_transform_blocks(const image2d_view<std::uint8_t>& img,
                                              const image2d_view<std::uint8_t>& mask, image2d_view<float>& dist,
                                              float lambda, float v = 1e10, int* round_per_blocks = nullptr);

@@ -193,10 +193,12 @@ dtk_status dtk_ggdt_backward_batched(int n, const dtk_image* imgs, const dtk_ima
                                      const dtk_image* preds, const dtk_image* grad_dists, dtk_image* grad_imgs,
                                      dtk_image* grad_masks, const dtk_params* p, void* stream);
 
-/* Tiles of the blocks schedule on a width x height image with single-wave
- * regions of tile_rows rows (0 = 32): the size of round_per_blocks for
- * dt::generalised_distance_transform_blocks
- * (dt/src/generalised_distance_transform_blocks.cu:18-19). */
+/* Tiles on a width x height image with single-wave regions of tile_rows
+ * rows (0 = 32), i.e. 62 x (tile_rows - 2) owned pixels: enough entries for
+ * round_per_blocks of dt::generalised_distance_transform_blocks
+ * (dt/src/generalised_distance_transform_blocks.cu:18-19) under any default
+ * tiling, whose tiles are taller.  dtk_tile_count_for gives a call's exact
+ * count. */
 int dtk_tile_count(int width, int height, int tile_rows);
 
 /* Tiles a call with parameters *p uses on a width x height image, for any

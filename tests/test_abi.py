@@ -112,7 +112,8 @@ def test_tile_count_closed_form(lib, W, H, rows):
 
 @pytest.mark.parametrize("sched,lam,rows,waves,owned_rows", [
     (2, 0.5, 0, 0, 56), (2, 0.0, 0, 0, 120), (2, 0.5, 32, 0, 60), (2, 0.0, 32, 0, 120), (2, 0.0, 16, 0, 56),
-    (2, 0.5, 16, 2, 28), (2, 0.0, 32, 1, 30), (1, 0.0, 0, 0, 30), (1, 0.5, 0, 0, 30), (0, 0.5, 16, 0, 14)])
+    (2, 0.5, 16, 2, 28), (2, 0.0, 32, 1, 30), (1, 0.0, 0, 0, 120), (1, 0.5, 0, 0, 56), (1, 0.5, 32, 1, 30),
+    (1, 0.5, 16, 2, 28), (0, 0.5, 16, 0, 14)])
 def test_tile_count_for_schedules(lib, sched, lam, rows, waves, owned_rows):
     """Default tilings: task 16-row waves stacked four high (lambda > 0) or
     32-row waves stacked four high (lambda 0); blocks single 32-row waves."""

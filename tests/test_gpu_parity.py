@@ -49,9 +49,10 @@ def _run(dt, dev, img, m, lam, v, sched, **kw):
 
 def _variants(sched, tile_rows=None):
     """Tile shapes a schedule can run: one wave of 16 or 32 rows per tile,
-    and for the task schedule 2 or 4 stacked waves of either height."""
+    and for the task and blocks schedules 2 or 4 stacked waves of either
+    height (the raster schedule has no tiles)."""
     rows = (16, 32) if tile_rows is None else (tile_rows,)
-    if sched != "task":
+    if sched == "raster":
         return [dict(tile_rows=r) for r in rows]
     return [dict(tile_rows=r, tile_waves=w) for r in rows for w in (1, 2, 4)]
 
