@@ -133,7 +133,10 @@ def tile_count(width: int, height: int, tile_rows: int = 0) -> int:
 
 
 def round_per_blocks_buffer(width: int, height: int, device="cuda") -> torch.Tensor:
-    """Zeroed int32 visit counters for the blocks schedule's instrumentation."""
+    """Zeroed int32 visit counters for the blocks schedule's instrumentation:
+    dtk_tile_count(W, H, 0) entries, enough for any default tiling (the
+    first tiles_y * tiles_x of them are used; ``visits_buffer`` gives the
+    exact (tiles_y, tiles_x) grid of a call)."""
     return torch.zeros(tile_count(width, height, 0), dtype=torch.int32, device=device)
 
 
