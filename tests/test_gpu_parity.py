@@ -191,6 +191,21 @@ def test_fixed_iterations_kinds(dt, dev, oracle, case, iters):
     assert_bit_exact(got, want, f"{case} iters={iters} (oracle ran {it})")
 
 
+@pytest.mark.parametrize("case", [("u8", 700, 300, 8, 0.5), ("u8", 300, 700, 8, 0.5), ("f32", 517, 261, 4, 1.0),
+                                  ("rgb8", 333, 444, 8, 0.0)], ids=lambda c: "-".join(map(str, c)))
+def test_fixed_iterations_many_chunks(dt, dev, oracle, case):
+    """Two raster iterations on images whose passes span many 16-step chunks
+    and many strips (the mailbox handoff, whole and partial chunks, the row
+    vectors), in both orientations: bit-exact against the CPU schedule."""
+    ik, W, H, conn, lam = case
+    seed = zlib.crc32(repr(case).encode()) & 0xFFFF
+    img, m = image(ik, W, H, seed, smooth=True), mask("u8", W, H, seed, 3)
+    want, it = oracle.raster(img, m, lam, 1e10, conn8=conn == 8, max_iters=2)
+    _say("fixed many chunks", case)
+    got = _run(dt, dev, img, m, lam, 1e10, "raster", connectivity=conn, max_iterations=2)
+    assert_bit_exact(got, want, f"{case} (oracle ran {it})")
+
+
 @pytest.mark.parametrize("shape", [(1, 1), (1, 9), (9, 1), (62, 14), (63, 15), (64, 16), (2, 70), (70, 2),
                                    (62, 60), (63, 61), (64, 62), (70, 121), (125, 31), (130, 241), (1100, 70),
                                    (70, 1100)])
