@@ -344,6 +344,31 @@ def test_batched_uneven(dt, dev, oracle):
         assert_bit_exact(o.cpu().numpy(), w, f"image {i}")
 
 
+@pytest.mark.parametrize("case", [("rgb8", "f32", 8, 0.5), ("f32", "u8", 8, 0.0), ("u8", "u8", 4, 1.0)],
+                         ids=lambda c: "-".join(map(str, c)))
+def test_batched_kinds(dt, dev, oracle, case):
+    """Batched launches over every image kind and cost mode (RGB8 with soft
+    masks, float images at lambda 0's constant costs, C4 with one add)."""
+    import torch
+
+    ik, mk, conn, lam = case
+    rng = np.random.default_rng(zlib.crc32(repr(case).encode()) & 0xFFFF)
+    v = 1e3 if mk == "f32" else 1e10
+    imgs, masks, wants = [], [], []
+    for i in range(7):
+        W, H = int(rng.integers(1, 260)), int(rng.integers(1, 260))
+        img, m = image(ik, W, H, 400 + i), mask(mk, W, H, 400 + i, 2)
+        imgs.append(img)
+        masks.append(m)
+        wants.append(oracle.raster(img, m, lam, v, conn8=conn == 8)[0])
+    _say("batched kinds", case)
+    outs = dt.ggdt_batched([_to_dev(a, dev) for a in imgs], [_to_dev(a, dev) for a in masks], lam, v,
+                           connectivity=conn)
+    torch.cuda.synchronize()
+    for i, (o, w) in enumerate(zip(outs, wants)):
+        assert_bit_exact(o.cpu().numpy(), w, f"{case} image {i}")
+
+
 def test_batch_plan_graph_replays(dt, dev, oracle):
     """hipGraph plan: several replays, with eager work in between, each bit-exact."""
     import torch
