@@ -156,3 +156,33 @@ def test_backward_batched_equals_per_image(dt, dev, mk):
             assert np.abs(a - b).max() <= 1e-5 * max(1.0, np.abs(b).max()), (i, np.abs(a - b).max())
         if mk == "u8":
             assert not gms[i].any()
+
+
+@pytest.mark.parametrize("lam", [1.0, 0.5])
+def test_adjoint_c4_vs_cpu_reference(dt, dev, oracle, lam):
+    """C4 predecessors equal the oracle's, and the C4 adjoint matches the CPU
+    reference gradient to 1e-4 of the largest one (the C8 cases run above)."""
+    import torch
+
+    rng = np.random.default_rng(77)
+    H, W = 96, 130
+    yy, xx = np.mgrid[0:H, 0:W]
+    I = (np.sin(xx / 9.0) * np.cos(yy / 7.0)).astype(np.float32) + 1e-3 * rng.normal(size=(H, W)).astype(np.float32)
+    M = mask("f32", W, H, 91, 3)
+    Wt = rng.random((H, W)).astype(np.float32)
+    di, dm = _t(I, dev), _t(M, dev)
+    _say("adjoint C4", lam)
+    D = dt.ggdt(di, dm, lam, 1e3, connectivity=4)
+    P = dt.predecessors(di, dm, D, lam, 1e3, connectivity=4)
+    gi, gm = dt.backward(di, dm, D, P, _t(Wt, dev), lam, 1e3, connectivity=4)
+    torch.cuda.synchronize()
+    Dn, Pn = D.cpu().numpy(), P.cpu().numpy()
+    Dr, _ = oracle.raster(I, M, lam, 1e3, conn8=False)
+    assert_bit_exact(Dn, Dr, "C4 forward")
+    Pr = oracle.pred(I, M, Dr, lam, 1e3, conn8=False)
+    assert np.array_equal(Pn, Pr)
+    assert set(np.unique(Pn)) <= {0, 2, 4, 6, 8}  # axis neighbours and roots only
+    gir, gmr, _ = oracle.backward(I, Dr, Pr, Wt, lam, 1e3)
+    for got, want, what in ((gi, gir, "dL/dimage"), (gm, gmr, "dL/dmask")):
+        err = np.abs(got.cpu().numpy().astype(np.float64) - want).max()
+        assert err <= 1e-4 * max(1.0, np.abs(want).max()), (what, err)
