@@ -383,7 +383,7 @@ namespace dtk
     __shared__ float    T[NW][(64 / S) * 65];
     __shared__ float    Sv[NW][2 * (64 / S)];
     __shared__ float    X[NW > 1 ? 2 * NW * 4 * 64 : 1];
-    __shared__ uint32_t F[3 * NW];
+    __shared__ uint32_t F[4 * NW];
     __shared__ int      s_tile;
     const int           w = NW > 1 ? (int)(threadIdx.x >> 6) : 0;
     uint32_t         visits = 0, sweeps = 0;
@@ -463,7 +463,7 @@ namespace dtk
     __shared__ float    T[NW][(64 / S) * 65];
     __shared__ float    Sv[NW][2 * (64 / S)];
     __shared__ float    X[NW > 1 ? 2 * NW * 4 * 64 : 1];
-    __shared__ uint32_t F[3 * NW];
+    __shared__ uint32_t F[4 * NW];
     const int           w = NW > 1 ? (int)(threadIdx.x >> 6) : 0;
     const int           t = blockIdx.x;
     if (!active[t])
