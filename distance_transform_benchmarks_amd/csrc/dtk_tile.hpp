@@ -84,11 +84,15 @@ namespace dtk
   // the DPP is never folded into an integer VOP2 instruction: measured on
   // gfx950, a folded v_subrev_u32_dpp with wave_shl:1 produced wrong values,
   // while a plain v_mov_b32_dpp followed by the subtract is correct
-  // (dtk_selftest checks both the shifts and this subtract pattern).
+  // (dtk_selftest checks both the shifts and this subtract pattern).  The asm
+  // is not volatile: it has no side effect, so the compiler may share one
+  // copy between identical shifts (a row's W edge and the next row's NW edge
+  // shift the same pixels), which saves a tenth of the tile kernels' VALU
+  // instructions.
   __device__ __forceinline__ uint32_t opaque_u(uint32_t v)
   {
     uint32_t r;
-    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(v));
+    asm("v_mov_b32 %0, %1" : "=v"(r) : "v"(v));
     return r;
   }
   __device__ __forceinline__ uint32_t shr_u(uint32_t v)
