@@ -77,7 +77,9 @@ namespace dtk
         const int  nx = x + kOffX[k], ny = y + kOffY[k];
         const bool in = (!diag || c8) && nx >= 0 && ny >= 0 && nx < d.W && ny < d.H;
         const long long qid = (long long)ny * d.W + nx;
-        float           g   = p.l_grad * pix_dist<KIND>(ip, iq[k]);
+        // lambda = 0 has no image term (the forward kernels do not read the
+        // image), so a NaN or infinite float pixel changes nothing there.
+        float           g   = p.l_grad == 0.f ? 0.f : p.l_grad * pix_dist<KIND>(ip, iq[k]);
         g                   = (g == g) ? g : DTK_INF;
         float cur           = Dq[k] + (diag ? p.e_diag : p.e_axis);
         cur                 = cur + g;

@@ -39,7 +39,11 @@
  * Extensions with no reference counterpart (definitions of this project):
  *   - C4 connectivity: only the axis neighbour (dy = 0 / dx = 0) is relaxed.
  *   - RGB8 cost: sqrtf((float)(dr*dr + dg*dg + db*db)).
- *   - float32 image: fabsf(I(p) - I(q)).
+ *   - float32 image: fabsf(I(p) - I(q)); an edge whose image term is NaN
+ *     (a NaN pixel, or two infinite pixels of the same sign) never wins.
+ *   - lambda = 0: no image term at all; every edge costs its chamfer step,
+ *     whatever the pixel values (for uint8 and RGB8 pixels 0 * |dI| is +0
+ *     anyway; for a NaN or infinite float pixel it would be NaN).
  *   - soft float mask s in [0,1]: D0 = nu * (1 - s).
  *   - fixed pass count: exactly N iterations of the four passes.
  *
@@ -95,7 +99,7 @@ static inline float local_dist(const cost_ctx* c, long p, long q)
 /* cur = (D(q) + l_eucl*ld) + l_grad*dist, two rounded adds. */
 static inline float relax_cost(const cost_ctx* c, float dq, float e, long p, long q)
 {
-  const float g   = c->l_grad * local_dist(c, p, q);
+  const float g   = c->l_grad == 0.f ? 0.f : c->l_grad * local_dist(c, p, q);
   float       cur = dq + e;
   cur             = cur + g;
   return cur;
@@ -496,7 +500,7 @@ int ora_raster_f64(const double* img, const double* smask, double* D, int W, int
             if (qx < 0 || qy < 0 || qx >= W || qy >= H)
               continue;
             const long   q   = (long)qy * W + qx;
-            const double cur = D[q] + (d == 0 ? e_axis : e_diag) + lambda * fabs(img[p] - img[q]);
+            const double cur = D[q] + (d == 0 ? e_axis : e_diag) + (lambda == 0.0 ? 0.0 : lambda * fabs(img[p] - img[q]));
             if (cur < nd)
               nd = cur;
           }

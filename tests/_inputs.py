@@ -96,3 +96,16 @@ def assert_rel(got: np.ndarray, want: np.ndarray, rtol: float, what: str = ""):
         i = int(rel.argmax())
         raise AssertionError(f"{what}: max relative error {rel.max():.3e} > {rtol} at {i} "
                              f"(got {got.ravel()[i]!r}, want {want.ravel()[i]!r}; {int((rel > rtol).sum())} bad)")
+
+
+def nonfinite_image(W, H, seed):
+    """float32 image with NaN, +inf and -inf pixels scattered over a smooth field."""
+    rng = np.random.default_rng(seed)
+    img = image("f32", W, H, seed)
+    flat = img.reshape(-1)
+    idx = rng.choice(flat.size, size=3 * max(1, flat.size // 40), replace=False)
+    k = len(idx) // 3
+    flat[idx[:k]] = np.nan
+    flat[idx[k:2 * k]] = np.inf
+    flat[idx[2 * k:]] = -np.inf
+    return img

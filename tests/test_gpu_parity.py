@@ -14,7 +14,7 @@ import sys
 import numpy as np
 import pytest
 
-from _inputs import assert_bit_exact, image, mask
+from _inputs import assert_bit_exact, image, mask, nonfinite_image
 
 pytestmark = pytest.mark.gpu
 
@@ -118,6 +118,21 @@ def test_random_vs_oracle(dt, dev, oracle, case, sched):
         _say(case, sched, kw)
         got = _run(dt, dev, img, m, lam, v, sched, connectivity=conn, **kw)
         assert_bit_exact(got, want, f"{case} {sched} {kw}")
+
+
+@pytest.mark.parametrize("conn", [8, 4])
+@pytest.mark.parametrize("lam", [0.0, 0.5, 1.0])
+@pytest.mark.parametrize("sched", ["raster", "blocks", "task"])
+def test_nonfinite_float_images(dt, dev, oracle, sched, lam, conn):
+    """float32 images with NaN and +-inf pixels: an edge with a NaN image term
+    never wins, and at lambda = 0 the image is not used at all."""
+    img = nonfinite_image(150, 110, 11)
+    m = mask("f32", 150, 110, 11)
+    want, _ = oracle.raster(img, m, lam, 1e3, conn8=conn == 8)
+    for kw in _variants(sched):
+        _say("non-finite image", sched, lam, conn, kw)
+        got = _run(dt, dev, img, m, lam, 1e3, sched, connectivity=conn, **kw)
+        assert_bit_exact(got, want, f"non-finite {sched} {lam} C{conn} {kw}")
 
 
 @pytest.mark.parametrize("case", [c for c in CASES if c[5] != 0.0], ids=lambda c: "-".join(map(str, c)))
@@ -437,6 +452,24 @@ def test_predecessors(dt, dev, oracle, conn):
     di, dm = _to_dev(img, dev), _to_dev(m, dev)
     Dg = dt.ggdt(di, dm, 1.0, 1e3, connectivity=conn)
     P = dt.predecessors(di, dm, Dg, 1.0, 1e3, connectivity=conn)
+    torch.cuda.synchronize()
+    assert_bit_exact(Dg.cpu().numpy(), D)
+    assert np.array_equal(P.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize("lam", [0.0, 1.0])
+def test_predecessors_nonfinite_image(dt, dev, oracle, lam):
+    """Predecessor codes on a float image with NaN and +-inf pixels equal the
+    oracle's: at lambda = 0 the image is ignored, at lambda 1 NaN edges never match."""
+    import torch
+
+    img, m = nonfinite_image(150, 110, 23), mask("f32", 150, 110, 23, 3)
+    D, _ = oracle.raster(img, m, lam, 1e3)
+    want = oracle.pred(img, m, D, lam, 1e3)
+    _say("pred non-finite", lam)
+    di, dm = _to_dev(img, dev), _to_dev(m, dev)
+    Dg = dt.ggdt(di, dm, lam, 1e3)
+    P = dt.predecessors(di, dm, Dg, lam, 1e3)
     torch.cuda.synchronize()
     assert_bit_exact(Dg.cpu().numpy(), D)
     assert np.array_equal(P.cpu().numpy(), want)

@@ -26,7 +26,7 @@ import zlib
 import numpy as np
 import pytest
 
-from _inputs import assert_bit_exact, assert_rel, diff_crops, fd_indices, image, mask
+from _inputs import assert_bit_exact, assert_rel, diff_crops, fd_indices, image, mask, nonfinite_image
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -255,3 +255,30 @@ def test_synth_reproducible():
         x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
         out.append(x ^ (x >> 31))
     assert [int(v) for v in z] == out
+
+
+@pytest.mark.parametrize("conn8", [True, False])
+def test_lambda0_ignores_nonfinite_pixels(oracle, conn8):
+    """At lambda = 0 the image plays no part: a float image with NaN and +-inf
+    pixels gives the same map as an all-zero image, by raster and Dijkstra."""
+    img = nonfinite_image(57, 43, 5)
+    m = mask("f32", 57, 43, 5)
+    want, _ = oracle.raster(np.zeros_like(img), m, 0.0, 1e3, conn8=conn8)
+    got, _ = oracle.raster(img, m, 0.0, 1e3, conn8=conn8)
+    assert_bit_exact(got, want, "lambda 0, non-finite pixels")
+    assert_bit_exact(oracle.dijkstra(img, m, 0.0, 1e3, conn8=conn8), want, "dijkstra")
+
+
+@pytest.mark.parametrize("lam", [0.5, 1.0])
+def test_nan_pixel_is_never_entered(oracle, lam):
+    """At lambda > 0 every edge of a NaN pixel has a NaN cost and never wins:
+    the pixel keeps its initial value, and raster and Dijkstra agree."""
+    H, W = 21, 25
+    img = np.full((H, W), 7.0, np.float32)
+    img[10, 12] = np.nan
+    m = np.zeros((H, W), np.uint8)
+    m[2, 3] = 1
+    D, _ = oracle.raster(img, m, lam, 1e10)
+    assert D[10, 12] == np.float32(1e10)
+    assert np.all(D[np.isfinite(img)] < 1e3)  # everything else is reached around it
+    assert_bit_exact(oracle.dijkstra(img, m, lam, 1e10), D, "dijkstra")
