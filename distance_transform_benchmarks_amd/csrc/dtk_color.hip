@@ -16,7 +16,8 @@
 //   k_inferno                  rgb8 = table[u8]
 //   k_colorize                 both in one pass (float in, rgb8 out), four
 //                              pixels per thread, three dword stores when
-//                              the rows allow it
+//                              the rows allow it, colours from a packed
+//                              copy of the table in LDS
 //
 // The other kernels run one thread per pixel over 64 x 4 blocks, so a wave
 // reads 64 consecutive pixels of one row.
@@ -184,6 +185,15 @@ namespace dtk
   __global__ void __launch_bounds__(256) k_colorize(const void* in, long long in_pitch, void* out, long long out_pitch,
                                                     int W, int H, const uint32_t* keys)
   {
+    // The colour table packed one colour per word (r | g << 8 | b << 16) in
+    // LDS: one LDS read per pixel instead of three byte loads.
+    __shared__ uint32_t s_lut[256];
+    {
+      const int t = (int)(threadIdx.y * 64 + threadIdx.x);
+      s_lut[t]    = (uint32_t)c_inferno[3 * t] | ((uint32_t)c_inferno[3 * t + 1] << 8) |
+                 ((uint32_t)c_inferno[3 * t + 2] << 16);
+    }
+    __syncthreads();
     const int x0 = (blockIdx.x * 64 + threadIdx.x) * 4;
     const int y  = blockIdx.y * 4 + threadIdx.y;
     if (x0 >= W || y >= H)
@@ -195,29 +205,22 @@ namespace dtk
     {
       typedef float f4v __attribute__((ext_vector_type(4)));
       const f4v v = *reinterpret_cast<const f4v*>(r + x0);
-      uint8_t   b[12];
+      uint32_t  c[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k)
-      {
-        const int i  = norm_u8(v[k], mn, mx);
-        b[3 * k]     = c_inferno[3 * i];
-        b[3 * k + 1] = c_inferno[3 * i + 1];
-        b[3 * k + 2] = c_inferno[3 * i + 2];
-      }
-      uint32_t w[3];
-      __builtin_memcpy(w, b, 12);
+        c[k] = s_lut[norm_u8(v[k], mn, mx)];
       uint32_t* o32 = reinterpret_cast<uint32_t*>(o);
-      o32[0]        = w[0];
-      o32[1]        = w[1];
-      o32[2]        = w[2];
+      o32[0]        = c[0] | (c[1] << 24);
+      o32[1]        = (c[1] >> 8) | (c[2] << 16);
+      o32[2]        = (c[2] >> 16) | (c[3] << 8);
       return;
     }
     for (int k = 0; k < 4 && x0 + k < W; ++k)
     {
-      const int i  = norm_u8(r[x0 + k], mn, mx);
-      o[3 * k]     = c_inferno[3 * i];
-      o[3 * k + 1] = c_inferno[3 * i + 1];
-      o[3 * k + 2] = c_inferno[3 * i + 2];
+      const uint32_t c = s_lut[norm_u8(r[x0 + k], mn, mx)];
+      o[3 * k]         = (uint8_t)c;
+      o[3 * k + 1]     = (uint8_t)(c >> 8);
+      o[3 * k + 2]     = (uint8_t)(c >> 16);
     }
   }
 
