@@ -124,16 +124,32 @@ namespace dtk
   }
 
   // Leaf-started climbs computing G.
-  __global__ void k_subtree(const BackJob* jobs)
+  // gx > 0: a one-dimensional grid in which image z runs on XCD z % 8
+  // (workgroups go to the XCDs in turn), gx x gy blocks per image.  The
+  // climbs of one image converge on shared trunks, so keeping an image on
+  // one XCD keeps those lines in one L2 instead of fetching them into all
+  // eight (diff32: 6.4 -> 5.1 ms per backward).
+  __global__ void k_subtree(const BackJob* jobs, int gx, int gy, int n)
   {
-    const BackJob&       j        = jobs[blockIdx.z];
+    int bx = (int)blockIdx.x, by = (int)blockIdx.y, bz = (int)blockIdx.z;
+    if (gx > 0)
+    {
+      const unsigned b = blockIdx.x, xcd = b & 7u, local = b >> 3;
+      const unsigned per = (unsigned)gx * (unsigned)gy, slot = local / per, rem = local - slot * per;
+      bz = (int)(xcd + 8u * slot);
+      if (bz >= n)
+        return;
+      bx = (int)(rem % (unsigned)gx);
+      by = (int)(rem / (unsigned)gx);
+    }
+    const BackJob&       j        = jobs[bz];
     const ImgDesc&       d        = j.d;
     const unsigned char* gd       = j.gd;
     const long long      gd_pitch = j.gd_pitch;
     unsigned long long*  cw       = j.cw;
     float*               G        = j.G;
-    int                  x        = blockIdx.x * blockDim.x + threadIdx.x;
-    int                  y        = blockIdx.y;
+    int                  x        = bx * blockDim.x + threadIdx.x;
+    int                  y        = by;
     if (x >= d.W || y >= d.H)
       return;
     long long            id   = (long long)y * d.W + x;
@@ -262,7 +278,13 @@ namespace dtk
       return hipSuccess;
     const dim3 grid((maxW + 255) / 256, maxH, n), blk(256);
     hipLaunchKernelGGL(k_child_count, grid, blk, 0, s, jobs);
-    hipLaunchKernelGGL(k_subtree, grid, blk, 0, s, jobs);
+    if (n >= 8) // fewer images would leave XCDs idle
+    {
+      const unsigned gx = grid.x, gy = grid.y, slots = (unsigned)(n + 7) / 8u;
+      hipLaunchKernelGGL(k_subtree, dim3(8u * slots * gx * gy), blk, 0, s, jobs, (int)gx, (int)gy, n);
+    }
+    else
+      hipLaunchKernelGGL(k_subtree, grid, blk, 0, s, jobs, 0, 0, n);
     hipLaunchKernelGGL(k_grad, grid, blk, 0, s, jobs, p);
     return hipGetLastError();
   }

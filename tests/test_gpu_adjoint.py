@@ -123,8 +123,16 @@ def test_autograd_equals_direct_adjoint(dt, dev):
     torch.cuda.synchronize()
 
 
+SHAPES_4 = ((80, 64), (50, 120), (33, 33), (1, 17))
+# Nine images: from eight on, the subtree kernel runs each image's blocks on
+# one XCD (a one-dimensional grid), and the ninth leaves slots of its round
+# of eight empty.
+SHAPES_9 = SHAPES_4 + ((64, 64), (17, 90), (120, 40), (2, 2), (70, 71))
+
+
+@pytest.mark.parametrize("shapes", [SHAPES_4, SHAPES_9], ids=["4", "9"])
 @pytest.mark.parametrize("mk", ["f32", "u8"])
-def test_backward_batched_equals_per_image(dt, dev, mk):
+def test_backward_batched_equals_per_image(dt, dev, mk, shapes):
     """dtk_ggdt_backward_batched (one launch per adjoint kernel for a batch of
     uneven images) gives the per-image dtk_ggdt_backward results.  The
     subtree sums add children in arrival order (float atomics), so the two
@@ -133,7 +141,6 @@ def test_backward_batched_equals_per_image(dt, dev, mk):
     import torch
 
     rng = np.random.default_rng(41)
-    shapes = ((80, 64), (50, 120), (33, 33), (1, 17))
     imgs, masks, Ds, Ps, Ws = [], [], [], [], []
     for i, (H, W) in enumerate(shapes):
         I = rng.random((H, W)).astype(np.float32) + 1e-3 * rng.normal(size=(H, W)).astype(np.float32)
