@@ -14,6 +14,7 @@ from .api import (  # noqa: F401
     autotune,
     backward,
     backward_batched,
+    predecessors_batched,
     block_priorities,
     colorize,
     generalised_distance_transform,

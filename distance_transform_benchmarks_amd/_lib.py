@@ -26,7 +26,7 @@ SCHED_RASTER, SCHED_BLOCKS, SCHED_TASK = 0, 1, 2
 EXPORTS = (
     "dtk_params_default", "dtk_last_error", "dtk_version", "dtk_ggdt", "dtk_ggdt_batched",
     "dtk_batch_plan_create", "dtk_batch_plan_launch", "dtk_batch_plan_destroy", "dtk_ggdt_pred",
-    "dtk_ggdt_backward", "dtk_ggdt_backward_batched", "dtk_tile_count", "dtk_malloc_pitch", "dtk_free", "dtk_host_to_device",
+    "dtk_ggdt_pred_batched", "dtk_ggdt_backward", "dtk_ggdt_backward_batched", "dtk_tile_count", "dtk_malloc_pitch", "dtk_free", "dtk_host_to_device",
     "dtk_device_to_host", "dtk_copy2d", "dtk_selftest", "dtk_device_cu_count", "dtk_profile_begin",
     "dtk_profile_end", "dtk_batch_autotune", "dtk_tile_count_for", "dtk_minmax_f32", "dtk_normalize_u8",
     "dtk_inferno", "dtk_colorize", "dtk_block_priorities",
@@ -98,6 +98,8 @@ def load() -> ctypes.CDLL:
     lib.dtk_batch_autotune.argtypes = [ctypes.c_int, P(DtkImage), P(DtkImage), P(DtkImage), P(DtkParams), vp,
                                        ctypes.c_int, P(DtkParams), P(DtkTuneReport)]
     lib.dtk_ggdt_pred.argtypes = [P(DtkImage), P(DtkImage), P(DtkImage), P(DtkImage), P(DtkParams), vp]
+    lib.dtk_ggdt_pred_batched.argtypes = [ctypes.c_int, P(DtkImage), P(DtkImage), P(DtkImage), P(DtkImage),
+                                          P(DtkParams), vp]
     lib.dtk_ggdt_backward.argtypes = [P(DtkImage), P(DtkImage), P(DtkImage), P(DtkImage), P(DtkImage),
                                       P(DtkImage), P(DtkImage), P(DtkParams), vp]
     lib.dtk_ggdt_backward_batched.argtypes = [ctypes.c_int, P(DtkImage), P(DtkImage), P(DtkImage), P(DtkImage),
@@ -122,7 +124,7 @@ def load() -> ctypes.CDLL:
     lib.dtk_colorize.argtypes = [P(DtkImage), P(DtkImage), vp]
     lib.dtk_block_priorities.argtypes = [P(DtkImage), vp, vp, P(ctypes.c_uint32), vp]
     for name in ("dtk_ggdt", "dtk_ggdt_batched", "dtk_batch_plan_create", "dtk_batch_plan_launch",
-                 "dtk_batch_plan_destroy", "dtk_batch_autotune", "dtk_ggdt_pred", "dtk_ggdt_backward", "dtk_ggdt_backward_batched", "dtk_malloc_pitch", "dtk_free",
+                 "dtk_batch_plan_destroy", "dtk_batch_autotune", "dtk_ggdt_pred", "dtk_ggdt_pred_batched", "dtk_ggdt_backward", "dtk_ggdt_backward_batched", "dtk_malloc_pitch", "dtk_free",
                  "dtk_host_to_device", "dtk_device_to_host", "dtk_copy2d", "dtk_selftest", "dtk_profile_begin",
                  "dtk_profile_end"):
         getattr(lib, name).restype = ctypes.c_int

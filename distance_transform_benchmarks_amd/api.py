@@ -270,6 +270,24 @@ def predecessors(img, mask, dist, lambda_, v=1e10, *, connectivity=8, out=None) 
     return out
 
 
+def predecessors_batched(imgs, masks, dists, lambda_, v=1e10, *, connectivity=8, outs=None):
+    """Predecessor codes of a batch of converged maps in one launch (images of
+    one dtype, masks of one dtype).  Returns the list of uint8 code maps."""
+    n = len(imgs)
+    if not (len(masks) == len(dists) == n) or (outs is not None and len(outs) != n):
+        raise ValueError("predecessors_batched: sequences of different lengths")
+    if outs is None:
+        outs = [torch.empty(d.shape, dtype=torch.uint8, device=d.device) for d in dists]
+    p = _params(lambda_, v, connectivity, "task", 0, 0, 0, 0)
+    arr = lambda ts, name: (DtkImage * max(1, n))(*[_as_image(t, name) for t in ts])  # noqa: E731
+    if n:
+        with torch.cuda.device(imgs[0].device):
+            check(_lib.load().dtk_ggdt_pred_batched(n, arr(imgs, "image"), arr(masks, "mask"), arr(dists, "dist"),
+                                                    arr(outs, "pred"), ctypes.byref(p),
+                                                    _stream_handle(imgs[0].device)))
+    return outs
+
+
 def backward(img, mask, dist, pred, grad_dist, lambda_, v=1e10, *, connectivity=8):
     """Adjoint of sum(grad_dist * D): returns (dL/dimg, dL/dmask)."""
     gi = torch.empty_like(img, dtype=torch.float32)

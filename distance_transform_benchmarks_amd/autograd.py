@@ -31,8 +31,11 @@ class GGDTFunction(torch.autograd.Function):
             api.ggdt_batched(list(imgs), list(masks), lambda_, nu, connectivity=connectivity, outs=list(D))
         elif imgs.shape[0] == 1:
             api.ggdt(imgs[0], masks[0], lambda_, nu, connectivity=connectivity, out=D[0])
-        for b in range(imgs.shape[0]):
-            api.predecessors(imgs[b], masks[b], D[b], lambda_, nu, connectivity=connectivity, out=P[b])
+        if imgs.shape[0] > 1:
+            api.predecessors_batched(list(imgs), list(masks), list(D), lambda_, nu, connectivity=connectivity,
+                                     outs=list(P))
+        elif imgs.shape[0] == 1:
+            api.predecessors(imgs[0], masks[0], D[0], lambda_, nu, connectivity=connectivity, out=P[0])
         ctx.save_for_backward(imgs, masks, D, P)
         ctx.lambda_, ctx.nu, ctx.connectivity, ctx.squeeze = lambda_, nu, connectivity, squeeze
         return D[0] if squeeze else D

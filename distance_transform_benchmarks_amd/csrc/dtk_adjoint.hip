@@ -40,10 +40,8 @@ namespace dtk
   }
 
   template <int KIND>
-  __global__ void k_pred(ImgDesc d, KParams p, int c8)
+  __device__ __forceinline__ void pred_pixel(const ImgDesc& d, const KParams& p, int c8, int x, int y)
   {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
     if (x >= d.W || y >= d.H)
       return;
     const unsigned char* Db = reinterpret_cast<const unsigned char*>(d.D);
@@ -89,6 +87,19 @@ namespace dtk
       }
     }
     d.pred[(long long)y * d.pred_pitch + x] = code;
+  }
+
+  template <int KIND>
+  __global__ void k_pred(ImgDesc d, KParams p, int c8)
+  {
+    pred_pixel<KIND>(d, p, c8, (int)(blockIdx.x * blockDim.x + threadIdx.x), (int)blockIdx.y);
+  }
+
+  // A batch of images in one launch: image blockIdx.z of the table.
+  template <int KIND>
+  __global__ void k_pred_batched(const ImgDesc* tab, KParams p, int c8)
+  {
+    pred_pixel<KIND>(tab[blockIdx.z], p, c8, (int)(blockIdx.x * blockDim.x + threadIdx.x), (int)blockIdx.y);
   }
 
   // Children count of every pixel (gather, no atomics).
@@ -248,6 +259,27 @@ namespace dtk
     reinterpret_cast<float*>(gi + (long long)y * gi_pitch)[x] = acc;
     if (gm)
       reinterpret_cast<float*>(gm + (long long)y * gm_pitch)[x] = (code >= 8) ? -p.v * G[id] : 0.f;
+  }
+
+  hipError_t launch_pred_batched(const ImgDesc* tab, int n, int maxW, int maxH, const KParams& p, bool c8,
+                                 hipStream_t s)
+  {
+    if (n <= 0 || maxW <= 0 || maxH <= 0)
+      return hipSuccess;
+    const dim3 grid((maxW + 255) / 256, maxH, n), blk(256);
+    switch (p.img_kind)
+    {
+    case IMG_U8:
+      hipLaunchKernelGGL(k_pred_batched<IMG_U8>, grid, blk, 0, s, tab, p, (int)c8);
+      break;
+    case IMG_RGB8:
+      hipLaunchKernelGGL(k_pred_batched<IMG_RGB8>, grid, blk, 0, s, tab, p, (int)c8);
+      break;
+    default:
+      hipLaunchKernelGGL(k_pred_batched<IMG_F32>, grid, blk, 0, s, tab, p, (int)c8);
+      break;
+    }
+    return hipGetLastError();
   }
 
   hipError_t launch_pred(const ImgDesc& d, const KParams& p, bool c8, hipStream_t s)

@@ -1147,6 +1147,60 @@ dtk_status dtk_ggdt_pred(const dtk_image* img, const dtk_image* mask, const dtk_
   return DTK_OK;
 }
 
+dtk_status dtk_ggdt_pred_batched(int n, const dtk_image* imgs, const dtk_image* masks, const dtk_image* dists,
+                                 dtk_image* preds, const dtk_params* p, void* stream)
+{
+  dtk_status st;
+  if (n < 0)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "negative image count");
+  if (n > 0 && (!imgs || !masks || !dists || !preds))
+    return fail(DTK_ERR_INVALID_ARGUMENT, "NULL image array");
+  if ((st = check_params(p)) != DTK_OK)
+    return st;
+  int maxW = 0, maxH = 0;
+  for (int i = 0; i < n; ++i)
+  {
+    if ((st = check_triplet(&imgs[i], &masks[i], &dists[i])) != DTK_OK)
+      return st;
+    if (imgs[i].dtype != imgs[0].dtype || masks[i].dtype != masks[0].dtype)
+      return fail(DTK_ERR_INVALID_ARGUMENT, "all images of a batch must share image and mask dtypes");
+    if ((st = check_image(&preds[i], "pred", 1 << DTK_U8)) != DTK_OK)
+      return st;
+    if (preds[i].width != imgs[i].width || preds[i].height != imgs[i].height)
+      return fail(DTK_ERR_INVALID_ARGUMENT, "pred size differs");
+    if (imgs[i].height > 65535)
+      return fail(DTK_ERR_UNSUPPORTED, "a batch takes at most 65535 rows per image");
+    maxW = std::max(maxW, imgs[i].width);
+    maxH = std::max(maxH, imgs[i].height);
+  }
+  if (n == 0 || maxW == 0 || maxH == 0)
+    return DTK_OK;
+  Workspace* w = nullptr;
+  if ((st = current_ws(&w, nullptr)) != DTK_OK)
+    return st;
+  std::lock_guard<std::mutex> lock(w->mu);
+  const size_t tbytes = align_up((size_t)n * sizeof(ImgDesc), 256);
+  if ((st = ws_reserve(*w, tbytes + 1024, std::max<size_t>(4096, tbytes))) != DTK_OK)
+    return st;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Carver      c(w->dbuf);
+  ImgDesc*    dtab = c.take<ImgDesc>(n);
+  ImgDesc*    htab = static_cast<ImgDesc*>(w->hbuf);
+  for (int i = 0; i < n; ++i)
+  {
+    dtk_image dc       = dists[i];
+    htab[i]            = make_desc(&imgs[i], &masks[i], &dc, 32, 1);
+    htab[i].pred       = static_cast<unsigned char*>(preds[i].data);
+    htab[i].pred_pitch = preds[i].pitch;
+  }
+  DTK_HIP(hipMemcpyAsync(dtab, htab, (size_t)n * sizeof(ImgDesc), hipMemcpyHostToDevice, s), "upload table");
+  // The staging buffer is reused by the next call: let the copy finish.
+  DTK_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
+  const KParams kp = make_kparams(p, imgs[0].dtype, masks[0].dtype, 32);
+  DTK_HIP(launch_pred_batched(dtab, n, maxW, maxH, kp, p->connectivity == DTK_C8, s), "launch pred");
+  return DTK_OK;
+}
+
 dtk_status dtk_ggdt_backward(const dtk_image* img, const dtk_image* mask, const dtk_image* dist,
                              const dtk_image* pred, const dtk_image* grad_dist, dtk_image* grad_img,
                              dtk_image* grad_mask, const dtk_params* p, void* stream)

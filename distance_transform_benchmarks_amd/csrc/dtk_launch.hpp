@@ -33,6 +33,8 @@ namespace dtk
 
   // dtk_adjoint.hip
   hipError_t launch_pred(const ImgDesc& d, const KParams& p, bool c8, hipStream_t s);
+  hipError_t launch_pred_batched(const ImgDesc* tab, int n, int maxW, int maxH, const KParams& p, bool c8,
+                                 hipStream_t s);
   hipError_t launch_backward(const BackJob* jobs, int n, int maxW, int maxH, const KParams& p, hipStream_t s);
 
   // dtk_selftest.hip

@@ -176,6 +176,12 @@ dtk_status dtk_batch_autotune(int n, const dtk_image* imgs, const dtk_image* mas
 dtk_status dtk_ggdt_pred(const dtk_image* img, const dtk_image* mask, const dtk_image* dist, dtk_image* pred,
                          const dtk_params* p, void* stream);
 
+/* Predecessor codes of n converged maps in one launch: arrays of n
+ * descriptors, as for dtk_ggdt_pred; every image of the batch has the same
+ * dtype, and so has every mask.  No reference counterpart. */
+dtk_status dtk_ggdt_pred_batched(int n, const dtk_image* imgs, const dtk_image* masks, const dtk_image* dists,
+                                 dtk_image* preds, const dtk_params* p, void* stream);
+
 /* Adjoint of L = sum(W * D) through the predecessor forest, for DTK_F32
  * images and soft DTK_F32 masks: writes dL/dimage and dL/dmask (DTK_F32).
  * `grad_dist` holds dL/dD (= W).  No reference counterpart. */
@@ -186,9 +192,10 @@ dtk_status dtk_ggdt_backward(const dtk_image* img, const dtk_image* mask, const 
 /* The adjoint of n images in one launch of each kernel: arrays of n
  * descriptors, as for dtk_ggdt_backward (grad_masks may be NULL; every mask
  * of the batch has the same dtype).  The subtree sums
- * climb predecessor chains with two atomics per step, so an image costs as
- * long as its longest chain; in one launch the chains of all images climb
- * together.  No reference counterpart. */
+ * climb predecessor chains (one dependent load per step, an atomic exchange
+ * where chains merge), so an image costs as long as its longest chain; in
+ * one launch the chains of all images climb together.  No reference
+ * counterpart. */
 dtk_status dtk_ggdt_backward_batched(int n, const dtk_image* imgs, const dtk_image* masks, const dtk_image* dists,
                                      const dtk_image* preds, const dtk_image* grad_dists, dtk_image* grad_imgs,
                                      dtk_image* grad_masks, const dtk_params* p, void* stream);

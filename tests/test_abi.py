@@ -245,3 +245,27 @@ def test_gbench_corpus_and_summary():
         {"name": "DistanceTransformTask/2/1/manual_time", "lambda": 0.5, "bytes_per_second": 4e8},
         {"name": "DistanceTransformGeos/0/0/manual_time", "lambda": 0.0, "bytes_per_second": 1e7}]}
     assert gbench.summarize(doc) == {"DistanceTransformGeos lambda=0.0": 10.0, "DistanceTransformTask lambda=0.5": 400.0}
+
+
+def test_batched_predecessors_reject_bad_arguments_on_host(lib):
+    """dtk_ggdt_pred_batched checks its arguments before the first HIP call."""
+    from distance_transform_benchmarks_amd import _lib
+
+    p = _lib.DtkParams()
+    lib.dtk_params_default(ctypes.byref(p))
+    arr = lambda *ims: (_lib.DtkImage * len(ims))(*ims)  # noqa: E731
+    f32 = lambda w=8, h=8: _img(w, h, 2)  # noqa: E731
+    u8 = lambda w=8, h=8: _img(w, h, 0)  # noqa: E731
+    assert lib.dtk_ggdt_pred_batched(-1, None, None, None, None, ctypes.byref(p), None) == _lib.DTK_ERR_INVALID_ARGUMENT
+    assert lib.dtk_ggdt_pred_batched(1, None, None, None, None, ctypes.byref(p), None) == _lib.DTK_ERR_INVALID_ARGUMENT
+    assert "NULL" in lib.dtk_last_error().decode()
+    assert lib.dtk_ggdt_pred_batched(0, None, None, None, None, ctypes.byref(p), None) == _lib.DTK_OK
+    cases = [
+        (arr(u8(), f32()), arr(u8(), u8()), arr(f32(), f32()), arr(u8(), u8()), "share image and mask dtypes"),
+        (arr(u8(), u8()), arr(u8(), u8()), arr(f32(), f32()), arr(u8(), u8(9)), "pred size differs"),
+        (arr(u8(), u8()), arr(u8(), u8()), arr(f32(), f32()), arr(u8(), f32()), "pred"),
+    ]
+    for imgs, masks, dists, preds, msg in cases:
+        assert lib.dtk_ggdt_pred_batched(2, imgs, masks, dists, preds, ctypes.byref(p), None) == \
+            _lib.DTK_ERR_INVALID_ARGUMENT, msg
+        assert msg in lib.dtk_last_error().decode(), (msg, lib.dtk_last_error())

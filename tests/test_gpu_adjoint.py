@@ -193,3 +193,24 @@ def test_adjoint_c4_vs_cpu_reference(dt, dev, oracle, lam):
     for got, want, what in ((gi, gir, "dL/dimage"), (gm, gmr, "dL/dmask")):
         err = np.abs(got.cpu().numpy().astype(np.float64) - want).max()
         assert err <= 1e-4 * max(1.0, np.abs(want).max()), (what, err)
+
+
+@pytest.mark.parametrize("kind", ["f32", "u8", "rgb8"])
+def test_predecessors_batched_equals_per_image(dt, dev, kind):
+    """dtk_ggdt_pred_batched (one launch for uneven images) writes the same
+    codes as one dtk_ggdt_pred call per image."""
+    import torch
+
+    shapes = SHAPES_9
+    imgs, masks, Ds = [], [], []
+    for i, (H, W) in enumerate(shapes):
+        di, dm = _t(image(kind, W, H, 300 + i), dev), _t(mask("f32", W, H, 300 + i, 2), dev)
+        imgs.append(di)
+        masks.append(dm)
+        Ds.append(dt.ggdt(di, dm, 0.7, 1e3))
+    _say("pred batched", kind)
+    got = dt.predecessors_batched(imgs, masks, Ds, 0.7, 1e3)
+    for i in range(len(shapes)):
+        want = dt.predecessors(imgs[i], masks[i], Ds[i], 0.7, 1e3)
+        torch.cuda.synchronize()
+        assert torch.equal(got[i], want), i
