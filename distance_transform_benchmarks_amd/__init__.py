@@ -22,14 +22,18 @@ from .api import (  # noqa: F401
     generalised_distance_transform_task,
     ggdt,
     ggdt_batched,
+    heatmap,
     inferno,
     minmax,
     normalize_u8,
     predecessors,
     round_per_blocks_buffer,
     selftest,
+    synchronize,
     tile_count,
     tile_grid,
+    tile_image,
+    tile_size,
     visits_buffer,
 )
 from .autograd import GGDTFunction, ggdt_differentiable  # noqa: F401

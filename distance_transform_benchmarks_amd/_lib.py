@@ -21,6 +21,7 @@ DTK_ERR_UNSUPPORTED = 4
 DTK_U8, DTK_RGB8, DTK_F32 = 0, 1, 2
 DTK_C4, DTK_C8 = 4, 8
 SCHED_RASTER, SCHED_BLOCKS, SCHED_TASK = 0, 1, 2
+ORDER_DEFAULT, ORDER_FIFO, ORDER_KEY, ORDER_SEED = 0, 1, 2, 3
 
 # Every symbol include/dtk.h declares (checked by the CPU test suite).
 EXPORTS = (
@@ -29,7 +30,8 @@ EXPORTS = (
     "dtk_ggdt_pred_batched", "dtk_ggdt_backward", "dtk_ggdt_backward_batched", "dtk_tile_count", "dtk_malloc_pitch", "dtk_free", "dtk_host_to_device",
     "dtk_device_to_host", "dtk_copy2d", "dtk_selftest", "dtk_device_cu_count", "dtk_profile_begin",
     "dtk_profile_end", "dtk_batch_autotune", "dtk_tile_count_for", "dtk_minmax_f32", "dtk_normalize_u8",
-    "dtk_inferno", "dtk_colorize", "dtk_block_priorities",
+    "dtk_inferno", "dtk_colorize", "dtk_block_priorities", "dtk_sync_errors", "dtk_tile_size_for",
+    "dtk_tile_image",
 )
 
 
@@ -42,7 +44,7 @@ class DtkParams(ctypes.Structure):
     _fields_ = [("lambda_", ctypes.c_float), ("v", ctypes.c_float), ("connectivity", ctypes.c_int),
                 ("schedule", ctypes.c_int), ("max_iterations", ctypes.c_int), ("tile_rows", ctypes.c_int),
                 ("max_sweeps", ctypes.c_int), ("waves_per_cu", ctypes.c_int), ("cost_plane", ctypes.c_int),
-                ("tile_waves", ctypes.c_int),
+                ("tile_waves", ctypes.c_int), ("queue_order", ctypes.c_int),
                 ("round_per_blocks", ctypes.c_void_p)]
 
 
@@ -51,7 +53,9 @@ class DtkStats(ctypes.Structure):
                 ("tiles", ctypes.c_int), ("tile_rows", ctypes.c_int), ("workers", ctypes.c_int),
                 ("cycles_wait", ctypes.c_uint32), ("cycles_visit", ctypes.c_uint32), ("cycles_pub", ctypes.c_uint32),
                 ("cycles_setup", ctypes.c_uint32), ("cycles_sweep", ctypes.c_uint32), ("cycles_store", ctypes.c_uint32),
-                ("cycles_release", ctypes.c_uint32), ("cycles_acquire", ctypes.c_uint32)]
+                ("cycles_release", ctypes.c_uint32), ("cycles_acquire", ctypes.c_uint32),
+                ("drains", ctypes.c_uint32), ("moved", ctypes.c_uint32), ("deferred", ctypes.c_uint32),
+                ("cycles_drain", ctypes.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -123,10 +127,16 @@ def load() -> ctypes.CDLL:
     lib.dtk_inferno.argtypes = [P(DtkImage), P(DtkImage), vp]
     lib.dtk_colorize.argtypes = [P(DtkImage), P(DtkImage), vp]
     lib.dtk_block_priorities.argtypes = [P(DtkImage), vp, vp, P(ctypes.c_uint32), vp]
+    lib.dtk_sync_errors.argtypes = [vp]
+    lib.dtk_tile_size_for.argtypes = [P(DtkParams), P(ctypes.c_int), P(ctypes.c_int)]
+    lib.dtk_tile_size_for.restype = ctypes.c_int
+    lib.dtk_tile_image.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   P(DtkImage), vp]
     for name in ("dtk_ggdt", "dtk_ggdt_batched", "dtk_batch_plan_create", "dtk_batch_plan_launch",
                  "dtk_batch_plan_destroy", "dtk_batch_autotune", "dtk_ggdt_pred", "dtk_ggdt_pred_batched", "dtk_ggdt_backward", "dtk_ggdt_backward_batched", "dtk_malloc_pitch", "dtk_free",
                  "dtk_host_to_device", "dtk_device_to_host", "dtk_copy2d", "dtk_selftest", "dtk_profile_begin",
-                 "dtk_profile_end"):
+                 "dtk_profile_end", "dtk_sync_errors", "dtk_block_priorities", "dtk_minmax_f32", "dtk_normalize_u8",
+                 "dtk_inferno", "dtk_colorize", "dtk_tile_image"):
         getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib

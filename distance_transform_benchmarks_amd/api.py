@@ -36,7 +36,7 @@ __all__ = [
     "generalised_distance_transform", "generalised_distance_transform_blocks",
     "generalised_distance_transform_task", "ggdt", "ggdt_batched", "BatchPlan", "predecessors",
     "backward", "selftest", "tile_count", "round_per_blocks_buffer", "tile_grid", "visits_buffer",
-    "minmax", "normalize_u8", "inferno", "colorize",
+    "minmax", "normalize_u8", "inferno", "colorize", "synchronize", "tile_size", "tile_image", "heatmap",
 ]
 
 
@@ -67,8 +67,12 @@ def _as_image(t: torch.Tensor, kind: str) -> DtkImage:
 _SCHED = {"raster": _lib.SCHED_RASTER, "blocks": _lib.SCHED_BLOCKS, "task": _lib.SCHED_TASK}
 
 
+_ORDER = {None: _lib.ORDER_DEFAULT, "default": _lib.ORDER_DEFAULT, "fifo": _lib.ORDER_FIFO, "key": _lib.ORDER_KEY,
+          "seed": _lib.ORDER_SEED}
+
+
 def _params(lambda_, v, connectivity, schedule, max_iterations, tile_rows, max_sweeps, waves_per_cu,
-            round_per_blocks=None, cost_plane=False, tile_waves=0) -> DtkParams:
+            round_per_blocks=None, cost_plane=False, tile_waves=0, queue_order=None) -> DtkParams:
     p = _lib.default_params()
     p.lambda_ = float(lambda_)
     p.v = float(v)
@@ -82,6 +86,9 @@ def _params(lambda_, v, connectivity, schedule, max_iterations, tile_rows, max_s
     p.waves_per_cu = int(waves_per_cu)
     p.cost_plane = 1 if cost_plane else 0
     p.tile_waves = int(tile_waves)
+    if queue_order not in _ORDER:
+        raise ValueError(f"queue_order must be one of {sorted(k for k in _ORDER if k)}")
+    p.queue_order = _ORDER[queue_order]
     if round_per_blocks is not None:
         if round_per_blocks.dtype != torch.int32 or not round_per_blocks.is_cuda:
             raise ValueError("round_per_blocks must be an int32 device tensor")
@@ -92,15 +99,20 @@ def _params(lambda_, v, connectivity, schedule, max_iterations, tile_rows, max_s
 def ggdt(img: torch.Tensor, mask: torch.Tensor, lambda_: float, v: float = 1e10, *, connectivity: int = 8,
          schedule: str = "task", max_iterations: int = 0, tile_rows: int = 0, max_sweeps: int = 0,
          waves_per_cu: int = 0, out: Optional[torch.Tensor] = None, stats: bool = False,
-         round_per_blocks: Optional[torch.Tensor] = None, cost_plane: bool = False, tile_waves: int = 0):
-    """GGDT of one image (any schedule).  Returns D, or (D, stats dict)."""
+         round_per_blocks: Optional[torch.Tensor] = None, cost_plane: bool = False, tile_waves: int = 0,
+         queue_order: Optional[str] = None):
+    """GGDT of one image (any schedule).  Returns D, or (D, stats dict).
+
+    queue_order (task schedule): "fifo" (the default for one image), "key"
+    or "seed"; the order in which dirty tiles are solved, which never changes
+    the result."""
     H, W = mask.shape[0], mask.shape[1]
     if img.shape[0] != H or img.shape[1] != W:
         raise ValueError("image and mask sizes differ")
     if out is None:
         out = torch.empty((H, W), dtype=torch.float32, device=img.device)
     p = _params(lambda_, v, connectivity, schedule, max_iterations, tile_rows, max_sweeps, waves_per_cu,
-                round_per_blocks, cost_plane, tile_waves)
+                round_per_blocks, cost_plane, tile_waves, queue_order)
     if round_per_blocks is not None:
         need = _lib.load().dtk_tile_count_for(W, H, ctypes.byref(p))
         if not round_per_blocks.is_contiguous() or round_per_blocks.numel() < need:
@@ -173,14 +185,14 @@ def _batch_arrays(imgs, masks, outs):
 def ggdt_batched(imgs: Sequence[torch.Tensor], masks: Sequence[torch.Tensor], lambda_: float, v: float = 1e10, *,
                  connectivity: int = 8, tile_rows: int = 0, max_sweeps: int = 0, waves_per_cu: int = 0,
                  outs: Optional[List[torch.Tensor]] = None, stats: bool = False, cost_plane: bool = False,
-                 tile_waves: int = 0):
+                 tile_waves: int = 0, queue_order: Optional[str] = None):
     """GGDT of many images of any sizes in one persistent launch."""
     if len(imgs) != len(masks):
         raise ValueError("imgs and masks differ in length")
     if outs is None:
         outs = [torch.empty(m.shape[:2], dtype=torch.float32, device=m.device) for m in masks]
     p = _params(lambda_, v, connectivity, "task", 0, tile_rows, max_sweeps, waves_per_cu,
-                cost_plane=cost_plane, tile_waves=tile_waves)
+                cost_plane=cost_plane, tile_waves=tile_waves, queue_order=queue_order)
     ai, am, ad = _batch_arrays(imgs, masks, outs)
     st = DtkStats()
     dev = imgs[0].device if imgs else torch.device("cuda")
@@ -203,8 +215,9 @@ def autotune(imgs, masks, lambda_, v=1e10, *, connectivity=8, outs=None, reps=2,
     with torch.cuda.device(dev):
         check(_lib.load().dtk_batch_autotune(len(imgs), ai, am, ad, ctypes.byref(p), _stream_handle(dev), int(reps),
                                              ctypes.byref(best), ctypes.byref(rep)))
+    order = {v: k for k, v in _ORDER.items() if k and k != "default"}.get(best.queue_order, "key")
     return ({"tile_rows": best.tile_rows, "tile_waves": best.tile_waves, "max_sweeps": best.max_sweeps,
-             "waves_per_cu": best.waves_per_cu}, rep.as_dict())
+             "waves_per_cu": best.waves_per_cu, "queue_order": order}, rep.as_dict())
 
 
 class BatchPlan:
@@ -217,7 +230,7 @@ class BatchPlan:
     """
 
     def __init__(self, imgs, masks, lambda_, v=1e10, *, connectivity=8, tile_rows=0, max_sweeps=0,
-                 waves_per_cu=0, outs=None, cost_plane=False, tile_waves=0, tune=False):
+                 waves_per_cu=0, outs=None, cost_plane=False, tile_waves=0, tune=False, queue_order=None):
         if outs is None:
             outs = [torch.empty(m.shape[:2], dtype=torch.float32, device=m.device) for m in masks]
         self.imgs, self.masks, self.outs = list(imgs), list(masks), list(outs)
@@ -228,11 +241,12 @@ class BatchPlan:
                                       cost_plane=cost_plane)
             tile_rows, max_sweeps, waves_per_cu = chosen["tile_rows"], chosen["max_sweeps"], chosen["waves_per_cu"]
             tile_waves = chosen["tile_waves"]
+            queue_order = chosen["queue_order"]
             self.tuning = {**chosen, **report}
         self.params = {"tile_rows": tile_rows, "tile_waves": tile_waves, "max_sweeps": max_sweeps,
-                       "waves_per_cu": waves_per_cu}
+                       "waves_per_cu": waves_per_cu, "queue_order": queue_order or "default"}
         p = _params(lambda_, v, connectivity, "task", 0, tile_rows, max_sweeps, waves_per_cu,
-                    cost_plane=cost_plane, tile_waves=tile_waves)
+                    cost_plane=cost_plane, tile_waves=tile_waves, queue_order=queue_order)
         ai, am, ad = _batch_arrays(self.imgs, self.masks, self.outs)
         self._plan = ctypes.c_void_p()
         with torch.cuda.device(self.device):
@@ -320,6 +334,56 @@ def backward_batched(imgs, masks, dists, preds, grad_dists, lambda_, v=1e10, *, 
         with torch.cuda.device(imgs[0].device):
             check(_lib.load().dtk_ggdt_backward_batched(n, *A, ctypes.byref(p), _stream_handle(imgs[0].device)))
     return gis, gms
+
+
+def tile_size(lambda_: float, *, schedule: str = "task", tile_rows: int = 0, tile_waves: int = 0,
+              max_sweeps: int = 0):
+    """(tile_w, tile_h): owned pixels of one tile of a call with these
+    parameters (dtk_tile_size_for); pair with ``tile_grid``."""
+    p = _params(lambda_, 1e10, 8, schedule, 0, tile_rows, max_sweeps, 0, tile_waves=tile_waves)
+    tw, th = ctypes.c_int(), ctypes.c_int()
+    if not _lib.load().dtk_tile_size_for(ctypes.byref(p), ctypes.byref(tw), ctypes.byref(th)):
+        raise ValueError("invalid tile parameters")
+    return tw.value, th.value
+
+
+def tile_image(values: torch.Tensor, width: int, height: int, tile_w: int, tile_h: int,
+               out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Per-tile values (an int32 or uint8 device tensor of shape (tiles_y,
+    tiles_x)) spread over their tiles' pixels: a float32 (height, width)
+    image, the reference's block_attribute_to_image
+    (bin/rounds_per_blocks.cu:13-23) for any tile size (dtk_tile_image)."""
+    if values.dim() != 2 or not values.is_cuda or not values.is_contiguous():
+        raise ValueError("values must be a contiguous (tiles_y, tiles_x) device tensor")
+    if values.dtype not in (torch.int32, torch.uint8):
+        raise ValueError("values must be int32 or uint8")
+    if out is None:
+        out = torch.empty((height, width), dtype=torch.float32, device=values.device)
+    nb = 4 if values.dtype == torch.int32 else 1
+    with torch.cuda.device(values.device):
+        check(_lib.load().dtk_tile_image(ctypes.c_void_p(values.data_ptr()), nb, values.shape[1], values.shape[0],
+                                         int(tile_w), int(tile_h), ctypes.byref(_as_image(out, "out")),
+                                         _stream_handle(values.device)))
+    return out
+
+
+def heatmap(values: torch.Tensor, width: int, height: int, tile_w: int, tile_h: int) -> torch.Tensor:
+    """Inferno heat map (uint8 (height, width, 3)) of per-tile values, e.g.
+    the visit counts of ``visits_buffer`` or the priorities of
+    ``block_priorities``: tile_image, then normalisation and the colour map
+    (the reference's bin/rounds_per_blocks.cu:51-58 and
+    bin/priority_block.cu:44-60)."""
+    return colorize(tile_image(values, width, height, tile_w, tile_h))
+
+
+def synchronize(device=None) -> None:
+    """Wait for torch's current stream on `device` and raise any error that
+    an earlier asynchronous call left on the device (dtk_sync_errors): a
+    soft mask value outside [0, 1] raises ValueError, a device fault
+    DtkError.  Without it such an error is raised by the next call."""
+    dev = torch.device("cuda") if device is None else torch.device(device)
+    with torch.cuda.device(dev):
+        check(_lib.load().dtk_sync_errors(_stream_handle(dev)))
 
 
 class KernelTimer:

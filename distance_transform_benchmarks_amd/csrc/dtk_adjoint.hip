@@ -310,13 +310,16 @@ namespace dtk
       return hipSuccess;
     const dim3 grid((maxW + 255) / 256, maxH, n), blk(256);
     hipLaunchKernelGGL(k_child_count, grid, blk, 0, s, jobs);
-    if (n >= 8) // fewer images would leave XCDs idle
-    {
-      const unsigned gx = grid.x, gy = grid.y, slots = (unsigned)(n + 7) / 8u;
-      hipLaunchKernelGGL(k_subtree, dim3(8u * slots * gx * gy), blk, 0, s, jobs, (int)gx, (int)gy, n);
-    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+      return e;
+    // Each kernel reads what the one before wrote: stop at the first failed launch.
+    if (const unsigned long long blocks = subtree_grid_1d(n, maxW, maxH))
+      hipLaunchKernelGGL(k_subtree, dim3((unsigned)blocks), blk, 0, s, jobs, (int)grid.x, (int)grid.y, n);
     else
       hipLaunchKernelGGL(k_subtree, grid, blk, 0, s, jobs, 0, 0, n);
+    if ((e = hipGetLastError()) != hipSuccess)
+      return e;
     hipLaunchKernelGGL(k_grad, grid, blk, 0, s, jobs, p);
     return hipGetLastError();
   }

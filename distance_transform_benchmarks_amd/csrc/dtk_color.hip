@@ -268,6 +268,40 @@ namespace dtk
     return hipGetLastError();
   }
 
+  // Per-tile values spread over the pixels of their tiles, as float: the
+  // image the reference's block_attribute_to_image builds
+  // (bin/rounds_per_blocks.cu:13-23, bin/priority_block.cu:12-22), here for
+  // any tile size.  Tile (tx, ty) covers x in [tx * tw, (tx + 1) * tw),
+  // y in [ty * th, (ty + 1) * th); values are row-major over tiles_x tiles a
+  // row, int32 (BYTES 4) or uint8 (BYTES 1).
+  template <int BYTES>
+  __global__ void __launch_bounds__(256) k_tile_image(const void* values, int tiles_x, int tw, int th, void* out,
+                                                       long long out_pitch, int W, int H)
+  {
+    const int x = blockIdx.x * 64 + (int)threadIdx.x, y = blockIdx.y * 4 + (int)threadIdx.y;
+    if (x >= W || y >= H)
+      return;
+    const long long t = (long long)(y / th) * tiles_x + x / tw;
+    float           v;
+    if constexpr (BYTES == 4)
+      v = (float)static_cast<const int32_t*>(values)[t];
+    else
+      v = (float)static_cast<const uint8_t*>(values)[t];
+    reinterpret_cast<float*>(static_cast<unsigned char*>(out) + (long long)y * out_pitch)[x] = v;
+  }
+
+  hipError_t launch_tile_image(const void* values, int value_bytes, int tiles_x, int tw, int th, void* out,
+                               long long out_pitch, int W, int H, hipStream_t s)
+  {
+    if (value_bytes == 4)
+      hipLaunchKernelGGL(k_tile_image<4>, pixel_grid(W, H), dim3(64, 4), 0, s, values, tiles_x, tw, th, out,
+                         out_pitch, W, H);
+    else
+      hipLaunchKernelGGL(k_tile_image<1>, pixel_grid(W, H), dim3(64, 4), 0, s, values, tiles_x, tw, th, out,
+                         out_pitch, W, H);
+    return hipGetLastError();
+  }
+
   float key_to_float(uint32_t k)
   {
     uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;

@@ -48,14 +48,36 @@ namespace
 
   constexpr int kMaxDevices = 64;
 
+  // Pinned host buffers for host-to-device uploads of call tables, used in
+  // turn; each is reused only once the copy that read it has completed (its
+  // event), so an upload never makes the caller wait for the device.
+  constexpr int kStages = 8;
+  struct Stage
+  {
+    void*      h   = nullptr;
+    size_t     cap = 0;
+    hipEvent_t ev  = nullptr;
+    bool       pending = false;
+  };
+
   struct Workspace
   {
-    std::mutex mu;
-    void*      dbuf = nullptr;
-    size_t     dcap = 0;
-    void*      hbuf = nullptr; // pinned host staging
-    size_t     hcap = 0;
-    int        cu   = 0;
+    std::mutex  mu;
+    void*       dbuf = nullptr;
+    size_t      dcap = 0;
+    void*       hbuf = nullptr; // pinned host buffer for synchronous read-backs
+    size_t      hcap = 0;
+    int         cu   = 0;
+    // Stream order of the calls that use dbuf: the event of the last such
+    // call and its stream.  A call on another stream first waits for it.
+    hipEvent_t  done = nullptr;
+    hipStream_t last = nullptr;
+    bool        used = false;
+    // Error mailbox in pinned host memory: the persistent kernels of an
+    // asynchronous call store their error bits here; the next call reads it.
+    uint32_t*   mailbox = nullptr;
+    Stage       stage[kStages];
+    int         next_stage = 0;
   };
 
   Workspace g_ws[kMaxDevices];
@@ -122,8 +144,18 @@ namespace
     Workspace& w = g_ws[dev];
     if (w.cu == 0)
     {
+      std::lock_guard<std::mutex> lock(w.mu);
       int cu = 0;
       DTK_HIP(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+      if (!w.done)
+        DTK_HIP(hipEventCreateWithFlags(&w.done, hipEventDisableTiming), "hipEventCreate");
+      if (!w.mailbox)
+      {
+        void* m = nullptr;
+        DTK_HIP(hipHostMalloc(&m, 256, hipHostMallocDefault), "hipHostMalloc(mailbox)");
+        w.mailbox                           = static_cast<uint32_t*>(m);
+        *static_cast<volatile uint32_t*>(m) = 0u;
+      }
       w.cu = cu > 0 ? cu : 1;
     }
     *out = &w;
@@ -160,6 +192,109 @@ namespace
       DTK_HIP(hipHostMalloc(&w.hbuf, cap, hipHostMallocDefault), "hipHostMalloc(staging)");
       w.hcap = cap;
     }
+    return DTK_OK;
+  }
+
+  // Device error bits -> status and message.
+  dtk_status error_from_bits(uint32_t bits, const char* where)
+  {
+    if (bits & ERR_MASK_RANGE)
+      return fail(DTK_ERR_INVALID_ARGUMENT, std::string("soft mask values must be in [0, 1]") + where);
+    if (bits & ERR_QUEUE)
+      return fail(DTK_ERR_HIP, std::string("work queue fault: a tile id read back from the queue was out of range") + where);
+    if (bits & ERR_WATCHDOG)
+      return fail(DTK_ERR_HIP, std::string("device watchdog expired in the work queue") + where);
+    return DTK_OK;
+  }
+
+  // Errors that the kernels of earlier asynchronous calls on this device
+  // left in its mailbox.  Reading it is a plain host read, so a call with no
+  // error pending never waits; when one is there, the device is synchronised
+  // first (no kernel writes the mailbox any more), then the error is taken
+  // and reported once.
+  dtk_status take_async_error(Workspace& w)
+  {
+    volatile uint32_t* mb = w.mailbox;
+    if (!mb || *mb == 0u)
+      return DTK_OK;
+    DTK_HIP(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    const uint32_t bits = *mb;
+    *mb                 = 0u;
+    return error_from_bits(bits, " (an earlier asynchronous call on this device)");
+  }
+
+  // A call's use of the device workspace: holds its lock, orders the call
+  // after the previous user of the workspace when that ran on another
+  // stream (the stream waits for its event; on one stream the order is
+  // already there), reports pending asynchronous errors, and at scope exit
+  // records the call's own event.
+  class WsUse
+  {
+  public:
+    WsUse(Workspace& w, hipStream_t s)
+      : w_(w)
+      , s_(s)
+      , lock_(w.mu)
+    {
+    }
+    dtk_status begin()
+    {
+      dtk_status st;
+      if ((st = take_async_error(w_)) != DTK_OK)
+        return st;
+      if (w_.used && w_.last != s_)
+        DTK_HIP(hipStreamWaitEvent(s_, w_.done, 0), "hipStreamWaitEvent");
+      entered_ = true;
+      return DTK_OK;
+    }
+    ~WsUse()
+    {
+      if (entered_ && hipEventRecord(w_.done, s_) == hipSuccess)
+      {
+        w_.last = s_;
+        w_.used = true;
+      }
+    }
+
+  private:
+    Workspace&                  w_;
+    hipStream_t                 s_;
+    std::lock_guard<std::mutex> lock_;
+    bool                        entered_ = false;
+  };
+
+  // Pinned staging buffer of at least `bytes` for an upload on stream s:
+  // the next buffer of the ring, once the last copy that read it is done.
+  // Call stage_done after enqueueing the copy.  Caller holds the workspace
+  // lock.
+  dtk_status stage_get(Workspace& w, size_t bytes, Stage** out)
+  {
+    Stage& st = w.stage[w.next_stage];
+    w.next_stage = (w.next_stage + 1) % kStages;
+    if (st.pending)
+    {
+      DTK_HIP(hipEventSynchronize(st.ev), "hipEventSynchronize");
+      st.pending = false;
+    }
+    if (!st.ev)
+      DTK_HIP(hipEventCreateWithFlags(&st.ev, hipEventDisableTiming), "hipEventCreate");
+    if (bytes > st.cap)
+    {
+      if (st.h)
+        DTK_HIP(hipHostFree(st.h), "hipHostFree");
+      st.h   = nullptr;
+      st.cap = 0;
+      const size_t cap = align_up(bytes, 1 << 16);
+      DTK_HIP(hipHostMalloc(&st.h, cap, hipHostMallocDefault), "hipHostMalloc(staging)");
+      st.cap = cap;
+    }
+    *out = &st;
+    return DTK_OK;
+  }
+  dtk_status stage_done(Stage* st, hipStream_t s)
+  {
+    DTK_HIP(hipEventRecord(st->ev, s), "hipEventRecord");
+    st->pending = true;
     return DTK_OK;
   }
 
@@ -223,7 +358,29 @@ namespace
       return fail(DTK_ERR_INVALID_ARGUMENT, "negative iteration / sweep / wave count");
     if (p->max_iterations > 0 && p->schedule != DTK_SCHED_RASTER)
       return fail(DTK_ERR_INVALID_ARGUMENT, "a fixed iteration count needs the raster schedule");
+    if (p->queue_order < DTK_ORDER_DEFAULT || p->queue_order > DTK_ORDER_SEED)
+      return fail(DTK_ERR_INVALID_ARGUMENT, "unknown queue order");
     return DTK_OK;
+  }
+
+  // Queue order of a task-schedule call over nimg images.  DEFAULT: KEY for
+  // batches (256 uneven uint8 images at lambda 0.8: 11.7 ms against 28.8 ms
+  // FIFO in one launch), FIFO for one image (4096^2 RGB8 at lambda 0.5:
+  // 1.61 ms against 1.83 ms KEY; the 8192^2 spiral maze: 1.52 s against
+  // 2.11 s).
+  int queue_order(const dtk_params* p, int nimg)
+  {
+    switch (p->queue_order)
+    {
+    case DTK_ORDER_FIFO:
+      return ORDER_FIFO;
+    case DTK_ORDER_KEY:
+      return ORDER_KEY;
+    case DTK_ORDER_SEED:
+      return ORDER_SEED;
+    default:
+      return nimg > 1 ? ORDER_KEY : ORDER_FIFO;
+    }
   }
 
   // lambda 0 without a sweep cap: the tile kernels use constant chamfer
@@ -352,7 +509,17 @@ namespace
     uint32_t* tile_img = nullptr;
   };
 
-  size_t task_layout(void* base, int ntiles, uint32_t cap, int nimg, TaskBufs* out)
+  // Buckets above the front whose wakes still go to the main ring (bucket
+  // orders; two buckets per octave of the distance for ORDER_KEY).
+#ifndef DTK_QUEUE_WINDOW
+#define DTK_QUEUE_WINDOW 0
+#endif
+  constexpr int kQueueWindow = DTK_QUEUE_WINDOW;
+
+  // The main ring (cap slots, kSlotStride words apart) and, for a bucket
+  // order, the later rings (kBuckets x cap slots) with per-tile keys and
+  // priorities.  cap is a power of two.
+  size_t task_layout(void* base, int ntiles, uint32_t cap, int nimg, int order, TaskBufs* out)
   {
     Carver c(base);
     TaskBufs b;
@@ -361,7 +528,19 @@ namespace
     b.q.tmax     = c.take<float>(ntiles);
     b.q.state    = c.take<uint32_t>((size_t)ntiles * kStateStride);
     b.q.slots    = c.take<unsigned long long>((size_t)cap * kSlotStride);
+    b.q.window   = order == ORDER_FIFO ? (int)kBuckets : kQueueWindow;
+    if (order != ORDER_FIFO)
+    {
+      b.q.bslots = c.take<unsigned long long>((size_t)kBuckets * cap);
+      b.q.tkey   = c.take<uint32_t>(ntiles);
+      b.q.tprio  = c.take<uint8_t>(ntiles);
+    }
     b.q.cap_mask = cap - 1;
+    b.q.cap_log2 = 0;
+    while ((1u << b.q.cap_log2) < cap)
+      ++b.q.cap_log2;
+    b.q.ntiles = ntiles;
+    b.q.order  = order;
     if (nimg > 1)
     {
       b.table    = c.take<ImgDesc>(nimg);
@@ -413,6 +592,10 @@ namespace
     st->cycles_store = ctrl_host ? ctrl_host[CTRL_T_STORE] : 0;
     st->cycles_release = ctrl_host ? ctrl_host[CTRL_T_REL] : 0;
     st->cycles_acquire = ctrl_host ? ctrl_host[CTRL_T_ACQ] : 0;
+    st->drains         = ctrl_host ? ctrl_host[CTRL_DRAINS] : 0;
+    st->moved          = ctrl_host ? ctrl_host[CTRL_MOVED] : 0;
+    st->deferred       = ctrl_host ? ctrl_host[CTRL_DEFER] : 0;
+    st->cycles_drain   = ctrl_host ? ctrl_host[CTRL_T_DRAIN] : 0;
     return DTK_OK;
   }
 
@@ -455,7 +638,7 @@ namespace
       prof_mark(s);
     DTK_HIP(launch_reset_ctrl(q.ctrl, s), "launch reset");
     DTK_HIP(launch_init(b, kp, q, s), "launch init");
-    DTK_HIP(launch_seed(b, q, s), "launch seed");
+    DTK_HIP(launch_seed(b, kp, q, s), "launch seed");
     if (profile)
       prof_mark(s);
     DTK_HIP(launch_task(S, c8, cm, b, kp, q, workers, s), "launch task");
@@ -464,17 +647,18 @@ namespace
     return DTK_OK;
   }
 
+  // Synchronous read-back of a task call's counters (calls with stats).  An
+  // error is reported here, so the copy the kernel left in the mailbox is
+  // taken too (the stream is synchronised: no worker writes it any more).
   dtk_status read_ctrl(Workspace& w, const Queue& q, hipStream_t s, uint32_t** host)
   {
     uint32_t* h = static_cast<uint32_t*>(w.hbuf);
     DTK_HIP(hipMemcpyAsync(h, q.ctrl, CTRL_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read counters");
     DTK_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
     *host = h;
-    if (h[CTRL_ERROR] & ERR_MASK_RANGE)
-      return fail(DTK_ERR_INVALID_ARGUMENT, "soft mask values must be in [0, 1]");
-    if (h[CTRL_ERROR] & ERR_WATCHDOG)
-      return fail(DTK_ERR_HIP, "device watchdog expired in the work queue");
-    return DTK_OK;
+    if (h[CTRL_ERROR] && w.mailbox)
+      *static_cast<volatile uint32_t*>(w.mailbox) = 0u;
+    return error_from_bits(h[CTRL_ERROR], "");
   }
 } // namespace
 
@@ -505,7 +689,9 @@ namespace
     Workspace* w = nullptr;
     if ((st = current_ws(&w, nullptr)) != DTK_OK)
       return st;
-    std::lock_guard<std::mutex> lock(w->mu);
+    WsUse use(*w, s);
+    if ((st = use.begin()) != DTK_OK)
+      return st;
     if ((st = ws_reserve(*w, 256 + sizeof(uint32_t) * kMinmaxPartialWords, 4096)) != DTK_OK)
       return st;
     uint32_t* keys    = static_cast<uint32_t*>(w->dbuf);
@@ -544,6 +730,8 @@ namespace
         return fail(DTK_ERR_INVALID_ARGUMENT, "gradient image sizes differ");
       if (H > 65535)
         return fail(DTK_ERR_UNSUPPORTED, "the adjoint takes at most 65535 rows");
+      if (n > 65535)
+        return fail(DTK_ERR_UNSUPPORTED, "the adjoint takes at most 65535 images per call");
       maxW = std::max(maxW, W);
       maxH = std::max(maxH, H);
       dbytes += align_up((size_t)W * H * 4, 256) + align_up((size_t)W * H * 8, 256) + align_up((size_t)W * H, 256);
@@ -553,14 +741,19 @@ namespace
     Workspace* w = nullptr;
     if ((st = current_ws(&w, nullptr)) != DTK_OK)
       return st;
-    std::lock_guard<std::mutex> lock(w->mu);
-    const size_t jbytes = align_up((size_t)n * sizeof(BackJob), 256);
-    if ((st = ws_reserve(*w, dbytes + jbytes + 1024, std::max<size_t>(4096, jbytes))) != DTK_OK)
-      return st;
     hipStream_t s = static_cast<hipStream_t>(stream);
+    WsUse       use(*w, s);
+    if ((st = use.begin()) != DTK_OK)
+      return st;
+    const size_t jbytes = align_up((size_t)n * sizeof(BackJob), 256);
+    if ((st = ws_reserve(*w, dbytes + jbytes + 1024, 4096)) != DTK_OK)
+      return st;
+    Stage* stg = nullptr;
+    if ((st = stage_get(*w, jbytes, &stg)) != DTK_OK)
+      return st;
     Carver      c(w->dbuf);
     BackJob*    djobs = c.take<BackJob>(n);
-    BackJob*    hjobs = static_cast<BackJob*>(w->hbuf);
+    BackJob*    hjobs = static_cast<BackJob*>(stg->h);
     const bool  soft  = masks[0].dtype == DTK_F32;
     for (int i = 0; i < n; ++i)
     {
@@ -582,8 +775,8 @@ namespace
       hjobs[i]       = j;
     }
     DTK_HIP(hipMemcpyAsync(djobs, hjobs, (size_t)n * sizeof(BackJob), hipMemcpyHostToDevice, s), "upload jobs");
-    // The staging buffer is reused by the next call: let the copy finish.
-    DTK_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if ((st = stage_done(stg, s)) != DTK_OK)
+      return st;
     const KParams kp = make_kparams(p, DTK_F32, masks[0].dtype, 32);
     DTK_HIP(launch_backward(djobs, n, maxW, maxH, kp, s), "launch backward");
     if (have_gm && !soft)
@@ -649,7 +842,9 @@ dtk_status dtk_ggdt(const dtk_image* img, const dtk_image* mask, dtk_image* dist
   Workspace* w = nullptr;
   if ((st = current_ws(&w, nullptr)) != DTK_OK)
     return st;
-  std::lock_guard<std::mutex> lock(w->mu);
+  WsUse use(*w, s);
+  if ((st = use.begin()) != DTK_OK)
+    return st;
   const bool c8  = p->connectivity == DTK_C8;
   const int  cm  = cost_mode(p);
   const int  S   = 64 / rh;
@@ -708,16 +903,17 @@ dtk_status dtk_ggdt(const dtk_image* img, const dtk_image* mask, dtk_image* dist
   if (p->schedule == DTK_SCHED_BLOCKS)
   {
     TaskBufs     tb;
-    const size_t head   = align_up(task_layout(nullptr, ntiles, 1, 1, nullptr) + 2 * align_up(ntiles, 256) + 512, 256);
+    const size_t head   = align_up(task_layout(nullptr, ntiles, 1, 1, ORDER_FIFO, nullptr) + 2 * align_up(ntiles, 256) + 512, 256);
     const size_t cbytes = use_cost_cache(p) ? place_costs(&b.d0, 1, c8, nullptr) : 0;
     if ((st = ws_reserve(*w, head + cbytes, 4096)) != DTK_OK)
       return st;
-    task_layout(w->dbuf, ntiles, 1, 1, &tb);
+    task_layout(w->dbuf, ntiles, 1, 1, ORDER_FIFO, &tb);
     if (cbytes)
       place_costs(&b.d0, 1, c8, static_cast<char*>(w->dbuf) + head);
     Queue q     = tb.q;
     q.slots     = nullptr; // blocks schedule: no queue
-    uint8_t* a0 = static_cast<uint8_t*>(w->dbuf) + align_up(task_layout(nullptr, ntiles, 1, 1, nullptr), 256);
+    q.bslots    = nullptr;
+    uint8_t* a0 = static_cast<uint8_t*>(w->dbuf) + align_up(task_layout(nullptr, ntiles, 1, 1, ORDER_FIFO, nullptr), 256);
     uint8_t* a1 = a0 + align_up(ntiles, 256);
     KParams  kb = kp;
     kb.max_sweeps = 0;
@@ -763,15 +959,17 @@ dtk_status dtk_ggdt(const dtk_image* img, const dtk_image* mask, dtk_image* dist
   // Task schedule.
   const int      workers = task_workers(*w, p, S, c8, cm, ntiles);
   const uint32_t cap     = next_pow2((uint32_t)(ntiles + workers + 1));
+  const int      order   = queue_order(p, 1);
   TaskBufs       tb;
-  const size_t   head   = align_up(task_layout(nullptr, ntiles, cap, 1, nullptr), 256);
+  const size_t   head   = align_up(task_layout(nullptr, ntiles, cap, 1, order, nullptr), 256);
   const size_t   cbytes = use_cost_cache(p) ? place_costs(&b.d0, 1, c8, nullptr) : 0;
   if ((st = ws_reserve(*w, head + cbytes, 4096)) != DTK_OK)
     return st;
-  task_layout(w->dbuf, ntiles, cap, 1, &tb);
+  task_layout(w->dbuf, ntiles, cap, 1, order, &tb);
   if (cbytes)
     place_costs(&b.d0, 1, c8, static_cast<char*>(w->dbuf) + head);
   tb.q.tile_visits = p->round_per_blocks;
+  tb.q.mailbox     = w->mailbox;
   if ((st = enqueue_task(b, kp, tb.q, S, c8, cm, workers, s)) != DTK_OK)
     return st;
   if (stats)
@@ -807,20 +1005,24 @@ dtk_status dtk_ggdt_batched(int n, const dtk_image* imgs, const dtk_image* masks
   Workspace* w = nullptr;
   if ((st = current_ws(&w, nullptr)) != DTK_OK)
     return st;
-  std::lock_guard<std::mutex> lock(w->mu);
-  hipStream_t    s   = static_cast<hipStream_t>(stream);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  WsUse       use(*w, s);
+  if ((st = use.begin()) != DTK_OK)
+    return st;
   const bool     c8  = p->connectivity == DTK_C8;
   const int      cm  = cost_mode(p);
   const int      S   = 64 / rh;
   const int      workers = task_workers(*w, p, S, c8, cm, ntiles);
   const uint32_t cap     = next_pow2((uint32_t)(ntiles + workers + 1));
+  const int      order   = queue_order(p, n);
   const size_t   hbytes  = descs.size() * sizeof(ImgDesc) + tile_img.size() * 4 + 1024;
-  const size_t   head    = align_up(task_layout(nullptr, ntiles, cap, n > 1 ? n : 2, nullptr), 256);
+  const size_t   head    = align_up(task_layout(nullptr, ntiles, cap, n > 1 ? n : 2, order, nullptr), 256);
   const size_t   cbytes  = use_cost_cache(p) ? place_costs(descs.data(), n, c8, nullptr) : 0;
-  if ((st = ws_reserve(*w, head + cbytes, hbytes)) != DTK_OK)
+  if ((st = ws_reserve(*w, head + cbytes, 4096)) != DTK_OK)
     return st;
   TaskBufs tb;
-  task_layout(w->dbuf, ntiles, cap, n > 1 ? n : 2, &tb);
+  task_layout(w->dbuf, ntiles, cap, n > 1 ? n : 2, order, &tb);
+  tb.q.mailbox = w->mailbox;
   if (cbytes)
     place_costs(descs.data(), n, c8, static_cast<char*>(w->dbuf) + head);
   Batch b{};
@@ -829,15 +1031,18 @@ dtk_status dtk_ggdt_batched(int n, const dtk_image* imgs, const dtk_image* masks
   b.ntiles = ntiles;
   if (n > 1)
   {
-    char* h = static_cast<char*>(w->hbuf);
+    Stage* stg = nullptr;
+    if ((st = stage_get(*w, hbytes, &stg)) != DTK_OK)
+      return st;
+    char* h = static_cast<char*>(stg->h);
     std::memcpy(h, descs.data(), descs.size() * sizeof(ImgDesc));
     std::memcpy(h + descs.size() * sizeof(ImgDesc), tile_img.data(), tile_img.size() * 4);
     DTK_HIP(hipMemcpyAsync(tb.table, h, descs.size() * sizeof(ImgDesc), hipMemcpyHostToDevice, s), "upload table");
     DTK_HIP(hipMemcpyAsync(tb.tile_img, h + descs.size() * sizeof(ImgDesc), tile_img.size() * 4,
                            hipMemcpyHostToDevice, s),
             "upload tile table");
-    // The staging buffer is reused by the next call: wait for the copies.
-    DTK_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if ((st = stage_done(stg, s)) != DTK_OK)
+      return st;
     b.table    = tb.table;
     b.tile_img = tb.tile_img;
   }
@@ -897,7 +1102,8 @@ dtk_status dtk_batch_plan_create(int n, const dtk_image* imgs, const dtk_image* 
   dtk_batch_plan* pl     = new dtk_batch_plan();
   pl->device             = dev;
   pl->ntiles             = ntiles;
-  const size_t head      = align_up(task_layout(nullptr, ntiles, cap, n > 1 ? n : 2, nullptr), 256);
+  const int    order     = queue_order(p, n);
+  const size_t head      = align_up(task_layout(nullptr, ntiles, cap, n > 1 ? n : 2, order, nullptr), 256);
   const size_t cbytes    = use_cost_cache(p) ? place_costs(descs.data(), n, c8, nullptr) : 0;
   hipError_t   e         = hipMalloc(&pl->dbuf, head + cbytes);
   if (e != hipSuccess)
@@ -906,7 +1112,8 @@ dtk_status dtk_batch_plan_create(int n, const dtk_image* imgs, const dtk_image* 
     return fail(DTK_ERR_OUT_OF_MEMORY, std::string("hipMalloc(plan): ") + hipGetErrorString(e));
   }
   TaskBufs tb;
-  task_layout(pl->dbuf, ntiles, cap, n > 1 ? n : 2, &tb);
+  task_layout(pl->dbuf, ntiles, cap, n > 1 ? n : 2, order, &tb);
+  tb.q.mailbox = w->mailbox;
   if (cbytes)
     place_costs(descs.data(), n, c8, static_cast<char*>(pl->dbuf) + head);
   Batch b{};
@@ -957,6 +1164,14 @@ dtk_status dtk_batch_plan_launch(dtk_batch_plan* plan, void* stream)
 {
   if (!plan || !plan->exec)
     return fail(DTK_ERR_INVALID_ARGUMENT, "invalid plan");
+  // Errors of earlier launches (this plan's or other calls' on its device).
+  if (plan->device >= 0 && plan->device < kMaxDevices)
+  {
+    Workspace& w = g_ws[plan->device];
+    dtk_status st;
+    if ((st = take_async_error(w)) != DTK_OK)
+      return st;
+  }
   DTK_HIP(hipGraphLaunch(plan->exec, static_cast<hipStream_t>(stream)), "hipGraphLaunch");
   return DTK_OK;
 }
@@ -1077,7 +1292,33 @@ dtk_status dtk_batch_autotune(int n, const dtk_image* imgs, const dtk_image* mas
   int        count  = 0;
   dtk_params dflt   = *p;
   dflt.tile_rows = dflt.tile_waves = dflt.waves_per_cu = dflt.max_sweeps = 0;
+  dflt.queue_order = DTK_ORDER_DEFAULT;
   const int drows = region_rows(&dflt), dnw = tile_waves(&dflt, drows);
+  // Queue orders first, at the default tile shape; the shapes are then
+  // timed under the fastest order.
+  {
+    float best_order_ms = 0.f;
+    int   best_order    = DTK_ORDER_KEY;
+    for (int order : {DTK_ORDER_KEY, DTK_ORDER_FIFO, DTK_ORDER_SEED})
+    {
+      dtk_params oc  = dflt;
+      oc.queue_order = order;
+      float ms       = 0.f;
+      if ((st = time_plan(n, imgs, masks, dists, &oc, s, reps, &ms)) != DTK_OK)
+        return st;
+      if (queue_order(&oc, n) == queue_order(&dflt, n))
+        default_ms = ms;
+      if (count == 0 || ms < best_order_ms)
+      {
+        best_order_ms = ms;
+        best_order    = order;
+        top           = oc;
+        top_ms        = ms;
+      }
+      ++count;
+    }
+    cand.queue_order = best_order;
+  }
   const int shapes[6][2] = {{16, 4}, {32, 4}, {32, 2}, {16, 2}, {32, 1}, {16, 1}}; // tile_rows, tile_waves
   for (const auto& shp : shapes)
   {
@@ -1105,13 +1346,11 @@ dtk_status dtk_batch_autotune(int n, const dtk_image* imgs, const dtk_image* mas
         float ms          = 0.f;
         if ((st = time_plan(n, imgs, masks, dists, &cand, s, reps, &ms)) != DTK_OK)
           return st;
-        if (count == 0 || ms < top_ms)
+        if (ms < top_ms)
         {
           top    = cand;
           top_ms = ms;
         }
-        if (rows == drows && nw == dnw && wi == 0 && cap == 0)
-          default_ms = ms;
         ++count;
       }
     }
@@ -1139,6 +1378,9 @@ dtk_status dtk_ggdt_pred(const dtk_image* img, const dtk_image* mask, const dtk_
     return st;
   if (pred->width != img->width || pred->height != img->height)
     return fail(DTK_ERR_INVALID_ARGUMENT, "pred size differs");
+  Workspace* w = nullptr;
+  if ((st = current_ws(&w, nullptr)) != DTK_OK || (st = take_async_error(*w)) != DTK_OK)
+    return st;
   ImgDesc d    = make_desc(img, mask, &dcopy, 32, 1);
   d.pred       = static_cast<unsigned char*>(pred->data);
   d.pred_pitch = pred->pitch;
@@ -1178,14 +1420,19 @@ dtk_status dtk_ggdt_pred_batched(int n, const dtk_image* imgs, const dtk_image* 
   Workspace* w = nullptr;
   if ((st = current_ws(&w, nullptr)) != DTK_OK)
     return st;
-  std::lock_guard<std::mutex> lock(w->mu);
-  const size_t tbytes = align_up((size_t)n * sizeof(ImgDesc), 256);
-  if ((st = ws_reserve(*w, tbytes + 1024, std::max<size_t>(4096, tbytes))) != DTK_OK)
-    return st;
   hipStream_t s = static_cast<hipStream_t>(stream);
+  WsUse       use(*w, s);
+  if ((st = use.begin()) != DTK_OK)
+    return st;
+  const size_t tbytes = align_up((size_t)n * sizeof(ImgDesc), 256);
+  if ((st = ws_reserve(*w, tbytes + 1024, 4096)) != DTK_OK)
+    return st;
+  Stage* stg = nullptr;
+  if ((st = stage_get(*w, tbytes, &stg)) != DTK_OK)
+    return st;
   Carver      c(w->dbuf);
   ImgDesc*    dtab = c.take<ImgDesc>(n);
-  ImgDesc*    htab = static_cast<ImgDesc*>(w->hbuf);
+  ImgDesc*    htab = static_cast<ImgDesc*>(stg->h);
   for (int i = 0; i < n; ++i)
   {
     dtk_image dc       = dists[i];
@@ -1194,8 +1441,8 @@ dtk_status dtk_ggdt_pred_batched(int n, const dtk_image* imgs, const dtk_image* 
     htab[i].pred_pitch = preds[i].pitch;
   }
   DTK_HIP(hipMemcpyAsync(dtab, htab, (size_t)n * sizeof(ImgDesc), hipMemcpyHostToDevice, s), "upload table");
-  // The staging buffer is reused by the next call: let the copy finish.
-  DTK_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
+  if ((st = stage_done(stg, s)) != DTK_OK)
+    return st;
   const KParams kp = make_kparams(p, imgs[0].dtype, masks[0].dtype, 32);
   DTK_HIP(launch_pred_batched(dtab, n, maxW, maxH, kp, p->connectivity == DTK_C8, s), "launch pred");
   return DTK_OK;
@@ -1347,7 +1594,10 @@ dtk_status dtk_block_priorities(const dtk_image* mask, uint8_t* priorities, uint
   Workspace*   w = nullptr;
   if ((st = current_ws(&w, nullptr)) != DTK_OK)
     return st;
-  std::lock_guard<std::mutex> lock(w->mu);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  WsUse       use(*w, s);
+  if ((st = use.begin()) != DTK_OK)
+    return st;
   const size_t dist_bytes = align_up(n * sizeof(uint32_t), 256);
   const size_t hist_bytes = align_up(((size_t)gw + gh) * sizeof(uint32_t), 256);
   if ((st = ws_reserve(*w, 256 + dist_bytes + hist_bytes, 4096)) != DTK_OK)
@@ -1356,7 +1606,6 @@ dtk_status dtk_block_priorities(const dtk_image* mask, uint8_t* priorities, uint
   uint32_t*   l0   = reinterpret_cast<uint32_t*>(base);
   uint32_t*   dist = reinterpret_cast<uint32_t*>(base + 256);
   uint32_t*   hist = reinterpret_cast<uint32_t*>(base + 256 + dist_bytes);
-  hipStream_t s    = static_cast<hipStream_t>(stream);
   DTK_HIP(launch_block_priorities(mask->data, mask->pitch, mask->dtype == DTK_F32, mask->width, mask->height, dist,
                                   hist, priorities, offsets, l0, s),
           "launch block priorities");
@@ -1432,6 +1681,51 @@ dtk_status dtk_colorize(const dtk_image* in, dtk_image* out, void* stream)
             "launch colorize");
     return DTK_OK;
   });
+}
+
+int dtk_tile_size_for(const dtk_params* p, int* tile_w, int* tile_h)
+{
+  if (!p || !tile_w || !tile_h || check_params(p) != DTK_OK)
+    return 0;
+  const int rh = region_rows(p), nw = tile_waves(p, rh);
+  *tile_w      = kTileW;
+  *tile_h      = nw * (rh - 2);
+  return 1;
+}
+
+dtk_status dtk_tile_image(const void* values, int value_bytes, int tiles_x, int tiles_y, int tile_w, int tile_h,
+                          dtk_image* out, void* stream)
+{
+  dtk_status st;
+  if ((st = check_image(out, "out", 1 << DTK_F32)) != DTK_OK)
+    return st;
+  if (value_bytes != 1 && value_bytes != 4)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "value_bytes must be 1 (uint8) or 4 (int32)");
+  if (tiles_x < 0 || tiles_y < 0 || tile_w <= 0 || tile_h <= 0)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "bad tile grid");
+  if ((long long)out->width * out->height == 0)
+    return DTK_OK;
+  if ((long long)tiles_x * tile_w < out->width || (long long)tiles_y * tile_h < out->height)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "the tile grid does not cover the image");
+  if (out->height > 4 * 65535)
+    return fail(DTK_ERR_UNSUPPORTED, "tile image: at most 262140 rows");
+  if (!values)
+    return fail(DTK_ERR_INVALID_ARGUMENT, "values is NULL");
+  DTK_HIP(launch_tile_image(values, value_bytes, tiles_x, tile_w, tile_h, out->data, out->pitch, out->width,
+                            out->height, static_cast<hipStream_t>(stream)),
+          "launch tile image");
+  return DTK_OK;
+}
+
+dtk_status dtk_sync_errors(void* stream)
+{
+  DTK_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "hipStreamSynchronize");
+  Workspace* w = nullptr;
+  dtk_status st;
+  if ((st = current_ws(&w, nullptr)) != DTK_OK)
+    return st;
+  std::lock_guard<std::mutex> lock(w->mu);
+  return take_async_error(*w);
 }
 
 dtk_status dtk_selftest(int* out3)

@@ -100,38 +100,55 @@ namespace dtk
                       // owns nw * (rh - 2) rows
   };
 
-  // Device work queue of dirty tiles (persistent kernel).
-  //   slots[CAP]: (seq << 32) | tile; seq = 2*ticket while free for that
-  //               ticket, 2*ticket + 1 once filled.
-  //   state[t]  : TileState of tile t.  A tile is in the ring at most once
-  //               and solved by at most one wave at a time; a wake-up while
-  //               it runs sets TILE_DIRTY and the running wave re-queues it
-  //               when it finishes.
-  //   ctrl      : counters, each on its own 64-byte line.
+  // Device work queue of dirty tiles (persistent kernel; the protocol is in
+  // dtk_queue.hpp).
+  //   slots[CAP]     : main ring, (seq << 32) | tile; seq = 2*ticket while
+  //                    free for that ticket, 2*ticket + 1 once filled.
+  //   bslots[B][CAP] : one such later ring per priority bucket (bucket orders).
+  //   state[t]       : TileState of tile t.
+  //   ctrl           : counters, each hot word on its own 128-byte line.
   enum CtrlWord : int
   {
-    CTRL_HEAD    = 0,
-    CTRL_TP      = 16, // 64-bit: low 32 bits = tail (tickets issued to pushers),
-                       //         high 32 bits = pending (tiles queued or running)
-    CTRL_DONE    = 32, // set once pending reaches 0 (idle workers poll it, not CTRL_TP)
-    CTRL_ERROR   = 48,
-    CTRL_VISITS  = 64,
-    CTRL_SWEEPS  = 80,
-    CTRL_CHANGED = 96,
-    CTRL_T_WAIT  = 100, // summed s_memtime cycles / 1024 of the task workers:
-    CTRL_T_VISIT = 101, //   waiting for a tile, solving tiles,
-    CTRL_T_PUB   = 102, //   publishing (release, wake-ups, re-queue)
-    CTRL_T_SETUP = 103, //   inside visits: loads + edge costs,
-    CTRL_T_SWEEP = 104, //   sweeps,
-    CTRL_T_STORE = 105, //   border check + write-back,
-    CTRL_T_REL   = 106, //   inside publishing: the release fence,
-    CTRL_T_ACQ   = 107, //   inside waiting: claiming a popped tile + acquire fence
-    CTRL_WORDS   = 128,
+    CTRL_HEAD    = 0,  // FIFO: tickets claimed by poppers
+    CTRL_TP      = 32, // FIFO, 64-bit: low 32 bits = tail (tickets issued to pushers),
+                       //              high 32 bits = pending (tiles queued or running)
+    CTRL_DONE    = 64, // set once pending reaches 0 (idle workers poll it)
+    CTRL_ERROR   = 96,
+    CTRL_VISITS  = 112,
+    CTRL_SWEEPS  = 113,
+    CTRL_CHANGED = 114,
+    CTRL_T_WAIT  = 116, // summed s_memtime cycles / 1024 of the task workers:
+    CTRL_T_VISIT = 117, //   waiting for a tile, solving tiles,
+    CTRL_T_PUB   = 118, //   publishing (release, wake-ups, re-queue)
+    CTRL_T_SETUP = 119, //   inside visits: loads + edge costs,
+    CTRL_T_SWEEP = 120, //   sweeps,
+    CTRL_T_STORE = 121, //   border check + write-back,
+    CTRL_T_REL   = 122, //   inside publishing: the release fence,
+    CTRL_T_ACQ   = 123, //   inside waiting: claiming a popped tile + acquire fence
+    CTRL_DRAINS  = 124, // bucket orders: drains that moved tiles,
+    CTRL_MOVED   = 125, //   tiles they moved,
+    CTRL_DEFER   = 126, //   tiles deferred into later rings,
+    CTRL_T_DRAIN = 127, //   summed cycles / 1024 spent draining (inside waiting)
+    CTRL_FRONT   = 128, // bucket orders: the front bucket (tiles beyond front + window are deferred)
+    CTRL_DRAIN   = 160, // bucket orders: 1 while a worker drains the later rings
+    CTRL_LHEAD   = 192, // bucket orders: per later ring, tickets drained (64 words)
+    CTRL_LTAIL   = 256, // bucket orders: per later ring, tickets issued to pushers (64 words)
+    CTRL_WORDS   = 320,
   };
 
-  // Tile states (one word per tile, changed only by read-modify-writes so
-  // every waker's D stores are in the release sequence the next visit
-  // acquires from):
+  // Priority buckets of the bucket orders (one later ring each; one wave
+  // lane per bucket when a drainer scans them).
+  constexpr uint32_t kBuckets = 64;
+
+  // How the queue orders dirty tiles (the result never depends on it).
+  enum QueueOrder : int
+  {
+    ORDER_FIFO = 0, // first queued first solved
+    ORDER_KEY  = 1, // deferral by the smallest improved value a wake carries
+    ORDER_SEED = 2, // deferral by a static priority: tile distance to the seeds
+  };
+
+  // Tile states (one word per tile, changed only by read-modify-writes):
   //   0                  idle
   //   TILE_DIRTY         queued (in the ring, not yet popped)
   //   TILE_RUNNING       being solved by one wave
@@ -145,13 +162,22 @@ namespace dtk
 
   struct Queue
   {
-    unsigned long long* slots;
+    unsigned long long* slots;  // main ring (kSlotStride words apart), or NULL
+    unsigned long long* bslots; // later rings (kBuckets x CAP words), or NULL
     uint32_t*           state;
     uint32_t*           ctrl;
     float*              tmin; // per tile min / max of the initial map (seed selection)
     float*              tmax;
+    uint32_t*           tkey;  // ORDER_KEY: per tile the smallest key of the wakes since its
+                               // last pop (float bits); ORDER_SEED: distance scratch
+    uint8_t*            tprio; // ORDER_SEED: per tile static priority (bucket)
     int*                tile_visits; // optional: visits per tile (instrumentation)
-    uint32_t            cap_mask;
+    uint32_t*           mailbox;     // pinned host word: error bits of the call, or NULL
+    uint32_t            cap_mask;    // CAP - 1 (CAP a power of two)
+    uint32_t            cap_log2;
+    int                 ntiles;
+    int                 order;  // QueueOrder
+    int                 window; // bucket orders: buckets above the front still queued in the main ring
     int                 pad_;
   };
 
@@ -159,12 +185,19 @@ namespace dtk
   {
     ERR_MASK_RANGE = 1u, // soft mask value outside [0, 1] or NaN
     ERR_WATCHDOG   = 2u, // a device-side wait exceeded kSpinLimit polls
+    ERR_QUEUE      = 4u, // a tile id or bucket read back from the queue was out of range
   };
 
   // Upper bound on the polls of any device-side wait (queue slots, raster
   // progress flags).  Legitimate waits end long before; an expired wait
   // flags ERR_WATCHDOG and the kernel finishes instead of hanging.
   constexpr uint32_t kSpinLimit = 1u << 26;
+
+  // Polls of a wait for a queue slot (a ring slot a pusher fills right after
+  // reserving its ticket, or frees right after reading it): such a wait
+  // ends within microseconds, and a poll is a memory round trip of about a
+  // microsecond, so this limit stops a faulty wait within a few seconds.
+  constexpr uint32_t kSlotWaitLimit = 1u << 20;
 
   // Ring slots are kSlotStride 8-byte words apart, so the slot an idle wave
   // polls shares its cache line with no other slot.

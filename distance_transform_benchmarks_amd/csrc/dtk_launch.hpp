@@ -10,7 +10,7 @@ namespace dtk
   // dtk_task.hip
   hipError_t launch_reset_ctrl(uint32_t* ctrl, hipStream_t s);
   hipError_t launch_init(const Batch& b, const KParams& p, const Queue& q, hipStream_t s);
-  hipError_t launch_seed(const Batch& b, const Queue& q, hipStream_t s);
+  hipError_t launch_seed(const Batch& b, const KParams& p, const Queue& q, hipStream_t s);
   hipError_t launch_blocks_seed(const Batch& b, const Queue& q, uint8_t* active, hipStream_t s);
   hipError_t launch_task(int S, bool c8, int cm, const Batch& b, const KParams& p, const Queue& q, int workers,
                          hipStream_t s);
@@ -37,6 +37,22 @@ namespace dtk
                                  hipStream_t s);
   hipError_t launch_backward(const BackJob* jobs, int n, int maxW, int maxH, const KParams& p, hipStream_t s);
 
+  // Grid of the adjoint's subtree-sum kernel for n images of at most maxW x
+  // maxH pixels (256-thread blocks, gx = ceil(maxW / 256) by maxH blocks per
+  // image).  From eight images on it is one-dimensional, image z on XCD
+  // z % 8: 8 * ceil(n / 8) * gx * maxH blocks; HIP bounds a dimension's
+  // blocks times threads below 2^32, so beyond that (and below eight
+  // images) the grid is gx x maxH x n.  Returns the x extent of the 1-D grid,
+  // or 0 for the 3-D grid.
+  inline unsigned long long subtree_grid_1d(int n, int maxW, int maxH)
+  {
+    if (n < 8 || maxW <= 0 || maxH <= 0)
+      return 0ull;
+    const unsigned long long gx = (unsigned long long)(maxW + 255) / 256ull, slots = (unsigned long long)(n + 7) / 8ull;
+    const unsigned long long blocks = 8ull * slots * gx * (unsigned long long)maxH;
+    return blocks * 256ull <= 0xffffffffull ? blocks : 0ull;
+  }
+
   // dtk_selftest.hip
   hipError_t launch_selftest(int* dpp_out, float* sqrt_out, hipStream_t s);
 
@@ -52,6 +68,9 @@ namespace dtk
   hipError_t launch_colorize(const void* in, long long in_pitch, void* out, long long out_pitch, int W, int H,
                              const uint32_t* keys, hipStream_t s);
   float      key_to_float(uint32_t k);
+  // Per-tile values (int32 or uint8) spread over their tiles' pixels (f32).
+  hipError_t launch_tile_image(const void* values, int value_bytes, int tiles_x, int tw, int th, void* out,
+                               long long out_pitch, int W, int H, hipStream_t s);
 
   // dtk_priority.hip: tile priorities of 32 x 32 tiles (gw x gh of them).
   // dist: gw * gh words; hist: gw + gh words; all device buffers.

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include "dtk_launch.hpp"
+#include "dtk_queue.hpp"
 #include "dtk_tile.hpp"
 
 namespace dtk
@@ -49,19 +50,9 @@ namespace dtk
     return v;
   }
 
-  // tail and pending share one 64-bit word, so one atomic both reserves
-  // tickets and counts the new items as pending.
-  constexpr unsigned long long kOnePendingOneTicket = (1ull << 32) | 1ull;
-  constexpr unsigned long long kMinusOnePending     = ~0ull << 32; // adds -1 to the high half
-
-  __device__ __forceinline__ unsigned long long* tp_word(const Queue& q)
-  {
-    return reinterpret_cast<unsigned long long*>(q.ctrl + CTRL_TP);
-  }
-  __device__ __forceinline__ uint32_t pending_now(const Queue& q)
-  {
-    return (uint32_t)(__hip_atomic_load(tp_word(q), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 32);
-  }
+  // Tiles queued or running (the kernel's first test: nothing seeded means
+  // no publish will ever raise DONE).
+  __device__ __forceinline__ uint32_t pending_now(const Queue& q) { return (uint32_t)(at_load(tp_word(q)) >> 32); }
   // k_init: one workgroup of kInitWaves waves per tile.  Each wave takes a
   // contiguous share of the tile's owned rows in groups of kInitRows rows
   // whose loads are all issued before the first use, so a tile costs a few
@@ -182,14 +173,21 @@ namespace dtk
       q.tmin[t] = mn;
       q.tmax[t] = mx;
     }
-    // Queue reset for the next launches of this call.
+    // Queue reset for the next launches of this call: every ring slot free
+    // for its first ticket, every tile idle with no pending key.
+    const uint32_t cap    = q.cap_mask + 1u;
+    const size_t   stride = (size_t)gridDim.x * blockDim.x;
     if (q.slots)
+      for (size_t k = (size_t)t * blockDim.x + threadIdx.x; k < cap; k += stride)
+        q.slots[k * kSlotStride] = (unsigned long long)(2u * (uint32_t)k) << 32;
+    if (q.bslots)
+      for (size_t k = (size_t)t * blockDim.x + threadIdx.x; k < (size_t)kBuckets * cap; k += stride)
+        q.bslots[k] = (unsigned long long)(2u * ((uint32_t)k & q.cap_mask)) << 32;
+    if (threadIdx.x == 0 && (q.slots || q.bslots))
     {
-      const uint32_t cap = q.cap_mask + 1u;
-      for (uint32_t k = (uint32_t)t * blockDim.x + threadIdx.x; k < cap; k += gridDim.x * blockDim.x)
-        q.slots[(size_t)k * kSlotStride] = (unsigned long long)(2u * k) << 32;
-      if (threadIdx.x == 0)
-        q.state[(size_t)t * kStateStride] = 0;
+      q.state[(size_t)t * kStateStride] = 0;
+      if (q.tkey)
+        q.tkey[t] = kInfBits;
     }
   }
 
@@ -203,7 +201,8 @@ namespace dtk
 
   // One thread per tile: a tile is initially dirty when some pixel of its
   // 3x3 tile neighbourhood starts below the largest initial value of the
-  // tile (otherwise no relaxation into the tile can succeed).
+  // tile (otherwise no relaxation into the tile can succeed).  Its key is
+  // that neighbourhood minimum.
   __global__ void k_seed(Batch b, Queue q, int ntiles)
   {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -220,56 +219,265 @@ namespace dtk
         continue;
       nmin = fminf(nmin, q.tmin[d.tile0 + ny * d.tiles_x + nx]);
     }
-    if (nmin < q.tmax[t])
+    if (!(nmin < q.tmax[t]))
+      return;
+    q.state[(size_t)t * kStateStride] = TILE_DIRTY; // queued
+    const uint32_t bucket = q.order == ORDER_KEY    ? key_bucket(__float_as_uint(nmin))
+                            : q.order == ORDER_SEED ? (uint32_t)q.tprio[t]
+                                                    : 0u;
+    if (q.tkey && q.order == ORDER_KEY)
+      q.tkey[t] = __float_as_uint(nmin);
+    // The front starts at bucket 0.
+    if (!deferred(q, bucket, 0u))
     {
-      q.state[(size_t)t * kStateStride] = TILE_DIRTY; // queued
-      const uint32_t ticket = (uint32_t)__hip_atomic_fetch_add(tp_word(q), kOnePendingOneTicket, __ATOMIC_RELAXED,
-                                                               __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t ticket = (uint32_t)at_add(tp_word(q), (unsigned long long)kOnePendingOneTicket);
       q.slots[(size_t)(ticket & q.cap_mask) * kSlotStride] = ((unsigned long long)(2u * ticket + 1u) << 32) | (uint32_t)t;
+      return;
+    }
+    at_add(tp_word(q), 1ull << 32); // pending only
+    later_push(q, bucket, (uint32_t)t, DeviceNap{});
+  }
+
+  // ORDER_SEED: static priority of every tile, one workgroup per image: the
+  // city-block distance in tiles to the nearest tile holding a seed (a pixel
+  // whose initial value is below v), then priority = cdf[d] * 64 / (N + 1)
+  // with cdf[d] the number of the image's N tiles at distance <= d.  The
+  // steps of dt/src/block_priority.cu:74-155 on this build's tiling
+  // (distances above kSeedPrioBins - 1 share the last histogram bin).
+  constexpr int      kSeedPrioBins = 4096;
+  constexpr uint32_t kSeedPrioFar  = 0x7fffffffu;
+  __global__ void __launch_bounds__(256) k_seedprio(Batch b, KParams p, Queue q)
+  {
+    __shared__ uint32_t hist[kSeedPrioBins];
+    __shared__ uint32_t s_max;
+    ImgDesc             d;
+    if (b.nimg == 1)
+      d = b.d0;
+    else
+    {
+      unsigned long long w[sizeof(ImgDesc) / 8];
+#pragma unroll
+      for (int k = 0; k < (int)(sizeof(ImgDesc) / 8); ++k)
+        w[k] = gload<unsigned long long>(b.table, (long long)blockIdx.x * (long long)sizeof(ImgDesc) + 8ll * k);
+      __builtin_memcpy(&d, w, sizeof(ImgDesc));
+    }
+    const int gw = d.tiles_x, gh = d.tiles_y, n = gw * gh;
+    if (n <= 0)
+      return;
+    const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
+    uint32_t* dist = q.tkey + d.tile0;
+    if (tid == 0)
+      s_max = 0u;
+    for (int i = tid; i < n; i += nt)
+      dist[i] = q.tmin[d.tile0 + i] < p.v ? 0u : kSeedPrioFar;
+    __syncthreads();
+    for (int y = tid; y < gh; y += nt)
+    {
+      uint32_t* r = dist + (size_t)y * gw;
+      for (int x = 1; x < gw; ++x)
+        r[x] = min(r[x - 1] + 1u, r[x]);
+      for (int x = gw - 2; x >= 0; --x)
+        r[x] = min(r[x + 1] + 1u, r[x]);
+    }
+    __syncthreads();
+    for (int x = tid; x < gw; x += nt)
+    {
+      for (int y = 1; y < gh; ++y)
+        dist[(size_t)y * gw + x] = min(dist[(size_t)(y - 1) * gw + x] + 1u, dist[(size_t)y * gw + x]);
+      for (int y = gh - 2; y >= 0; --y)
+        dist[(size_t)y * gw + x] = min(dist[(size_t)(y + 1) * gw + x] + 1u, dist[(size_t)y * gw + x]);
+    }
+    __syncthreads();
+    uint32_t m = 0u;
+    for (int i = tid; i < n; i += nt)
+      m = max(m, dist[i]);
+    atomicMax(&s_max, m);
+    __syncthreads();
+    const bool none = s_max >= kSeedPrioFar; // no seed tile in this image
+    const int  top  = none ? 0 : (int)min(s_max, (uint32_t)(kSeedPrioBins - 1));
+    for (int i = tid; i <= top; i += nt)
+      hist[i] = 0u;
+    __syncthreads();
+    if (!none)
+      for (int i = tid; i < n; i += nt)
+        atomicAdd(&hist[min(dist[i], (uint32_t)(kSeedPrioBins - 1))], 1u);
+    __syncthreads();
+    if (tid == 0 && !none)
+    {
+      uint32_t acc = 0u;
+      for (int i = 0; i <= top; ++i)
+      {
+        acc += hist[i];
+        hist[i] = acc;
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += nt)
+    {
+      const unsigned long long c = none ? (unsigned long long)n : hist[min(dist[i], (uint32_t)(kSeedPrioBins - 1))];
+      q.tprio[d.tile0 + i]       = (uint8_t)(c * 64ull / (unsigned long long)(n + 1));
     }
   }
 
-  // ---- queue operations (lane 0 of a wave) --------------------------------
-  // Pop in two steps.  Claim: take the next ticket.  Take: wait until the
-  // ticket's slot is filled or no work is pending; the popped tile goes from
-  // QUEUED to RUNNING.  A worker claims its next ticket when a visit starts.
+  // A worker leaving the persistent kernel copies the call's error bits (if
+  // any) into the host mailbox with a plain system-scope store; the host
+  // reports them at its next call on this device.  Every worker stores the
+  // word it reads; the host only needs it non-zero.
+  __device__ __forceinline__ void report_error(const Queue& q)
+  {
+    if (threadIdx.x != 0 || !q.mailbox)
+      return;
+    const uint32_t e = at_load(&q.ctrl[CTRL_ERROR]);
+    if (e)
+      __hip_atomic_store(q.mailbox, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+
+  // ---- queue operations of one wave -----------------------------------------
+  // Bucket orders: move the lowest non-empty later rings into the main ring
+  // (all lanes of the wave; the caller holds CTRL_DRAIN): the lowest
+  // non-empty bucket and the `window` buckets above it, then further
+  // non-empty buckets in order while fewer tiles than `want` (the claims
+  // waiting for a tile) have moved; the front moves to the last bucket
+  // drained.  Lane b reads later ring b's head and tail; then each lane moves
+  // one item per pass: it waits for the item's slot (its pusher fills it
+  // right after reserving the ticket), frees it, and fills a main-ring slot
+  // reserved for the whole pass with one atomic.  Items stay counted as
+  // pending.  Returns the number of tiles moved (the same in every lane).
+  static_assert(kBuckets == 64, "one wave lane per later ring");
+  __device__ uint32_t drain_later(const Queue& q, uint32_t want)
+  {
+    const int      lane = threadIdx.x & 63;
+    const uint32_t h    = at_load(&q.ctrl[CTRL_LHEAD + lane]);
+    const uint32_t t    = at_load(&q.ctrl[CTRL_LTAIL + lane]);
+    uint64_t       m    = __ballot((int)(t - h) > 0);
+    uint32_t       moved = 0;
+    if (m == 0)
+      return 0;
+    const int b0    = __builtin_ctzll(m);
+    const int first = min(b0 + q.window, (int)kBuckets - 1);
+    while (m)
+    {
+      const int b = __builtin_ctzll(m);
+      if (b > first && moved >= want)
+        break;
+      m &= m - 1ull;
+      if (lane == 0)
+        at_store(&q.ctrl[CTRL_FRONT], (uint32_t)b);
+      const uint32_t hb = (uint32_t)__shfl((int)h, b), tb = (uint32_t)__shfl((int)t, b);
+      for (uint32_t base = hb; (int)(tb - base) > 0; base += 64u)
+      {
+        const uint32_t n    = min(64u, tb - base);
+        int            tile = -1;
+        if ((uint32_t)lane < n)
+          for (uint32_t spin = 0;; ++spin)
+          {
+            tile = later_poll(q, (uint32_t)b, base + (uint32_t)lane);
+            if (tile != -2)
+              break;
+            if (spin >= kSlotWaitLimit)
+            {
+              at_or(&q.ctrl[CTRL_ERROR], (uint32_t)ERR_WATCHDOG);
+              tile = -1;
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+          }
+        const uint64_t ok  = __ballot(tile >= 0);
+        uint32_t       mb  = 0;
+        if (lane == 0)
+          mb = (uint32_t)at_add(tp_word(q), (unsigned long long)__popcll(ok)); // tickets only
+        mb = (uint32_t)__shfl((int)mb, 0);
+        if (tile >= 0)
+        {
+          const uint32_t ticket = mb + (uint32_t)__popcll(ok & ((1ull << lane) - 1ull));
+          ring_fill(q, &q.slots[(size_t)(ticket & q.cap_mask) * kSlotStride], ticket, (uint32_t)tile, DeviceNap{});
+        }
+        moved += (uint32_t)__popcll(ok);
+      }
+      if (lane == 0)
+        at_store(&q.ctrl[CTRL_LHEAD + b], tb);
+    }
+    return moved;
+  }
+
+  // Pop: wait until the claimed ticket's slot is filled, or no work is
+  // pending (-1), all lanes of one wave (the same result in every lane).
+  // The popped tile goes from queued to RUNNING; *key receives the key of
+  // its queued entry.  A worker claims its next ticket when a visit starts.
   // That reorders the work (a tile queued during the visit goes to this
   // worker after it), which cuts the visits of 4096^2 RGB8 at lambda 0.5 by
   // 9%, and the ticket atomic's latency overlaps the visit.  Each worker
-  // holds at most one claim, as before, so the ring bound holds.
-  __device__ uint32_t queue_claim(const Queue& q)
+  // holds at most one claim, so the ring bound (capacity > tiles + workers)
+  // holds.  Bucket orders: a waiter whose ticket is not yet issued to any
+  // pusher (the main ring is empty for it) drains the later rings: the
+  // first waiter in line at once, any other after about 16 us.
+  __device__ int pop(const Queue& q, uint32_t ticket, uint32_t* key, uint64_t& t_acq)
   {
-    return __hip_atomic_fetch_add(&q.ctrl[CTRL_HEAD], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-
-  __device__ int queue_take(const Queue& q, uint32_t ticket, uint64_t& t_acq)
-  {
-    unsigned long long* slot   = &q.slots[(size_t)(ticket & q.cap_mask) * kSlotStride];
-    const uint32_t      want   = 2u * ticket + 1u;
-    uint64_t            t_start = 0;
+    const int lane    = threadIdx.x & 63;
+    uint64_t  t_start = 0;
     for (uint32_t spin = 0;; ++spin)
     {
-      const unsigned long long s = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if ((uint32_t)(s >> 32) == want)
+      int t = -2;
+      if (lane == 0)
+        t = fifo_poll(q, ticket);
+      t = __shfl(t, 0);
+      if (t >= 0)
       {
-        const int      t  = (int)(uint32_t)s;
-        const uint64_t a0 = __builtin_amdgcn_s_memtime();
-        __hip_atomic_store(slot, (unsigned long long)(2u * (ticket + q.cap_mask + 1u)) << 32, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-        // queued -> RUNNING.  This RMW continues the release sequence of every
-        // earlier state RMW (the wakers' and the previous visit's), so the
-        // acquire fence makes their D stores visible to the loads of this visit.
-        __hip_atomic_exchange(&q.state[(size_t)t * kStateStride], (uint32_t)TILE_RUNNING, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        t_acq += __builtin_amdgcn_s_memtime() - a0;
+        uint32_t k = kInfBits;
+        if (lane == 0)
+        {
+          const uint64_t a0 = __builtin_amdgcn_s_memtime();
+          k                 = begin_visit(q, t);
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+          t_acq += __builtin_amdgcn_s_memtime() - a0;
+        }
+        *key = (uint32_t)__shfl((int)k, 0);
         return t;
       }
+      if (t == -1)
+        return -1;
       // Termination: the wave whose publish took pending to 0 raised DONE.
       // It is polled on its own line, every 8th poll, so idle waves do not
-      // contend with the publishers' atomics on CTRL_TP.
-      if ((spin & 7u) == 7u &&
-          __hip_atomic_load(&q.ctrl[CTRL_DONE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+      // contend with the publishers' atomics on CTRL_TP.  (It comes before
+      // the drain below, which would otherwise take every 8th poll.)
+      if ((spin & 7u) == 7u && is_done(q))
         return -1;
+      if (q.order != ORDER_FIFO && (spin & 3u) == 3u)
+      {
+        int drain = 0;
+        if (lane == 0)
+        {
+          const uint32_t tail = (uint32_t)at_load(tp_word(q));
+          if ((int)(tail - ticket) <= 0 && (tail == ticket || spin > 32u))
+            drain = drain_try_lock(q) ? 1 : 0;
+        }
+        if (__shfl(drain, 0))
+        {
+          // Every lane reads and writes later-ring heads: the whole wave
+          // takes part in the lock's acquire and release.
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+          const uint64_t d0 = __builtin_amdgcn_s_memtime();
+          // Claims not yet served: tickets taken beyond the main ring's tail.
+          uint32_t want = 0;
+          if (lane == 0)
+            want = at_load(&q.ctrl[CTRL_HEAD]) - (uint32_t)at_load(tp_word(q));
+          want                 = (uint32_t)__shfl((int)want, 0);
+          const uint32_t moved = drain_later(q, (int)want > 0 ? want : 1u);
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+          if (lane == 0)
+          {
+            drain_unlock(q);
+            if (moved)
+            {
+              at_add(&q.ctrl[CTRL_DRAINS], 1u);
+              at_add(&q.ctrl[CTRL_MOVED], moved);
+            }
+            at_add(&q.ctrl[CTRL_T_DRAIN], (uint32_t)((__builtin_amdgcn_s_memtime() - d0) >> 10));
+          }
+          if (moved)
+            continue; // poll the own slot again at once
+        }
+      }
       // Watchdog on the constant 100 MHz clock: an idle wave legitimately
       // waits as long as the whole solve runs, so only a wait far beyond any
       // real solve (kIdleLimitTicks) is taken as a hang.
@@ -279,12 +487,11 @@ namespace dtk
         if (spin == 0u)
           t_start = now;
         else if (now - t_start > kIdleLimitTicks)
-          spin = kSpinLimit;
-      }
-      if (spin >= kSpinLimit)
-      {
-        __hip_atomic_fetch_or(&q.ctrl[CTRL_ERROR], (uint32_t)ERR_WATCHDOG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return -1;
+        {
+          if (lane == 0)
+            at_or(&q.ctrl[CTRL_ERROR], (uint32_t)ERR_WATCHDOG);
+          return -1;
+        }
       }
       // Short backoff (at most kPollNaps x 512 cycles between polls of the own
       // slot): a tile queued on a serial chain is picked up within about 0.5 us.
@@ -296,86 +503,71 @@ namespace dtk
 
   // Publish a finished visit, wave-wide (called by all 64 lanes after a
   // release fence): lanes 0..7 wake the neighbours whose pixels this visit
-  // improved (fetch_or DIRTY: idle -> queued, running -> running+dirty),
-  // lane 8 moves the tile itself to idle or, when a wake-up arrived while it
-  // ran (or the visit hit its sweep cap), back to queued.  Tiles that became
-  // queued go into the ring with one reservation of tickets and pending
-  // count for all of them.
-  __device__ void publish(const Queue& q, const ImgDesc& d, int t, int tx, int ty, uint32_t border, bool requeue)
+  // improved, lane 8 ends the tile's own visit (idle, or queued again when
+  // woken meanwhile or stopped at its sweep cap).  Tiles that became queued
+  // are inserted: into the main ring, or (bucket orders) into the later ring
+  // of their bucket when it lies beyond the front read at the visit's start.
+  // One atomic reserves the main-ring tickets and counts every new item as
+  // pending, retiring this visit's item; the wave that takes pending to 0
+  // raises DONE.  Kw: the visit's wake keys per wave.
+  template <int NW>
+  __device__ void publish(const Queue& q, const ImgDesc& d, int t, int tx, int ty, uint32_t border, bool requeue,
+                          uint32_t own_key, uint32_t front, const uint32_t* Kw)
   {
-    const int lane   = threadIdx.x;
+    const int lane   = threadIdx.x & 63;
     int       target = -1;
+    bool      insert = false;
+    uint32_t  bucket = 0;
     if (lane < 8)
     {
       const int nx = tx + kNbrDX[lane], ny = ty + kNbrDY[lane];
       if (((border >> lane) & 1u) && nx >= 0 && ny >= 0 && nx < d.tiles_x && ny < d.tiles_y)
-        target = d.tile0 + ny * d.tiles_x + nx;
+      {
+        target      = d.tile0 + ny * d.tiles_x + nx;
+        uint32_t kb = Kw[lane];
+#pragma unroll
+        for (int w = 1; w < NW; ++w)
+          kb = min(kb, Kw[8 * w + lane]);
+        insert = wake_tile(q, target, kb, &bucket);
+      }
     }
     else if (lane == 8)
-      target = t;
-    bool insert = false;
-    if (target >= 0)
     {
-      if (lane == 8)
-      {
-        // RUNNING -> idle, or -> queued when woken meanwhile (DIRTY stays set)
-        // or when the visit hit its sweep cap.
-        const uint32_t old = __hip_atomic_fetch_and(&q.state[(size_t)target * kStateStride], ~(uint32_t)TILE_RUNNING, __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_AGENT);
-        if (old & TILE_DIRTY)
-          insert = true;
-        else if (requeue)
-          insert = __hip_atomic_fetch_or(&q.state[(size_t)target * kStateStride], (uint32_t)TILE_DIRTY, __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT) == 0u;
-      }
-      else
-        insert = __hip_atomic_fetch_or(&q.state[(size_t)target * kStateStride], (uint32_t)TILE_DIRTY, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT) == 0u;
+      target = t;
+      insert = end_visit(q, t, requeue, own_key, &bucket);
     }
-    const uint64_t ins  = __ballot(insert);
-    const uint32_t n    = (uint32_t)__popcll(ins);
-    uint32_t       base = 0;
+    const bool     later = insert && deferred(q, bucket, front);
+    const uint64_t all   = __ballot(insert);
+    const uint64_t mains = __ballot(insert && !later);
+    uint32_t       base  = 0;
     if (lane == 0)
     {
-      // One atomic: n tickets, n more pending items and this visit's item
-      // retired.  Pending stays >= n until the new items are popped and
-      // finished, and they become poppable only after their slots are
-      // written below.  The wave that takes pending to 0 raises DONE.
-      const unsigned long long old = __hip_atomic_fetch_add(
-        tp_word(q), (unsigned long long)n * kOnePendingOneTicket + kMinusOnePending, __ATOMIC_RELAXED,
-        __HIP_MEMORY_SCOPE_AGENT);
-      base = (uint32_t)old;
-      if (n == 0u && (uint32_t)(old >> 32) == 1u)
-        __hip_atomic_store(&q.ctrl[CTRL_DONE], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned long long old = publish_count(q, (uint32_t)__popcll(mains), (uint32_t)__popcll(all));
+      base                         = (uint32_t)old;
+      if (all == 0u && (uint32_t)(old >> 32) == 1u)
+        raise_done(q);
     }
-    base = __shfl(base, 0);
-    if (insert)
+    base = (uint32_t)__shfl((int)base, 0);
+    // No insertion is issued before that count has returned: the new items
+    // are pending before any of them can be popped.
+    __asm__ volatile("" ::: "memory");
+    if (insert && !later)
     {
-      const uint32_t      ticket   = base + (uint32_t)__popcll(ins & ((1ull << lane) - 1ull));
-      unsigned long long* slot     = &q.slots[(size_t)(ticket & q.cap_mask) * kSlotStride];
-      const uint32_t      free_seq = 2u * ticket;
-      const unsigned long long full = ((unsigned long long)(free_seq + 1u) << 32) | (uint32_t)target;
-      for (uint32_t spin = 0;; ++spin)
-      {
-        unsigned long long expect = (unsigned long long)free_seq << 32;
-        if (__hip_atomic_compare_exchange_strong(slot, &expect, full, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT))
-          break;
-        if (spin >= kSpinLimit)
-        {
-          __hip_atomic_fetch_or(&q.ctrl[CTRL_ERROR], (uint32_t)ERR_WATCHDOG, __ATOMIC_RELAXED,
-                                __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
+      const uint32_t ticket = base + (uint32_t)__popcll(mains & ((1ull << lane) - 1ull));
+      ring_fill(q, &q.slots[(size_t)(ticket & q.cap_mask) * kSlotStride], ticket, (uint32_t)target, DeviceNap{});
     }
+    if (later)
+      later_push(q, bucket, (uint32_t)target, DeviceNap{});
+    const uint64_t lat = __ballot(later);
+    if (lane == 0 && lat)
+      at_add(&q.ctrl[CTRL_DEFER], (uint32_t)__popcll(lat));
   }
 
 #ifndef DTK_TASK_WAVES_PER_EU_16
 #define DTK_TASK_WAVES_PER_EU_16 2
 #endif
   // One workgroup of NW waves per worker; NW stacked waves solve one tile.
+  // Wave 0 pops and publishes for the tile.
   template <int S, bool C8, int CM, int NW>
   __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(S == 4 ? DTK_TASK_WAVES_PER_EU_16 : 1)))
   k_task(Batch b, KParams p, Queue q)
@@ -384,22 +576,31 @@ namespace dtk
     __shared__ float    Sv[NW][2 * (64 / S)];
     __shared__ float    X[NW > 1 ? 2 * NW * 4 * 64 : 1];
     __shared__ uint32_t F[4 * NW];
+    __shared__ uint32_t Kw[8 * NW];
     __shared__ int      s_tile;
-    const int           w = NW > 1 ? (int)(threadIdx.x >> 6) : 0;
+    const int           w    = NW > 1 ? (int)(threadIdx.x >> 6) : 0;
+    const int           lane = threadIdx.x & 63;
     uint32_t         visits = 0, sweeps = 0;
     uint64_t         t_wait = 0, t_visit = 0, t_pub = 0, t_setup = 0, t_sweep = 0, t_store = 0;
     uint64_t         t_rel = 0, t_acq = 0;
     // Nothing seeded: no publish will ever raise DONE.
     if (pending_now(q) == 0u)
+    {
+      report_error(q);
       return;
-    uint32_t ticket = 0;
+    }
+    uint32_t ticket = 0, own_key = kInfBits, front = 0;
     if (threadIdx.x == 0)
-      ticket = queue_claim(q);
+      ticket = fifo_claim(q);
     for (;;)
     {
       const uint64_t c0 = __builtin_amdgcn_s_memtime();
-      if (threadIdx.x == 0)
-        s_tile = queue_take(q, ticket, t_acq);
+      if (w == 0)
+      {
+        const int t = pop(q, ticket, &own_key, t_acq);
+        if (lane == 0)
+          s_tile = t;
+      }
       __syncthreads();
       const int t = s_tile;
       __syncthreads();
@@ -408,14 +609,18 @@ namespace dtk
       if (t < 0)
         break;
       if (threadIdx.x == 0)
-        ticket = queue_claim(q);
+      {
+        ticket = fifo_claim(q);
+        if (q.order != ORDER_FIFO)
+          front = at_load(&q.ctrl[CTRL_FRONT]);
+      }
       ImgDesc d;
       int     tx, ty;
       resolve_tile(b, t, d, tx, ty);
-      const VisitResult vr = visit_tile<S, C8, CM, NW>(d, p, tx, ty, T[w], Sv[w], X, F);
+      const VisitResult vr = visit_tile<S, C8, CM, NW, true>(d, p, tx, ty, T[w], Sv[w], X, F, Kw);
       ++visits;
       if (q.tile_visits && threadIdx.x == 0)
-        __hip_atomic_fetch_add(&q.tile_visits[t], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        at_add(&q.tile_visits[t], 1);
       sweeps += (uint32_t)vr.sweeps;
       t_setup += vr.cyc_setup;
       t_sweep += vr.cyc_sweep;
@@ -430,26 +635,27 @@ namespace dtk
         if (vr.changed)
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         t_rel += __builtin_amdgcn_s_memtime() - c2;
-        publish(q, d, t, tx, ty, vr.border, !vr.converged);
+        publish<NW>(q, d, t, tx, ty, vr.border, !vr.converged, own_key, (uint32_t)__shfl((int)front, 0), Kw);
       }
       t_pub += __builtin_amdgcn_s_memtime() - c2;
     }
     if (threadIdx.x == 0 && visits)
     {
-      __hip_atomic_fetch_add(&q.ctrl[CTRL_VISITS], visits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&q.ctrl[CTRL_SWEEPS], sweeps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      at_add(&q.ctrl[CTRL_VISITS], visits);
+      at_add(&q.ctrl[CTRL_SWEEPS], sweeps);
     }
     if (threadIdx.x == 0)
     {
-      __hip_atomic_fetch_add(&q.ctrl[CTRL_T_WAIT], (uint32_t)(t_wait >> 10), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&q.ctrl[CTRL_T_VISIT], (uint32_t)(t_visit >> 10), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&q.ctrl[CTRL_T_PUB], (uint32_t)(t_pub >> 10), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&q.ctrl[CTRL_T_SETUP], (uint32_t)(t_setup >> 10), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&q.ctrl[CTRL_T_SWEEP], (uint32_t)(t_sweep >> 10), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&q.ctrl[CTRL_T_STORE], (uint32_t)(t_store >> 10), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&q.ctrl[CTRL_T_REL], (uint32_t)(t_rel >> 10), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&q.ctrl[CTRL_T_ACQ], (uint32_t)(t_acq >> 10), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      at_add(&q.ctrl[CTRL_T_WAIT], (uint32_t)(t_wait >> 10));
+      at_add(&q.ctrl[CTRL_T_VISIT], (uint32_t)(t_visit >> 10));
+      at_add(&q.ctrl[CTRL_T_PUB], (uint32_t)(t_pub >> 10));
+      at_add(&q.ctrl[CTRL_T_SETUP], (uint32_t)(t_setup >> 10));
+      at_add(&q.ctrl[CTRL_T_SWEEP], (uint32_t)(t_sweep >> 10));
+      at_add(&q.ctrl[CTRL_T_STORE], (uint32_t)(t_store >> 10));
+      at_add(&q.ctrl[CTRL_T_REL], (uint32_t)(t_rel >> 10));
+      at_add(&q.ctrl[CTRL_T_ACQ], (uint32_t)(t_acq >> 10));
     }
+    report_error(q);
   }
 
   // One host-driven round: every active tile solves to its local fixed point
@@ -577,8 +783,15 @@ namespace dtk
     return hipGetLastError();
   }
 
-  hipError_t launch_seed(const Batch& b, const Queue& q, hipStream_t s)
+  hipError_t launch_seed(const Batch& b, const KParams& p, const Queue& q, hipStream_t s)
   {
+    if (q.order == ORDER_SEED)
+    {
+      hipLaunchKernelGGL(k_seedprio, dim3(b.nimg), dim3(256), 0, s, b, p, q);
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess)
+        return e;
+    }
     hipLaunchKernelGGL(k_seed, dim3((b.ntiles + 255) / 256), dim3(256), 0, s, b, q, b.ntiles);
     return hipGetLastError();
   }

@@ -837,9 +837,16 @@ namespace dtk
   // CM: 0 = one add per edge with per-pixel costs (lambda 1, or lambda 0
   // with masked costs), 1 = two adds (0 < lambda < 1), 2 = lambda 0 with
   // ConstCosts.
-  template <int S, bool C8, int CM, int NW>
+  //
+  // KEYS: also report, per neighbour k (border-bit order), the smallest value
+  // of this visit's improved halo pixels that belong to neighbour k (float
+  // bits; +inf when none): Kw[8 * w + k] for wave w, written before the
+  // visit's last workgroup barrier.  The bucket orders queue a woken
+  // neighbour by it.
+  template <int S, bool C8, int CM, int NW, bool KEYS = false>
   __device__ __attribute__((always_inline)) VisitResult visit_tile(const ImgDesc& d, const KParams& p, int tx, int ty,
-                                                                   float* T, float* Sv, float* X, uint32_t* F)
+                                                                   float* T, float* Sv, float* X, uint32_t* F,
+                                                                   uint32_t* Kw = nullptr)
   {
     constexpr int RH   = 64 / S;
     constexpr int SW   = 64 / S;
@@ -1042,17 +1049,63 @@ namespace dtk
       // Only pixels inside the image count: with constant costs the pixels
       // outside it start at +inf on every visit and always decrease, which
       // would wake the neighbours (and through them this tile) forever.
-      bool colimp = false;
+      bool     colimp = false;
+      uint32_t colkey = 0x7f800000u; // smallest improved halo-column value (lanes 0 and 63)
       if ((lane == 0 || lane == 63) && cin)
       {
         const int o = (lane == 63) ? RH : 0;
 #pragma unroll
         for (int r = 1; r < RH - 1; ++r)
-          colimp |= Y0 + r < d.H && lt_nn(Dc[r], Sv[o + r]);
+        {
+          const bool imp = Y0 + r < d.H && lt_nn(Dc[r], Sv[o + r]);
+          colimp |= imp;
+          if constexpr (KEYS)
+            colkey = imp ? min(colkey, __float_as_uint(Dc[r])) : colkey;
+        }
       }
-      const uint64_t top  = ballot(cin && Y0 >= 0 && lt_nn(Dc[0], init_top));
-      const uint64_t bot  = ballot(cin && Y0 + RH - 1 < d.H && lt_nn(Dc[RH - 1], init_bot));
+      const bool     timp = cin && Y0 >= 0 && lt_nn(Dc[0], init_top);
+      const bool     bimp = cin && Y0 + RH - 1 < d.H && lt_nn(Dc[RH - 1], init_bot);
+      const uint64_t top  = ballot(timp);
+      const uint64_t bot  = ballot(bimp);
       const uint64_t cols = ballot(colimp);
+      if constexpr (KEYS)
+      {
+        // Per wave: L, R from the halo columns; T, TL, TR from row 0 of
+        // the top wave; B, BL, BR from the last row of the bottom wave.
+        const bool     mid_lane = lane >= 1 && lane <= 62;
+        uint32_t       tk = (timp && mid_lane) ? __float_as_uint(Dc[0]) : 0x7f800000u;
+        uint32_t       bk = (bimp && mid_lane) ? __float_as_uint(Dc[RH - 1]) : 0x7f800000u;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1)
+        {
+          tk = min(tk, (uint32_t)__shfl_xor((int)tk, o));
+          bk = min(bk, (uint32_t)__shfl_xor((int)bk, o));
+        }
+        const uint32_t c0  = (uint32_t)__shfl((int)colkey, 0), c63 = (uint32_t)__shfl((int)colkey, 63);
+        const uint32_t t0  = (uint32_t)__shfl((int)(timp ? __float_as_uint(Dc[0]) : 0x7f800000u), 0);
+        const uint32_t t63 = (uint32_t)__shfl((int)(timp ? __float_as_uint(Dc[0]) : 0x7f800000u), 63);
+        const uint32_t b0  = (uint32_t)__shfl((int)(bimp ? __float_as_uint(Dc[RH - 1]) : 0x7f800000u), 0);
+        const uint32_t b63 = (uint32_t)__shfl((int)(bimp ? __float_as_uint(Dc[RH - 1]) : 0x7f800000u), 63);
+        if (lane < 8)
+        {
+          uint32_t k = 0x7f800000u;
+          k          = lane == 0 ? c0 : k;
+          k          = lane == 1 ? c63 : k;
+          if (w == 0)
+          {
+            k = lane == 2 ? tk : k;
+            k = lane == 4 ? t0 : k;
+            k = lane == 5 ? t63 : k;
+          }
+          if (w == NW - 1)
+          {
+            k = lane == 3 ? bk : k;
+            k = lane == 6 ? b0 : k;
+            k = lane == 7 ? b63 : k;
+          }
+          Kw[8 * w + lane] = k;
+        }
+      }
       const uint64_t mid  = 0x7FFFFFFFFFFFFFFEull; // lanes 1..62
       b |= (uint32_t)(cols & 1u);                 // L : left halo column
       b |= (uint32_t)((cols >> 63) & 1u) << 1;    // R : right halo column
@@ -1083,8 +1136,17 @@ namespace dtk
         }
       }
     }
+    if constexpr (KEYS)
+    {
+      if (!any && lane < 8)
+        Kw[8 * w + lane] = 0x7f800000u;
+    }
     if constexpr (NW == 1)
+    {
       res.border = b;
+      if constexpr (KEYS)
+        lds_wave_sync();
+    }
     else
     {
       // Border bits and change flags of all waves (F[2*NW ..]).

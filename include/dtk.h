@@ -90,9 +90,32 @@ typedef struct
                             (16 B/px C8, 8 B/px C4) that every visit reads */
   int   tile_waves;      /* task schedule: waves stacked per tile, 1, 2 or 4; a tile owns 62 x tile_waves *
                             (tile_rows - 2) pixels.  0 = library default (4 with 16-row, 2 with 32-row waves) */
+  int   queue_order;     /* task schedule: order in which dirty tiles are solved (the result never depends
+                            on it): DTK_ORDER_DEFAULT, DTK_ORDER_FIFO, DTK_ORDER_KEY or DTK_ORDER_SEED */
   int*  round_per_blocks; /* optional device int[dtk_tile_count_for()]: visits per tile (blocks and task
                             schedules, single-image calls), added to the existing counts */
 } dtk_params;
+
+/* Queue orders of the task schedule (dtk_params.queue_order).
+ *   FIFO  one ring: tiles are solved in the order they became dirty;
+ *   KEY   64 priority buckets by the smallest improved value a wake carries
+ *         (two buckets per octave of the distance): tiles near the current
+ *         front of final values go first, which cuts re-visits;
+ *   SEED  64 priority buckets by a static tile priority: the city-block
+ *         tile distance to the nearest seed tile, quantised by its cdf, the
+ *         priority of the reference's task queue (dt/src/block_priority.cu:74-155,
+ *         dt/src/task_queue.cuh:40-63) on this library's tiling.
+ * KEY and SEED defer a woken tile whose bucket lies beyond the current
+ * front into a later ring; a worker that finds the main ring empty moves
+ * the lowest later buckets into it.  DEFAULT is KEY for batched calls and
+ * plans (n > 1), FIFO for one image. */
+typedef enum
+{
+  DTK_ORDER_DEFAULT = 0,
+  DTK_ORDER_FIFO    = 1,
+  DTK_ORDER_KEY     = 2,
+  DTK_ORDER_SEED    = 3
+} dtk_queue_order;
 
 /* Filled by calls that report statistics (may be NULL). */
 typedef struct
@@ -111,6 +134,10 @@ typedef struct
   uint32_t cycles_store; /*   inside visits: border check and write-back */
   uint32_t cycles_release; /* inside publishing: the release fence after a changing visit */
   uint32_t cycles_acquire; /* inside waiting: claiming the popped tile and the acquire fence */
+  uint32_t drains;       /* task, KEY / SEED orders: drains of the later rings that moved tiles */
+  uint32_t moved;        /*   tiles those drains moved into the main ring */
+  uint32_t deferred;     /*   wakes deferred into a later ring (seeds excluded) */
+  uint32_t cycles_drain; /*   summed cycles / 1024 of the drains (inside waiting) */
 } dtk_stats;
 
 void dtk_params_default(dtk_params* p);
@@ -146,14 +173,15 @@ dtk_status dtk_batch_plan_create(int n, const dtk_image* imgs, const dtk_image* 
 dtk_status dtk_batch_plan_launch(dtk_batch_plan* plan, void* stream);
 dtk_status dtk_batch_plan_destroy(dtk_batch_plan* plan);
 
-/* Autotuner of batched plans.  Times candidate tile shapes (16-row waves
- * stacked 4, 2 or 1 high, 32-row waves stacked 2 or 1 high), sweep caps per
- * visit (max_sweeps 0 /
- * 12) and persistent waves per CU (the occupancy limit, 3/4 and 1/2 of it)
- * on the given batch: each
+/* Autotuner of batched plans.  First times the queue orders (KEY, FIFO,
+ * SEED) at the default tile shape, then, under the fastest order, candidate
+ * tile shapes (16-row waves stacked 4, 2 or 1 high, 32-row waves stacked 2
+ * or 1 high), sweep caps per visit (max_sweeps 0 / 12) and persistent waves
+ * per CU (the occupancy limit, 3/4 and 1/2 of it) on the given batch: each
  * candidate is a captured plan, launched once to warm up and then `reps`
  * times between HIP events on `stream`.  Writes *p with the fastest
- * tile_rows / tile_waves / max_sweeps / waves_per_cu into *best.  Every candidate
+ * queue_order / tile_rows / tile_waves / max_sweeps / waves_per_cu into
+ * *best (default_ms: the library defaults).  Every candidate
  * computes the same bit-exact maps (the fixed point does not depend on the
  * schedule), so tuning never changes results; dists are overwritten.
  * Results are memoised per process and device by a signature of the batch
@@ -237,6 +265,19 @@ dtk_status dtk_copy2d(void* dst, long long dst_pitch, const void* src, long long
 dtk_status dtk_profile_begin(void);
 dtk_status dtk_profile_end(double* setup_ms, double* solve_ms, int* calls);
 
+/* Asynchronous errors.  The task schedule (dtk_ggdt, dtk_ggdt_batched,
+ * batch plans) returns before its kernels finish, so a device-side error
+ * (a soft mask value outside [0, 1] or NaN, a queue watchdog) cannot be
+ * its return value.  The kernels leave it in a per-device mailbox that
+ * every later call on the device reads first (a host read, no wait when
+ * there is no error): that call then synchronises the device and returns
+ * the error, once, with "(an earlier asynchronous call on this device)" in
+ * its message.  dtk_sync_errors waits for `stream` and returns any such
+ * error now.  Calls with a non-NULL stats pointer synchronise and return
+ * their own error directly.  (The reference reports CUDA errors after a
+ * synchronous call, dt/src/generalised_distance_transform.cu:136-137.) */
+dtk_status dtk_sync_errors(void* stream);
+
 /* Device self-check of the assumptions the kernels rely on: DPP lane
  * direction of wave shifts and correctly rounded sqrtf over every RGB8
  * squared distance.  out[0] = DPP ok, out[1] = number of sqrtf mismatches
@@ -263,6 +304,22 @@ dtk_status dtk_minmax_f32(const dtk_image* img, float* min_max, void* stream);
 dtk_status dtk_normalize_u8(const dtk_image* in, dtk_image* out, void* stream);
 dtk_status dtk_inferno(const dtk_image* in, dtk_image* out, void* stream);
 dtk_status dtk_colorize(const dtk_image* in, dtk_image* out, void* stream);
+
+/* Tile size of a call with parameters *p (tile schedules): tiles are
+ * tile_w x tile_h owned pixels, row-major, ceil(W / tile_w) a row; the
+ * layout of round_per_blocks.  Returns 0 for invalid parameters. */
+int dtk_tile_size_for(const dtk_params* p, int* tile_w, int* tile_h);
+
+/* Per-tile values spread over their tiles' pixels: out(x, y) =
+ * values[(y / tile_h) * tiles_x + x / tile_w] as float, for `values` a
+ * device array of tiles_x * tiles_y int32 (value_bytes 4, e.g. visit counts)
+ * or uint8 (value_bytes 1, e.g. tile priorities) and `out` a DTK_F32 device
+ * image that the tile grid covers.  With dtk_colorize this gives the
+ * reference's heat maps of per-block rounds and priorities: its
+ * block_attribute_to_image kernel (bin/rounds_per_blocks.cu:13-23,
+ * bin/priority_block.cu:12-22), for any tile size.  Asynchronous. */
+dtk_status dtk_tile_image(const void* values, int value_bytes, int tiles_x, int tiles_y, int tile_w, int tile_h,
+                          dtk_image* out, void* stream);
 
 /* Tile priorities of the reference's task queue:
  * dt::compute_block_priorities (dt/include/dt/block_priority.hpp:7-8,

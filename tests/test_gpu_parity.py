@@ -120,6 +120,103 @@ def test_random_vs_oracle(dt, dev, oracle, case, sched):
         assert_bit_exact(got, want, f"{case} {sched} {kw}")
 
 
+@pytest.mark.parametrize("order", ["fifo", "key", "seed"])
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "-".join(map(str, c)))
+def test_queue_orders_vs_oracle(dt, dev, oracle, case, order):
+    """Every queue order of the task schedule (FIFO ring, buckets by wake
+    key, buckets by seed-distance priority) under every tile shape gives the
+    oracle's map bit for bit: the order only changes which tile is solved
+    when."""
+    ik, mk, W, H, conn, lam = case
+    seed = zlib.crc32(repr(case).encode()) & 0xFFFF
+    img, m = image(ik, W, H, seed), mask(mk, W, H, seed)
+    v = 1e10 if mk == "u8" else 1e3
+    want, _ = oracle.raster(img, m, lam, v, conn8=conn == 8)
+    for kw in _variants("task"):
+        _say(case, order, kw)
+        got = _run(dt, dev, img, m, lam, v, "task", connectivity=conn, queue_order=order, **kw)
+        assert_bit_exact(got, want, f"{case} {order} {kw}")
+
+
+@pytest.mark.parametrize("order", ["fifo", "key", "seed"])
+@pytest.mark.parametrize("lam", [0.0, 0.5, 1.0])
+def test_queue_orders_large(dt, dev, oracle, order, lam):
+    """Images large enough that many tiles are queued at once and re-visited
+    (1100 x 900 RGB8, soft mask of 40 discs, C8): bit-exact under every
+    order, and the visit statistics are consistent."""
+    from distance_transform_benchmarks_amd import synth
+
+    rng = synth.SplitMix64(77)
+    img = np.ascontiguousarray(np.stack([synth.to_u8(synth.blob_field(rng, 1100, 900)) for _ in range(3)], -1))
+    m = synth.soft_discs(rng, 1100, 900, 40)
+    want, _ = oracle.raster(img, m, lam, 1e3, conn8=True, threads=8)
+    _say("large", order, lam)
+    import torch
+
+    D, st = dt.ggdt(_to_dev(img, dev), _to_dev(m, dev), lam, 1e3, queue_order=order, stats=True)
+    torch.cuda.synchronize()
+    assert_bit_exact(D.cpu().numpy(), want, f"large {order} {lam}")
+    assert st["visits"] >= 1 and st["tiles"] > 0 and st["sweeps"] >= 4 * st["visits"]
+
+
+@pytest.mark.parametrize("order", ["fifo", "key", "seed"])
+def test_queue_orders_batched(dt, dev, oracle, order):
+    """A batch of uneven images shares one queue under every order (the seed
+    order computes each image's priorities on its own tile grid)."""
+    import torch
+
+    shapes = [(300, 200), (64, 64), (513, 129), (1, 1), (90, 700), (200, 300)]
+    imgs, masks, wants = [], [], []
+    for i, (W, H) in enumerate(shapes):
+        img, m = image("u8", W, H, 500 + i), mask("u8", W, H, 500 + i)
+        imgs.append(_to_dev(img, dev))
+        masks.append(_to_dev(m, dev))
+        wants.append(oracle.raster(img, m, 0.8, 1e10, conn8=True)[0])
+    _say("batched orders", order)
+    outs = dt.ggdt_batched(imgs, masks, 0.8, 1e10, queue_order=order)
+    torch.cuda.synchronize()
+    for i, (o, w) in enumerate(zip(outs, wants)):
+        assert_bit_exact(o.cpu().numpy(), w, f"batched {order} image {i}")
+
+
+def test_batch256_vs_oracle(dt, dev, oracle):
+    """The batched workload (256 uint8 blob images, sides log-uniform in
+    [64, 2048] with aspect in [0.5, 2], 1-16 seed discs each, C8, lambda =
+    0.8) through the autotuned graph plan and through ggdt_batched: a seeded
+    subset of 16 images, the three largest and the three smallest among
+    them, is bit-exact against the oracle, and the two launches agree on
+    every image.  Per-image semantics as the reference task schedule
+    (dt/src/generalised_distance_transform_task.cu:152-197)."""
+    import torch
+
+    from distance_transform_benchmarks_amd import synth
+
+    _say("batch256: generating")
+    data = synth.batch256()
+    imgs = [_to_dev(i, dev) for i, _ in data]
+    masks = [_to_dev(m, dev) for _, m in data]
+    _say("batch256: autotuned plan")
+    plan = dt.BatchPlan(imgs, masks, 0.8, 1e10, tune=True)
+    try:
+        plan.launch()
+        _say("batch256: ggdt_batched", plan.tuning)
+        outs = dt.ggdt_batched(imgs, masks, 0.8, 1e10)
+        torch.cuda.synchronize()
+        for i in range(len(data)):
+            assert torch.equal(plan.outs[i].view(torch.int32), outs[i].view(torch.int32)), f"image {i}"
+        px = np.array([m.size for _, m in data])
+        by_size = np.argsort(px)
+        rng = np.random.default_rng(256)
+        pick = sorted(set(by_size[:3]) | set(by_size[-3:]) | set(rng.choice(len(data), 10, replace=False)))
+        for i in pick:
+            img, m = data[i]
+            _say("batch256 oracle image", i, m.shape)
+            want, _ = oracle.raster(img, m, 0.8, 1e10, conn8=True, threads=16)
+            assert_bit_exact(plan.outs[i].cpu().numpy(), want, f"batch256 image {i} {m.shape}")
+    finally:
+        plan.close()
+
+
 @pytest.mark.parametrize("conn", [8, 4])
 @pytest.mark.parametrize("lam", [0.0, 0.5, 1.0])
 @pytest.mark.parametrize("sched", ["raster", "blocks", "task"])
