@@ -1720,12 +1720,18 @@ dtk_status dtk_tile_image(const void* values, int value_bytes, int tiles_x, int 
 dtk_status dtk_sync_errors(void* stream)
 {
   DTK_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "hipStreamSynchronize");
-  Workspace* w = nullptr;
-  dtk_status st;
-  if ((st = current_ws(&w, nullptr)) != DTK_OK)
-    return st;
-  std::lock_guard<std::mutex> lock(w->mu);
-  return take_async_error(*w);
+  int dev = 0;
+  DTK_HIP(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(DTK_ERR_UNSUPPORTED, "device index out of range");
+  dtk_status st = DTK_OK;
+  Workspace& w  = g_ws[dev];
+  if (w.mailbox)
+  {
+    std::lock_guard<std::mutex> lock(w.mu);
+    st = take_async_error(w);
+  }
+  return st;
 }
 
 dtk_status dtk_selftest(int* out3)

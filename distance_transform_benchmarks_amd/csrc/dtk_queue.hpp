@@ -173,28 +173,34 @@ namespace dtk
     return &q.bslots[((size_t)b << q.cap_log2) + (ticket & q.cap_mask)];
   }
 
-  // Reads the tile of ticket `ticket` of later ring b once its slot is
-  // filled (its pusher fills it right after reserving the ticket) and frees
-  // the slot for ticket + capacity.  -2: not filled yet; -1: protocol fault.
-  DTK_HD int later_poll(const Queue& q, uint32_t b, uint32_t ticket)
+  // Given the word `s` just read from the slot of ticket `ticket` of later
+  // ring b: when it is filled (its pusher fills it right after reserving the
+  // ticket) frees the slot for ticket + capacity and returns the tile.
+  // -2: not filled yet; -1: protocol fault.  (Split from the load so that a
+  // drainer can issue the loads of several slots together.)
+  DTK_HD int later_take(const Queue& q, uint32_t b, uint32_t ticket, unsigned long long s)
   {
     if (b >= kBuckets)
     {
       at_or(&q.ctrl[CTRL_ERROR], (uint32_t)ERR_QUEUE);
       return -1;
     }
-    unsigned long long*      slot = later_slot(q, b, ticket);
-    const unsigned long long s    = at_load(slot);
     if ((uint32_t)(s >> 32) != 2u * ticket + 1u)
       return -2;
     const uint32_t t = (uint32_t)s;
-    at_store(slot, (unsigned long long)(2u * (ticket + q.cap_mask + 1u)) << 32);
+    at_store(later_slot(q, b, ticket), (unsigned long long)(2u * (ticket + q.cap_mask + 1u)) << 32);
     if (t >= (uint32_t)q.ntiles)
     {
       at_or(&q.ctrl[CTRL_ERROR], (uint32_t)ERR_QUEUE);
       return -1;
     }
     return (int)t;
+  }
+  DTK_HD int later_poll(const Queue& q, uint32_t b, uint32_t ticket)
+  {
+    if (b >= kBuckets)
+      return later_take(q, b, ticket, 0ull);
+    return later_take(q, b, ticket, at_load(later_slot(q, b, ticket)));
   }
 
   // Defers a tile (already counted as pending) into later ring b.
@@ -276,6 +282,9 @@ namespace dtk
       ins = true;
     else if (requeue)
       ins = at_or(st, (uint32_t)TILE_DIRTY) == 0u;
+    *bucket = 0;
+    if (!ins)
+      return false;
     if (q.order == ORDER_KEY)
     {
       const uint32_t k = at_load(&q.tkey[t]);
@@ -283,9 +292,7 @@ namespace dtk
     }
     else if (q.order == ORDER_SEED)
       *bucket = q.tprio[t];
-    else
-      *bucket = 0;
-    return ins;
+    return true;
   }
 
   // The one atomic of a publish: n_main tickets of the main ring, n_all new

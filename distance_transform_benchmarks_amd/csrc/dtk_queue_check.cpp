@@ -220,6 +220,35 @@ namespace
     return moved;
   }
 
+  // The kernel's fill_main for one item: fill the main-ring slot of
+  // `ticket`; while it is still held one lap back, take this worker's own
+  // claimed tile as soon as it is filled (*pre).
+  void fill_main_host(const Queue& q, uint32_t ticket, uint32_t tile, uint32_t claim, int* pre)
+  {
+    unsigned long long*      slot = &q.slots[(size_t)(ticket & q.cap_mask) * kSlotStride];
+    const unsigned long long full = ((unsigned long long)(2u * ticket + 1u) << 32) | tile;
+    const Deadline           dl;
+    for (uint32_t spin = 0;; ++spin)
+    {
+      unsigned long long expect = (unsigned long long)(2u * ticket) << 32;
+      if (at_cas(slot, expect, full))
+        return;
+      if (*pre < 0)
+      {
+        const int t = fifo_poll(q, claim);
+        if (t >= 0)
+          *pre = t;
+      }
+      if (dl.expired(spin))
+      {
+        at_or(&q.ctrl[CTRL_ERROR], (uint32_t)ERR_WATCHDOG);
+        return;
+      }
+      if (spin > 64)
+        std::this_thread::yield();
+    }
+  }
+
   void worker(Model& m, uint32_t seed)
   {
     const Queue& q = m.q;
@@ -227,6 +256,7 @@ namespace
     if (at_load(tp_word(q)) >> 32 == 0)
       return;
     uint32_t ticket = fifo_claim(q);
+    int      pre    = -1;
     for (;;)
     {
       // ---- pop (the kernel's pop, with the same poll cadences)
@@ -234,7 +264,8 @@ namespace
       const Deadline dl;
       for (uint32_t spin = 0;; ++spin)
       {
-        t = fifo_poll(q, ticket);
+        t   = pre >= 0 ? pre : fifo_poll(q, ticket);
+        pre = -1;
         if (t != -2)
           break;
         if ((spin & 7u) == 7u && is_done(q))
@@ -294,6 +325,8 @@ namespace
       }
       if ((rng() % 4u) == 0u)
         std::this_thread::yield();
+      if ((rng() % 64u) == 0u) // a slow visit: the other workers run laps of the ring meanwhile
+        std::this_thread::sleep_for(std::chrono::microseconds(200));
       m.running[t].fetch_sub(1);
       // ---- publish (lanes 0..nt-1 wake, lane nt ends the visit)
       bool     insert[9], later[9];
@@ -320,7 +353,7 @@ namespace
       {
         if (insert[i] && !later[i])
         {
-          ring_fill(q, &q.slots[(size_t)(tk & q.cap_mask) * kSlotStride], tk, (uint32_t)targets[i], HostNap{});
+          fill_main_host(q, tk, (uint32_t)targets[i], ticket, &pre);
           ++tk;
         }
         if (later[i])

@@ -364,35 +364,63 @@ namespace dtk
       if (lane == 0)
         at_store(&q.ctrl[CTRL_FRONT], (uint32_t)b);
       const uint32_t hb = (uint32_t)__shfl((int)h, b), tb = (uint32_t)__shfl((int)t, b);
-      for (uint32_t base = hb; (int)(tb - base) > 0; base += 64u)
+      // Passes of up to kPer * 64 items, kPer per lane, their slot loads all
+      // in flight together.
+      constexpr int kPer = 4;
+      for (uint32_t base = hb; (int)(tb - base) > 0; base += 64u * kPer)
       {
-        const uint32_t n    = min(64u, tb - base);
-        int            tile = -1;
-        if ((uint32_t)lane < n)
-          for (uint32_t spin = 0;; ++spin)
-          {
-            tile = later_poll(q, (uint32_t)b, base + (uint32_t)lane);
-            if (tile != -2)
-              break;
-            if (spin >= kSlotWaitLimit)
+        const uint32_t n = min(64u * kPer, tb - base);
+        unsigned long long word[kPer];
+        int                tile[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+          word[k] = (uint32_t)(lane + 64 * k) < n ? at_load(later_slot(q, (uint32_t)b, base + (uint32_t)(lane + 64 * k)))
+                                                  : 0ull;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+          tile[k] = (uint32_t)(lane + 64 * k) < n ? later_take(q, (uint32_t)b, base + (uint32_t)(lane + 64 * k), word[k])
+                                                  : -1;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+          if (tile[k] == -2) // not filled yet: its pusher is about to
+            for (uint32_t spin = 0;; ++spin)
             {
-              at_or(&q.ctrl[CTRL_ERROR], (uint32_t)ERR_WATCHDOG);
-              tile = -1;
-              break;
+              __builtin_amdgcn_s_sleep(1);
+              tile[k] = later_poll(q, (uint32_t)b, base + (uint32_t)(lane + 64 * k));
+              if (tile[k] != -2)
+                break;
+              if (spin >= kSlotWaitLimit)
+              {
+                at_or(&q.ctrl[CTRL_ERROR], (uint32_t)ERR_WATCHDOG);
+                tile[k] = -1;
+                break;
+              }
             }
-            __builtin_amdgcn_s_sleep(1);
-          }
-        const uint64_t ok  = __ballot(tile >= 0);
-        uint32_t       mb  = 0;
-        if (lane == 0)
-          mb = (uint32_t)at_add(tp_word(q), (unsigned long long)__popcll(ok)); // tickets only
-        mb = (uint32_t)__shfl((int)mb, 0);
-        if (tile >= 0)
+        uint64_t ok[kPer];
+        uint32_t total = 0;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
         {
-          const uint32_t ticket = mb + (uint32_t)__popcll(ok & ((1ull << lane) - 1ull));
-          ring_fill(q, &q.slots[(size_t)(ticket & q.cap_mask) * kSlotStride], ticket, (uint32_t)tile, DeviceNap{});
+          ok[k] = __ballot(tile[k] >= 0);
+          total += (uint32_t)__popcll(ok[k]);
         }
-        moved += (uint32_t)__popcll(ok);
+        uint32_t mb = 0;
+        if (lane == 0 && total)
+          mb = (uint32_t)at_add(tp_word(q), (unsigned long long)total); // tickets only
+        mb = (uint32_t)__shfl((int)mb, 0);
+        uint32_t before = 0; // moved items of the earlier groups of this pass
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+        {
+          if (tile[k] >= 0)
+          {
+            const uint32_t ticket = mb + before + (uint32_t)__popcll(ok[k] & ((1ull << lane) - 1ull));
+            ring_fill(q, &q.slots[(size_t)(ticket & q.cap_mask) * kSlotStride], ticket, (uint32_t)tile[k],
+                      DeviceNap{});
+          }
+          before += (uint32_t)__popcll(ok[k]);
+        }
+        moved += total;
       }
       if (lane == 0)
         at_store(&q.ctrl[CTRL_LHEAD + b], tb);
@@ -400,6 +428,9 @@ namespace dtk
     return moved;
   }
 
+#ifndef DTK_DRAIN_SPIN
+#define DTK_DRAIN_SPIN 32u
+#endif
   // Pop: wait until the claimed ticket's slot is filled, or no work is
   // pending (-1), all lanes of one wave (the same result in every lane).
   // The popped tile goes from queued to RUNNING; *key receives the key of
@@ -411,16 +442,23 @@ namespace dtk
   // holds.  Bucket orders: a waiter whose ticket is not yet issued to any
   // pusher (the main ring is empty for it) drains the later rings: the
   // first waiter in line at once, any other after about 16 us.
-  __device__ int pop(const Queue& q, uint32_t ticket, uint32_t* key, uint64_t& t_acq)
+  //   pre: the claimed tile when the last publish already took it (see
+  //   fill_main), else -1; reset here.
+  __device__ int pop(const Queue& q, uint32_t ticket, uint32_t* key, int& pre, uint64_t& t_acq)
   {
     const int lane    = threadIdx.x & 63;
     uint64_t  t_start = 0;
     for (uint32_t spin = 0;; ++spin)
     {
-      int t = -2;
-      if (lane == 0)
-        t = fifo_poll(q, ticket);
-      t = __shfl(t, 0);
+      int t = pre;
+      pre   = -1;
+      if (t < 0)
+      {
+        t = -2;
+        if (lane == 0)
+          t = fifo_poll(q, ticket);
+        t = __shfl(t, 0);
+      }
       if (t >= 0)
       {
         uint32_t k = kInfBits;
@@ -448,7 +486,7 @@ namespace dtk
         if (lane == 0)
         {
           const uint32_t tail = (uint32_t)at_load(tp_word(q));
-          if ((int)(tail - ticket) <= 0 && (tail == ticket || spin > 32u))
+          if ((int)(tail - ticket) <= 0 && (tail == ticket || spin > DTK_DRAIN_SPIN))
             drain = drain_try_lock(q) ? 1 : 0;
         }
         if (__shfl(drain, 0))
@@ -501,6 +539,47 @@ namespace dtk
     }
   }
 
+  // Fills the main-ring slots of the lanes with `need` (tickets the caller
+  // reserved).  A slot is free for its ticket once the claimer of the ticket
+  // one lap earlier has taken its tile.  Claims are taken out of order (a
+  // worker claims its next ticket when a visit starts and takes it after
+  // the visit), so that claimer may be this very worker, or another one
+  // that is itself waiting here: while any lane waits, lane 0 takes the
+  // worker's own claimed tile as soon as it is filled and keeps it in `pre`
+  // for the next pop.  The oldest filled claim is then always taken, and no
+  // wait can close a cycle.
+  __device__ void fill_main(const Queue& q, bool need, uint32_t ticket, uint32_t tile, uint32_t claim, int& pre)
+  {
+    const int                lane = threadIdx.x & 63;
+    unsigned long long*      slot = &q.slots[(size_t)(ticket & q.cap_mask) * kSlotStride];
+    const unsigned long long full = ((unsigned long long)(2u * ticket + 1u) << 32) | tile;
+    bool                     done = !need;
+    for (uint32_t spin = 0;; ++spin)
+    {
+      if (!done)
+      {
+        unsigned long long expect = (unsigned long long)(2u * ticket) << 32;
+        done                      = at_cas(slot, expect, full);
+      }
+      if (__ballot(!done) == 0)
+        return;
+      int p = pre;
+      if (lane == 0 && p < 0)
+      {
+        const int t = fifo_poll(q, claim);
+        p           = t >= 0 ? t : -1;
+      }
+      pre = __shfl(p, 0);
+      if (spin >= kSlotWaitLimit)
+      {
+        if (!done)
+          at_or(&q.ctrl[CTRL_ERROR], (uint32_t)ERR_WATCHDOG);
+        return;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+
   // Publish a finished visit, wave-wide (called by all 64 lanes after a
   // release fence): lanes 0..7 wake the neighbours whose pixels this visit
   // improved, lane 8 ends the tile's own visit (idle, or queued again when
@@ -510,9 +589,11 @@ namespace dtk
   // One atomic reserves the main-ring tickets and counts every new item as
   // pending, retiring this visit's item; the wave that takes pending to 0
   // raises DONE.  Kw: the visit's wake keys per wave.
+  // claim: the ticket this worker claimed when the visit started (lane 0);
+  // pre: see fill_main.
   template <int NW>
   __device__ void publish(const Queue& q, const ImgDesc& d, int t, int tx, int ty, uint32_t border, bool requeue,
-                          uint32_t own_key, uint32_t front, const uint32_t* Kw)
+                          uint32_t own_key, uint32_t front, const uint32_t* Kw, uint32_t claim, int& pre)
   {
     const int lane   = threadIdx.x & 63;
     int       target = -1;
@@ -551,11 +632,8 @@ namespace dtk
     // No insertion is issued before that count has returned: the new items
     // are pending before any of them can be popped.
     __asm__ volatile("" ::: "memory");
-    if (insert && !later)
-    {
-      const uint32_t ticket = base + (uint32_t)__popcll(mains & ((1ull << lane) - 1ull));
-      ring_fill(q, &q.slots[(size_t)(ticket & q.cap_mask) * kSlotStride], ticket, (uint32_t)target, DeviceNap{});
-    }
+    const uint32_t ticket = base + (uint32_t)__popcll(mains & ((1ull << lane) - 1ull));
+    fill_main(q, insert && !later, ticket, (uint32_t)target, claim, pre);
     if (later)
       later_push(q, bucket, (uint32_t)target, DeviceNap{});
     const uint64_t lat = __ballot(later);
@@ -590,6 +668,7 @@ namespace dtk
       return;
     }
     uint32_t ticket = 0, own_key = kInfBits, front = 0;
+    int      pre    = -1; // wave 0: claimed tile taken early by fill_main
     if (threadIdx.x == 0)
       ticket = fifo_claim(q);
     for (;;)
@@ -597,7 +676,7 @@ namespace dtk
       const uint64_t c0 = __builtin_amdgcn_s_memtime();
       if (w == 0)
       {
-        const int t = pop(q, ticket, &own_key, t_acq);
+        const int t = pop(q, ticket, &own_key, pre, t_acq);
         if (lane == 0)
           s_tile = t;
       }
@@ -635,7 +714,8 @@ namespace dtk
         if (vr.changed)
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         t_rel += __builtin_amdgcn_s_memtime() - c2;
-        publish<NW>(q, d, t, tx, ty, vr.border, !vr.converged, own_key, (uint32_t)__shfl((int)front, 0), Kw);
+        publish<NW>(q, d, t, tx, ty, vr.border, !vr.converged, own_key, (uint32_t)__shfl((int)front, 0), Kw, ticket,
+                    pre);
       }
       t_pub += __builtin_amdgcn_s_memtime() - c2;
     }
