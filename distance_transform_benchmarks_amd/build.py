@@ -3,6 +3,7 @@
   lib/libdtk.so   HIP kernels + the C ABI of include/dtk.h
   lib/libdt.so    C++ API of include/dt/ (namespace dt), on top of libdtk
   lib/dt_tests    self-contained C++ test runner of the C++ API
+  lib/dtk_queue_check  CPU model check of the work-queue protocol
 
 All HIP code is compiled for gfx950 with -ffp-contract=off (the relaxation
 must not be contracted into FMAs: bit-exactness with the CPU fixed point)
@@ -37,9 +38,11 @@ COMMON = [
 
 DTK_SOURCES = ["dtk_task.hip", "dtk_raster.hip", "dtk_adjoint.hip", "dtk_color.hip", "dtk_priority.hip", "dtk_host.cpp"]
 DT_SOURCES = ["dt_api.cpp", "dt_color.cpp", "dt_png.cpp"]
-# Command-line tools (the reference's bin/distance_transform.cpp and
-# bin/empiric_check.cpp), linked against libdt.
-TOOLS = {"dt_distance_transform": "dt_cli_distance_transform.cpp", "dt_empiric_check": "dt_cli_empiric_check.cpp"}
+# Command-line tools (the reference's bin/distance_transform.cpp,
+# bin/empiric_check.cpp, bin/rounds_per_blocks.cu and bin/priority_block.cu),
+# linked against libdt.
+TOOLS = {"dt_distance_transform": "dt_cli_distance_transform.cpp", "dt_empiric_check": "dt_cli_empiric_check.cpp",
+         "dt_rounds_per_blocks": "dt_cli_rounds_per_blocks.cpp", "dt_priority_block": "dt_cli_priority_block.cpp"}
 
 
 def _run(cmd):
@@ -81,6 +84,12 @@ def build(jobs: int = 8, verbose: bool = True) -> dict:
               f"-L{LIB}", "-ldt", "-ldtk", "-Wl,-rpath,$ORIGIN"])
         os.replace(tmp, exe)
         tools[name] = exe
+    # CPU model check of the work-queue protocol (host threads only).
+    qcheck = os.path.join(LIB, "dtk_queue_check")
+    tmp = qcheck + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", *COMMON, "-x", "hip", os.path.join(CSRC, "dtk_queue_check.cpp"), "-o", tmp,
+          "-pthread"])
+    os.replace(tmp, qcheck)
     tests = os.path.join(LIB, "dt_tests")
     tmp = tests + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", *COMMON, "-x", "hip", os.path.join(CSRC, "dt_tests.cpp"), "-o", tmp,
@@ -88,7 +97,7 @@ def build(jobs: int = 8, verbose: bool = True) -> dict:
     os.replace(tmp, tests)
     if verbose:
         print(f"built {libdtk}, {libdt}, {tests}, " + ", ".join(tools.values()))
-    return {"libdtk": libdtk, "libdt": libdt, "dt_tests": tests, **tools}
+    return {"libdtk": libdtk, "libdt": libdt, "dt_tests": tests, "dtk_queue_check": qcheck, **tools}
 
 
 def clean():

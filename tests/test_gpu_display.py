@@ -1,8 +1,9 @@
 """GPU tests of the callers on either side of the transform: the device
 display kernels (dtk_minmax_f32 / dtk_normalize_u8 / dtk_inferno /
-dtk_colorize, through the C ABI) against the oracle's numpy restatements,
-bit for bit, and the command-line tools (the reference's
-bin/distance_transform.cpp and bin/empiric_check.cpp) end to end.
+dtk_colorize, dtk_tile_image, through the C ABI) against the oracle's numpy
+restatements, bit for bit, and the command-line tools (the reference's
+bin/distance_transform.cpp, bin/empiric_check.cpp, bin/rounds_per_blocks.cu
+and bin/priority_block.cu) end to end.
 """
 from __future__ import annotations
 
@@ -160,3 +161,87 @@ def test_empiric_check_tool_and_driver(tmp_path):
     _say("empiric_check driver")
     res = empiric_check.run(tmp_path / "img", tmp_path / "mask", 0.7)
     assert len(res["passed"]) == 3 and not res["failed"] and not res["missing_mask"]
+
+
+def _expand(values, tw, th, W, H):
+    """Per-tile values spread over their tiles (host restatement of dtk_tile_image)."""
+    ys, xs = np.mgrid[0:H, 0:W]
+    return values[ys // th, xs // tw].astype(np.float32)
+
+
+def test_rounds_per_blocks_tool(dt, dev, oracle, table, tmp_path):
+    """bin/rounds_per_blocks.cu:25-63: the blocks schedule's per-tile round
+    counts as an inferno heat map (rounds_per_block.png, image-sized, one
+    colour per tile).  The counts, taken again through the Python API, sum
+    to the call's visits and give the same image through the oracle's
+    normalisation and colour map."""
+    import torch
+
+    W, H, lam = 300, 200, 0.5
+    img, m = image("u8", W, H, 77, smooth=True), mask("u8", W, H, 77, 2)
+    PIL.fromarray(img).save(tmp_path / "img.png")
+    PIL.fromarray(m).save(tmp_path / "mask.png")
+    _say("rounds_per_blocks tool")
+    r = subprocess.run([os.path.join(LIB, "dt_rounds_per_blocks"), tmp_path / "img.png", tmp_path / "mask.png",
+                        str(lam)], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    got = np.asarray(PIL.open(tmp_path / "rounds_per_block.png"))
+    assert got.shape == (H, W, 3)
+    tw, th = dt.tile_size(lam, schedule="blocks")
+    counts = dt.visits_buffer(W, H, lam, schedule="blocks")
+    _, st = dt.ggdt(torch.from_numpy(img).to(dev), torch.from_numpy(m).to(dev), lam, schedule="blocks",
+                    round_per_blocks=counts, stats=True)
+    c = counts.cpu().numpy()
+    assert int(c.sum()) == st["visits"] and c.max() >= 1
+    want = oracle.inferno_rgb(oracle.normalize_u8(_expand(c, tw, th, W, H)), table)
+    assert np.array_equal(got, want)
+    for ty in range(c.shape[0]):
+        for tx in range(c.shape[1]):
+            block = got[ty * th:(ty + 1) * th, tx * tw:(tx + 1) * tw].reshape(-1, 3)
+            assert (block == block[0]).all(), (tx, ty)
+    _say("rounds_per_blocks usage")
+    r = subprocess.run([os.path.join(LIB, "dt_rounds_per_blocks"), tmp_path / "img.png"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 1 and "Usage:" in r.stderr and "img.png mask.png lmd" in r.stderr
+
+
+def test_priority_block_tool(dt, dev, oracle, table, tmp_path):
+    """bin/priority_block.cu:24-71: prints the 32 x 32 grid size and the 64
+    bucket offsets, and writes the priorities as a heat map
+    (priorities.png)."""
+    import torch
+
+    W, H = 333, 170
+    m = mask("u8", W, H, 91, 3)
+    PIL.fromarray(m).save(tmp_path / "mask.png")
+    _say("priority_block tool")
+    r = subprocess.run([os.path.join(LIB, "dt_priority_block"), tmp_path / "mask.png"], capture_output=True,
+                       text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()
+    gw, gh = (W + 31) // 32, (H + 31) // 32
+    assert lines[0] == f"Grid with {gw * gh} blocks"
+    prio, offsets, _ = dt.block_priorities(torch.from_numpy(m).to(dev))
+    assert [int(v) for v in lines[1].split()] == offsets.cpu().tolist()
+    got = np.asarray(PIL.open(tmp_path / "priorities.png"))
+    want = oracle.inferno_rgb(oracle.normalize_u8(_expand(prio.cpu().numpy(), 32, 32, W, H)), table)
+    assert got.shape == (H, W, 3) and np.array_equal(got, want)
+
+
+def test_task_visit_heatmap(dt, dev, oracle, table):
+    """Per-tile visit counts of the task schedule as a heat map through the
+    Python API (visits_buffer, tile_size, heatmap): one colour per tile,
+    counts summing to the call's visits."""
+    import torch
+
+    W, H, lam = 700, 450, 0.5
+    img, m = image("rgb8", W, H, 12), mask("f32", W, H, 12, 5)
+    counts = dt.visits_buffer(W, H, lam)
+    _say("task visit heat map")
+    _, st = dt.ggdt(torch.from_numpy(img).to(dev), torch.from_numpy(m).to(dev), lam, 1e3, round_per_blocks=counts,
+                    stats=True)
+    tw, th = dt.tile_size(lam)
+    rgb = dt.heatmap(counts, W, H, tw, th).cpu().numpy()
+    c = counts.cpu().numpy()
+    assert int(c.sum()) == st["visits"]
+    assert np.array_equal(rgb, oracle.inferno_rgb(oracle.normalize_u8(_expand(c, tw, th, W, H)), table))
