@@ -16,9 +16,18 @@ The JSON line also carries:
                 (null when absent);
   cpu_baseline  the CPU oracle (a restatement of the reference's raster GGDT)
                 on the same 4096x4096 image at lambda=0.5, OpenMP over the
-                box's cores, also checked bit-exact against the GPU result;
+                box's cores (at most 16), median of 5 runs after a warm-up,
+                also checked bit-exact against the GPU result; with the
+                core counts, and a single-thread run on blob512;
+  orders        per lambda, ms and visits of each queue order (FIFO, KEY,
+                SEED) on the same image;
+  maze          the 8192x8192 spiral maze (C8, lambda=1): ms per converged
+                call, Mpix/s, visits per tile per queue order, D == I;
+  blob512       the reference-CPU-sized workload (512x512 u8, C4,
+                lambda=1): GPU ms (median of 10) and the oracle;
   batched       256 uneven uint8 images (C8, lambda=0.8): one graph-captured
-                persistent launch vs 256 per-image calls, images/s;
+                persistent launch vs 256 per-image calls, images/s, and
+                visits per queue order;
   display       the reference CLI's step after the transform on the
                 lambda=0.5 map: min / max, normalisation to uint8 and the
                 inferno colour map (dtk_colorize, two kernels), timed with
@@ -34,6 +43,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -59,12 +70,30 @@ def load_traffic():
         return None
 
 
-def cpu_cores() -> int:
+# CPU threads of one GPU's share of the box (worker pools are sized to it).
+CPU_SHARE = 16
+
+
+def cpu_counts():
+    """(hardware threads, affinity cores, threads used)."""
+    hw = os.cpu_count() or 1
     try:
-        n = len(os.sched_getaffinity(0))
+        aff = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+        aff = hw
+    return hw, aff, max(1, min(CPU_SHARE, aff))
+
+
+def median_time(fn, runs=5):
+    """Median wall time (monotonic clock) of `runs` calls after one warm-up."""
+    fn()
+    ts = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t)
+    ts.sort()
+    return ts[len(ts) // 2], ts
 
 
 def main(argv=None) -> int:
@@ -80,6 +109,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable (adjoint) leg")
+    ap.add_argument("--no-maze", action="store_true", help="skip the maze leg")
     ap.add_argument("--batch-count", type=int, default=256)
     a = ap.parse_args(argv)
 
@@ -87,7 +117,6 @@ def main(argv=None) -> int:
         print(json.dumps({"error": "this build runs on exactly one GPU (no multi-GPU path); use --gpus 1"}))
         return 2
 
-    import numpy as np
     import torch
 
     import distance_transform_benchmarks_amd as dt
@@ -115,21 +144,26 @@ def main(argv=None) -> int:
         step()
     torch.cuda.synchronize()
 
-    # Per-lambda statistics (outside the timed region; these calls synchronise).
+    # Per-lambda statistics (outside the timed region; these calls synchronise),
+    # and the same call under each queue order.
     per_lambda = {}
     for lam in lambdas:
         _, st = dt.ggdt(img, mask, lam, 1e3, out=outs[lam], stats=True, **kw)
-        s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        s0.record()
-        for _ in range(3):
-            dt.ggdt(img, mask, lam, 1e3, out=outs[lam], **kw)
-        s1.record()
-        torch.cuda.synchronize()
-        ms = s0.elapsed_time(s1) / 3
+        ms = event_ms(torch, lambda: dt.ggdt(img, mask, lam, 1e3, out=outs[lam], **kw), 3)
         per_lambda[str(lam)] = {"ms": round(ms, 4), "mpix_s": round(npx / ms / 1e3, 1), "visits": st["visits"],
                                 "sweeps": st["sweeps"], "tiles": st["tiles"], "tile_rows": st["tile_rows"],
                                 "workers": st["workers"]}
         log(f"lambda={lam}: {ms:.3f} ms/call, {per_lambda[str(lam)]['mpix_s']} Mpix/s, stats {st}")
+        orders = {}
+        for order in ("fifo", "key", "seed"):
+            okw = dict(kw, queue_order=order)
+            _, so = dt.ggdt(img, mask, lam, 1e3, out=outs[lam], stats=True, **okw)
+            oms = event_ms(torch, lambda: dt.ggdt(img, mask, lam, 1e3, out=outs[lam], **okw), 3)
+            orders[order] = {"ms": round(oms, 4), "visits": so["visits"],
+                             "sweeps_per_visit": round(so["sweeps"] / max(1, so["visits"]), 2),
+                             "visits_per_tile": round(so["visits"] / max(1, so["tiles"]), 3)}
+        per_lambda[str(lam)]["orders"] = orders
+        log(f"lambda={lam} orders: {orders}")
 
     # Timed region.
     torch.cuda.synchronize()
@@ -175,26 +209,111 @@ def main(argv=None) -> int:
     if not a.no_adjoint:
         result["adjoint"] = bench_adjoint(dt, synth, torch, dev)
 
+    if not a.no_maze:
+        result["maze"] = bench_maze(dt, synth, torch, dev)
+
+    result["blob512"] = bench_blob512(dt, synth, torch, dev, a.no_cpu)
+
     if not a.no_cpu:
         from oracle import oracle as O
 
-        cores = cpu_cores()
+        hw, aff, threads = cpu_counts()
         lam = 0.5
-        t = time.time()
-        want, iters = O.raster(img_np, mask_np, lam, 1e3, conn8=True, threads=cores)
-        cpu_s = time.time() - t
+        res = {}
+
+        def run():
+            res["D"], res["iters"] = O.raster(img_np, mask_np, lam, 1e3, conn8=True, threads=threads)
+
+        cpu_s, runs = median_time(run, 5)
         got = outs[lam].cpu().numpy()
-        exact = bool(np.array_equal(got.view(np.uint32), want.view(np.uint32)))
+        exact = bool(np.array_equal(got.view(np.uint32), res["D"].view(np.uint32)))
         result["cpu_baseline"] = {
-            "value": round(npx / cpu_s / 1e6, 4), "unit": "Mpix/s", "cores": cores, "kind": "port",
-            "sample": f"CPU oracle raster GGDT, full {a.size}x{a.size} RGB8 image at lambda=0.5 ({iters} raster "
-                      f"iterations, {cpu_s:.1f}s, OpenMP {cores} threads)",
+            "value": round(npx / cpu_s / 1e6, 4), "unit": "Mpix/s", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle raster GGDT, full {a.size}x{a.size} RGB8 image at lambda=0.5 ({res['iters']} raster "
+                      f"iterations; median of 5 runs after a warm-up: {cpu_s:.2f}s; OpenMP {threads} threads)",
+            "hardware_concurrency": hw, "affinity_cores": aff, "threads": threads,
+            "threads_note": f"OpenMP over the affinity cores, at most {CPU_SHARE} (one GPU's share of the box)",
+            "runs_s": [round(t, 3) for t in runs], "timer": "monotonic wall clock (time.perf_counter)",
             "gpu_bit_exact": exact,
+            "single_thread": result["blob512"].get("cpu_single_thread"),
         }
-        log(f"cpu baseline: {cpu_s:.1f}s ({iters} iterations, {cores} threads); GPU bit-exact: {exact}")
+        log(f"cpu baseline: {cpu_s:.2f}s median ({res['iters']} iterations, {threads} of {aff} affinity cores, "
+            f"{hw} hardware threads); GPU bit-exact: {exact}")
 
     print(json.dumps(result), flush=True)
     return 0
+
+
+def event_ms(torch, fn, reps):
+    """Mean ms per call of `fn` between two events on the current stream."""
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def bench_maze(dt, synth, torch, dev):
+    """The adversarial spiral maze: 8192^2 u8, walls 255, one seed in the
+    corridor, C8, lambda=1; the known answer is D = I.  6 B/px algorithmic
+    (image 1 + mask 1 + D 4)."""
+    t = time.time()
+    img_np, mask_np = synth.maze8192()
+    img, mask = torch.from_numpy(img_np).to(dev), torch.from_numpy(mask_np).to(dev)
+    out = torch.empty(img.shape, dtype=torch.float32, device=dev)
+    npx = img.numel()
+    log(f"maze inputs generated in {time.time() - t:.1f}s")
+    dt.ggdt(img, mask, 1.0, 1e10, out=out)
+    ms = event_ms(torch, lambda: dt.ggdt(img, mask, 1.0, 1e10, out=out), 2)
+    exact = bool(torch.equal(out, img.float()))
+    gbs = 6 * npx / (ms * 1e-3) / 1e9
+    r = {"workload": "maze8192: 8192x8192 u8 square spiral maze (walls 3 px = 255, corridor 9 px = 0), one seed "
+                     "at the centre, C8, lambda=1, converged, task schedule (default order)",
+         "ms": round(ms, 2), "mpix_s": round(npx / ms / 1e3, 1), "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
+         "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 6), "algorithmic_bytes": 6 * npx, "D_equals_I": exact,
+         "orders": {}}
+    for order in ("fifo", "key", "seed"):
+        t0 = time.time()
+        _, st = dt.ggdt(img, mask, 1.0, 1e10, out=out, stats=True, queue_order=order)
+        r["orders"][order] = {"ms": round((time.time() - t0) * 1e3, 1), "visits": st["visits"],
+                              "visits_per_tile": round(st["visits"] / max(1, st["tiles"]), 1)}
+    log(f"maze: {ms:.1f} ms ({r['mpix_s']} Mpix/s), D == I: {exact}, orders {r['orders']}")
+    return r
+
+
+def bench_blob512(dt, synth, torch, dev, no_cpu):
+    """The reference-CPU-sized workload: 512^2 u8 blob field, 8 seed discs,
+    C4, lambda=1: GPU time (median of 10 calls) and, unless no_cpu, the
+    single-thread oracle (median of 5 runs after a warm-up)."""
+    img_np, mask_np = synth.blob512()
+    img, mask = torch.from_numpy(img_np).to(dev), torch.from_numpy(mask_np).to(dev)
+    out = torch.empty(img.shape, dtype=torch.float32, device=dev)
+    npx = img.numel()
+    for _ in range(3):
+        dt.ggdt(img, mask, 1.0, 1e10, connectivity=4, out=out)
+    torch.cuda.synchronize()
+    ts = sorted(event_ms(torch, lambda: dt.ggdt(img, mask, 1.0, 1e10, connectivity=4, out=out), 1)
+                for _ in range(10))
+    ms = ts[len(ts) // 2]
+    r = {"workload": "blob512: 512x512 u8 blob field, 8 seed discs (r=5), C4, lambda=1, converged, task schedule",
+         "ms": round(ms, 4), "mpix_s": round(npx / ms / 1e3, 1)}
+    if not no_cpu:
+        from oracle import oracle as O
+
+        res = {}
+
+        def run():
+            res["D"], res["iters"] = O.raster(img_np, mask_np, 1.0, 1e10, conn8=False, threads=1)
+
+        cpu_s, runs = median_time(run, 5)
+        r["gpu_bit_exact"] = bool(np.array_equal(out.cpu().numpy().view(np.uint32), res["D"].view(np.uint32)))
+        r["cpu_single_thread"] = {"value": round(npx / cpu_s / 1e6, 3), "unit": "Mpix/s", "threads": 1,
+                                  "median_s": round(cpu_s, 4), "runs_s": [round(t, 4) for t in runs],
+                                  "sample": f"CPU oracle, blob512 ({res['iters']} raster iterations), one thread"}
+    log(f"blob512: {ms:.4f} ms GPU; {r.get('cpu_single_thread')}")
+    return r
 
 
 def bench_display(dt, torch, D, npx, reps=20):
@@ -295,13 +414,22 @@ def bench_batched(dt, synth, torch, dev, count):
     bat_ms = e[2].elapsed_time(e[3]) / reps
     same = all(torch.equal(x.view(torch.int32), y.view(torch.int32)) for x, y in zip(outs_a, outs_b))
     plan.close()
+    # One launch per queue order (ggdt_batched, not captured): ms and visits.
+    orders = {}
+    for order in ("fifo", "key", "seed"):
+        _, st = dt.ggdt_batched(imgs, masks, lam, 1e10, outs=outs_a, stats=True, queue_order=order)
+        oms = event_ms(torch, lambda: dt.ggdt_batched(imgs, masks, lam, 1e10, outs=outs_a, queue_order=order), 2)
+        orders[order] = {"ms": round(oms, 3), "visits": st["visits"],
+                         "visits_per_tile": round(st["visits"] / max(1, st["tiles"]), 3),
+                         "sweeps_per_visit": round(st["sweeps"] / max(1, st["visits"]), 2)}
+    log(f"batched orders: {orders}")
     r = {"workload": f"batch256: {count} uint8 images, sides log-uniform [64, 2048], aspect [0.5, 2], C8, "
                      f"lambda=0.8", "pixels": px,
          "per_image_ms": round(per_ms, 3), "per_image_imgs_s": round(count / per_ms * 1e3, 1),
          "batched_graph_ms": round(bat_ms, 3), "batched_imgs_s": round(count / bat_ms * 1e3, 1),
-         "speedup": round(per_ms / bat_ms, 3), "outputs_identical": bool(same),
-         "autotuned": {k: plan.tuning[k] for k in ("tile_rows", "tile_waves", "max_sweeps", "waves_per_cu", "candidates",
-                                                   "best_ms", "default_ms")}}
+         "speedup": round(per_ms / bat_ms, 3), "outputs_identical": bool(same), "orders": orders,
+         "autotuned": {k: plan.tuning[k] for k in ("queue_order", "tile_rows", "tile_waves", "max_sweeps",
+                                                   "waves_per_cu", "candidates", "best_ms", "default_ms")}}
     log(f"batched: per-image {per_ms:.2f} ms, graph {bat_ms:.2f} ms, x{per_ms / bat_ms:.2f}, identical {same}")
     return r
 
