@@ -31,7 +31,7 @@ EXPORTS = (
     "dtk_device_to_host", "dtk_copy2d", "dtk_selftest", "dtk_device_cu_count", "dtk_profile_begin",
     "dtk_profile_end", "dtk_batch_autotune", "dtk_tile_count_for", "dtk_minmax_f32", "dtk_normalize_u8",
     "dtk_inferno", "dtk_colorize", "dtk_block_priorities", "dtk_sync_errors", "dtk_tile_size_for",
-    "dtk_tile_image",
+    "dtk_tile_image", "dtk_ggdt_noalloc",
 )
 
 
@@ -128,6 +128,7 @@ def load() -> ctypes.CDLL:
     lib.dtk_colorize.argtypes = [P(DtkImage), P(DtkImage), vp]
     lib.dtk_block_priorities.argtypes = [P(DtkImage), vp, vp, P(ctypes.c_uint32), vp]
     lib.dtk_sync_errors.argtypes = [vp]
+    lib.dtk_ggdt_noalloc.argtypes = [P(DtkImage), P(DtkImage), P(DtkImage), P(DtkParams), vp, P(DtkStats)]
     lib.dtk_tile_size_for.argtypes = [P(DtkParams), P(ctypes.c_int), P(ctypes.c_int)]
     lib.dtk_tile_size_for.restype = ctypes.c_int
     lib.dtk_tile_image.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -136,7 +137,7 @@ def load() -> ctypes.CDLL:
                  "dtk_batch_plan_destroy", "dtk_batch_autotune", "dtk_ggdt_pred", "dtk_ggdt_pred_batched", "dtk_ggdt_backward", "dtk_ggdt_backward_batched", "dtk_malloc_pitch", "dtk_free",
                  "dtk_host_to_device", "dtk_device_to_host", "dtk_copy2d", "dtk_selftest", "dtk_profile_begin",
                  "dtk_profile_end", "dtk_sync_errors", "dtk_block_priorities", "dtk_minmax_f32", "dtk_normalize_u8",
-                 "dtk_inferno", "dtk_colorize", "dtk_tile_image"):
+                 "dtk_inferno", "dtk_colorize", "dtk_tile_image", "dtk_ggdt_noalloc"):
         getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib

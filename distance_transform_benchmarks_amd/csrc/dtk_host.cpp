@@ -481,8 +481,9 @@ namespace
     return DTK_OK;
   }
 
-  // Persistent workers (workgroups of nw waves) of a task-schedule call.
-  int task_workers(const Workspace& w, const dtk_params* p, int S, bool c8, int cm, int ntiles)
+  // Persistent workers (workgroups of nw waves) of a task-schedule call on a
+  // device with `cu` compute units.
+  int task_workers_cu(int cu, const dtk_params* p, int S, bool c8, int cm, int ntiles)
   {
     const int nw     = tile_waves(p, 64 / S);
     int       per_cu = task_occupancy(S, c8, cm, nw);
@@ -495,10 +496,14 @@ namespace
     // takes 0.56 ms with two per CU and 0.60 ms with three.
     if (p->waves_per_cu == 0 && nw == 4 && per_cu > 2)
       per_cu = 2;
-    long long n = (long long)per_cu * w.cu;
+    long long n = (long long)per_cu * cu;
     if (n > ntiles)
       n = ntiles;
     return (int)(n > 0 ? n : 1);
+  }
+  int task_workers(const Workspace& w, const dtk_params* p, int S, bool c8, int cm, int ntiles)
+  {
+    return task_workers_cu(w.cu, p, S, c8, cm, ntiles);
   }
 
   // Task-schedule buffers of one call, carved from a device block.
@@ -1717,6 +1722,135 @@ dtk_status dtk_tile_image(const void* values, int value_bytes, int tiles_x, int 
   return DTK_OK;
 }
 
+} // extern "C"
+
+// ---- the allocation-free entry point ---------------------------------------
+namespace
+{
+  // Scratch of dtk_ggdt_noalloc: the queue, tile statistics and control
+  // words of one call, in a file-scope device array (each device holds its
+  // own copy from module load until the process ends).  Host code reaches
+  // it only through the address hipGetSymbolAddress returns with that
+  // device current.
+  constexpr size_t kNoAllocScratchBytes = (size_t)16 << 20;
+  static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+  // Error bits that the kernels of dtk_ggdt_noalloc calls leave on a device,
+  // kept until dtk_sync_errors or a call with statistics takes them.
+  static __device__ __align__(256) uint32_t scratch_error[64];
+
+  struct NoAllocDevice
+  {
+    std::once_flag once;
+    hipError_t     status = hipSuccess;
+    void*          ptr    = nullptr; // device address of `scratch`
+    uint32_t*      err    = nullptr; // device address of `scratch_error`
+    int            cu     = 0;
+  };
+  NoAllocDevice g_noalloc[kMaxDevices];
+
+  // Set-up on the device's first call (the device is current): the device
+  // addresses of the scratch arrays and the CU count.  Allocates nothing.
+  dtk_status noalloc_device(int dev, NoAllocDevice** out)
+  {
+    NoAllocDevice& nd = g_noalloc[dev];
+    std::call_once(nd.once, [&nd, dev] {
+      hipError_t e = hipGetSymbolAddress(&nd.ptr, HIP_SYMBOL(scratch));
+      if (e == hipSuccess)
+        e = hipGetSymbolAddress(reinterpret_cast<void**>(&nd.err), HIP_SYMBOL(scratch_error));
+      if (e == hipSuccess)
+        e = hipDeviceGetAttribute(&nd.cu, hipDeviceAttributeMultiprocessorCount, dev);
+      if (e == hipSuccess && nd.cu <= 0)
+        nd.cu = 1;
+      nd.status = e;
+    });
+    if (nd.status != hipSuccess)
+      return fail(DTK_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(nd.status));
+    *out = &nd;
+    return DTK_OK;
+  }
+
+  // The sticky error bits of the allocation-free calls on device `dev`
+  // (synchronous; only once that device has been set up).
+  dtk_status take_noalloc_error(int dev)
+  {
+    NoAllocDevice& nd = g_noalloc[dev];
+    if (!nd.err)
+      return DTK_OK;
+    uint32_t bits = 0u;
+    DTK_HIP(hipMemcpy(&bits, nd.err, sizeof(bits), hipMemcpyDeviceToHost), "read scratch error");
+    if (!bits)
+      return DTK_OK;
+    const uint32_t zero = 0u;
+    DTK_HIP(hipMemcpy(nd.err, &zero, sizeof(zero), hipMemcpyHostToDevice), "clear scratch error");
+    return error_from_bits(bits, " (an earlier dtk_ggdt_noalloc call on this device)");
+  }
+} // namespace
+
+extern "C" {
+
+dtk_status dtk_ggdt_noalloc(const dtk_image* img, const dtk_image* mask, dtk_image* dist, const dtk_params* p,
+                            void* stream, dtk_stats* stats)
+{
+  dtk_status st;
+  if ((st = check_params(p)) != DTK_OK || (st = check_triplet(img, mask, dist)) != DTK_OK)
+    return st;
+  if (p->schedule != DTK_SCHED_TASK)
+    return fail(DTK_ERR_UNSUPPORTED, "dtk_ggdt_noalloc runs the task schedule");
+  if (use_cost_cache(p))
+    return fail(DTK_ERR_UNSUPPORTED, "dtk_ggdt_noalloc takes no cost plane");
+  const int   rh = region_rows(p);
+  hipStream_t s  = static_cast<hipStream_t>(stream);
+  if (img->width == 0 || img->height == 0)
+    return fill_stats(stats, nullptr, 0, 0, rh, 0);
+  int dev = 0;
+  DTK_HIP(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(DTK_ERR_UNSUPPORTED, "device index out of range");
+  NoAllocDevice* nd = nullptr;
+  if ((st = noalloc_device(dev, &nd)) != DTK_OK)
+    return st;
+  const bool     c8      = p->connectivity == DTK_C8;
+  const int      cm      = cost_mode(p);
+  const int      S       = 64 / rh;
+  const KParams  kp      = make_kparams(p, img->dtype, mask->dtype, rh);
+  const ImgDesc  d       = make_desc(img, mask, dist, rh, tile_waves(p, rh));
+  const int      ntiles  = d.tiles_x * d.tiles_y;
+  const int      workers = task_workers_cu(nd->cu, p, S, c8, cm, ntiles);
+  const uint32_t cap     = next_pow2((uint32_t)(ntiles + workers + 1));
+  const int      order   = queue_order(p, 1);
+  const size_t   need    = task_layout(nullptr, ntiles, cap, 1, order, nullptr);
+  if (need > kNoAllocScratchBytes)
+    return fail(DTK_ERR_UNSUPPORTED, "dtk_ggdt_noalloc: this call needs " + std::to_string(need) +
+                                       " bytes of scratch, at most " + std::to_string(kNoAllocScratchBytes) +
+                                       " are available (use dtk_ggdt)");
+  TaskBufs tb;
+  task_layout(nd->ptr, ntiles, cap, 1, order, &tb);
+  tb.q.tile_visits = p->round_per_blocks;
+  tb.q.mailbox     = nd->err;
+  Batch b{};
+  b.d0     = d;
+  b.nimg   = 1;
+  b.ntiles = ntiles;
+  // Every word that must start at zero (control words, ring slots, tile
+  // states and keys) is cleared by the call's own kernels.
+  if ((st = enqueue_task(b, kp, tb.q, S, c8, cm, workers, s, false)) != DTK_OK)
+    return st;
+  if (stats)
+  {
+    uint32_t h[CTRL_WORDS];
+    DTK_HIP(hipMemcpyAsync(h, tb.q.ctrl, sizeof(h), hipMemcpyDeviceToHost, s), "read counters");
+    DTK_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if (h[CTRL_ERROR])
+    {
+      const uint32_t zero = 0u; // reported here, not again
+      DTK_HIP(hipMemcpy(nd->err, &zero, sizeof(zero), hipMemcpyHostToDevice), "clear scratch error");
+      return error_from_bits(h[CTRL_ERROR], "");
+    }
+    return fill_stats(stats, h, 1, ntiles, rh, workers);
+  }
+  return DTK_OK;
+}
+
 dtk_status dtk_sync_errors(void* stream)
 {
   DTK_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "hipStreamSynchronize");
@@ -1731,7 +1865,8 @@ dtk_status dtk_sync_errors(void* stream)
     std::lock_guard<std::mutex> lock(w.mu);
     st = take_async_error(w);
   }
-  return st;
+  const dtk_status st2 = take_noalloc_error(dev);
+  return st != DTK_OK ? st : st2;
 }
 
 dtk_status dtk_selftest(int* out3)

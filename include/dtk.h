@@ -265,6 +265,23 @@ dtk_status dtk_copy2d(void* dst, long long dst_pitch, const void* src, long long
 dtk_status dtk_profile_begin(void);
 dtk_status dtk_profile_end(double* setup_ms, double* solve_ms, int* calls);
 
+/* GGDT of one image like dtk_ggdt with the task schedule, allocating
+ * nothing: no hipMalloc, hipFree, pinned host memory, event or stream.  No
+ * reference counterpart; for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation).
+ * The call's queue, tile statistics and control words live in a scratch
+ * array of 16 MiB that every device holds from library load until the
+ * process ends; a call that needs more (about 8192^2 pixels in FIFO order,
+ * 4096^2 in KEY or SEED order) is rejected with DTK_ERR_UNSUPPORTED.  The
+ * kernels clear what must start at zero on every call.  Every call on a
+ * device shares that device's scratch, so calls that overlap on one device
+ * (several streams, several threads) race: order them on one stream.  The
+ * cost plane is not available (DTK_ERR_UNSUPPORTED).  Device-side errors
+ * are returned at once when `stats` is non-NULL (the call then waits for
+ * the stream), else kept until dtk_sync_errors. */
+dtk_status dtk_ggdt_noalloc(const dtk_image* img, const dtk_image* mask, dtk_image* dist, const dtk_params* p,
+                            void* stream, dtk_stats* stats);
+
 /* Asynchronous errors.  The task schedule (dtk_ggdt, dtk_ggdt_batched,
  * batch plans) returns before its kernels finish, so a device-side error
  * (a soft mask value outside [0, 1] or NaN, a queue watchdog) cannot be
