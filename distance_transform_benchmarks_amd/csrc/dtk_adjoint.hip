@@ -353,7 +353,7 @@ namespace dtk
   {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n)
-      out[i] = __builtin_sqrtf((float)i);
+      out[i] = sqrt_int_rn((uint32_t)i); // the RGB8 edge-cost square root
   }
 
   hipError_t launch_selftest(int* dpp_out, float* sqrt_out, hipStream_t s)

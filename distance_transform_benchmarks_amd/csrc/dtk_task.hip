@@ -428,6 +428,12 @@ namespace dtk
     return moved;
   }
 
+  // Phases whose shader-clock cycles k_task sums per worker.
+  enum CycPhase : int
+  {
+    CY_WAIT = 0, CY_VISIT, CY_PUB, CY_SETUP, CY_SWEEP, CY_STORE, CY_REL, CY_ACQ, CY_COUNT
+  };
+
 #ifndef DTK_DRAIN_SPIN
 #define DTK_DRAIN_SPIN 32u
 #endif
@@ -444,7 +450,7 @@ namespace dtk
   // first waiter in line at once, any other after about 16 us.
   //   pre: the claimed tile when the last publish already took it (see
   //   fill_main), else -1; reset here.
-  __device__ int pop(const Queue& q, uint32_t ticket, uint32_t* key, int& pre, uint64_t& t_acq)
+  __device__ int pop(const Queue& q, uint32_t ticket, uint32_t* key, int& pre, uint64_t* cyc)
   {
     const int lane    = threadIdx.x & 63;
     uint64_t  t_start = 0;
@@ -467,7 +473,7 @@ namespace dtk
           const uint64_t a0 = __builtin_amdgcn_s_memtime();
           k                 = begin_visit(q, t);
           __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-          t_acq += __builtin_amdgcn_s_memtime() - a0;
+          cyc[CY_ACQ] += __builtin_amdgcn_s_memtime() - a0;
         }
         *key = (uint32_t)__shfl((int)k, 0);
         return t;
@@ -659,8 +665,12 @@ namespace dtk
     const int           w    = NW > 1 ? (int)(threadIdx.x >> 6) : 0;
     const int           lane = threadIdx.x & 63;
     uint32_t         visits = 0, sweeps = 0;
-    uint64_t         t_wait = 0, t_visit = 0, t_pub = 0, t_setup = 0, t_sweep = 0, t_store = 0;
-    uint64_t         t_rel = 0, t_acq = 0;
+    // Cycles per phase, kept in LDS by thread 0 (64-bit registers here
+    // would add to the spills of the visit's peak register pressure).
+    __shared__ uint64_t cyc[CY_COUNT];
+    if (threadIdx.x == 0)
+      for (int k = 0; k < CY_COUNT; ++k)
+        cyc[k] = 0;
     // Nothing seeded: no publish will ever raise DONE.
     if (pending_now(q) == 0u)
     {
@@ -676,7 +686,7 @@ namespace dtk
       const uint64_t c0 = __builtin_amdgcn_s_memtime();
       if (w == 0)
       {
-        const int t = pop(q, ticket, &own_key, pre, t_acq);
+        const int t = pop(q, ticket, &own_key, pre, cyc);
         if (lane == 0)
           s_tile = t;
       }
@@ -684,7 +694,8 @@ namespace dtk
       const int t = s_tile;
       __syncthreads();
       const uint64_t c1 = __builtin_amdgcn_s_memtime();
-      t_wait += c1 - c0;
+      if (threadIdx.x == 0)
+        cyc[CY_WAIT] += c1 - c0;
       if (t < 0)
         break;
       if (threadIdx.x == 0)
@@ -701,23 +712,29 @@ namespace dtk
       if (q.tile_visits && threadIdx.x == 0)
         at_add(&q.tile_visits[t], 1);
       sweeps += (uint32_t)vr.sweeps;
-      t_setup += vr.cyc_setup;
-      t_sweep += vr.cyc_sweep;
-      t_store += vr.cyc_store;
+      if (threadIdx.x == 0)
+      {
+        cyc[CY_SETUP] += vr.cyc_setup;
+        cyc[CY_SWEEP] += vr.cyc_sweep;
+        cyc[CY_STORE] += vr.cyc_store;
+      }
       // Every wave's D stores are ordered (workgroup barrier) before the
       // release below; wave 0 publishes for the whole tile.
       __syncthreads();
       const uint64_t c2 = __builtin_amdgcn_s_memtime();
-      t_visit += c2 - c1;
+      if (threadIdx.x == 0)
+        cyc[CY_VISIT] += c2 - c1;
       if (w == 0)
       {
         if (vr.changed)
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        t_rel += __builtin_amdgcn_s_memtime() - c2;
+        if (threadIdx.x == 0)
+          cyc[CY_REL] += __builtin_amdgcn_s_memtime() - c2;
         publish<NW>(q, d, t, tx, ty, vr.border, !vr.converged, own_key, (uint32_t)__shfl((int)front, 0), Kw, ticket,
                     pre);
       }
-      t_pub += __builtin_amdgcn_s_memtime() - c2;
+      if (threadIdx.x == 0)
+        cyc[CY_PUB] += __builtin_amdgcn_s_memtime() - c2;
     }
     if (threadIdx.x == 0 && visits)
     {
@@ -726,14 +743,14 @@ namespace dtk
     }
     if (threadIdx.x == 0)
     {
-      at_add(&q.ctrl[CTRL_T_WAIT], (uint32_t)(t_wait >> 10));
-      at_add(&q.ctrl[CTRL_T_VISIT], (uint32_t)(t_visit >> 10));
-      at_add(&q.ctrl[CTRL_T_PUB], (uint32_t)(t_pub >> 10));
-      at_add(&q.ctrl[CTRL_T_SETUP], (uint32_t)(t_setup >> 10));
-      at_add(&q.ctrl[CTRL_T_SWEEP], (uint32_t)(t_sweep >> 10));
-      at_add(&q.ctrl[CTRL_T_STORE], (uint32_t)(t_store >> 10));
-      at_add(&q.ctrl[CTRL_T_REL], (uint32_t)(t_rel >> 10));
-      at_add(&q.ctrl[CTRL_T_ACQ], (uint32_t)(t_acq >> 10));
+      at_add(&q.ctrl[CTRL_T_WAIT], (uint32_t)(cyc[CY_WAIT] >> 10));
+      at_add(&q.ctrl[CTRL_T_VISIT], (uint32_t)(cyc[CY_VISIT] >> 10));
+      at_add(&q.ctrl[CTRL_T_PUB], (uint32_t)(cyc[CY_PUB] >> 10));
+      at_add(&q.ctrl[CTRL_T_SETUP], (uint32_t)(cyc[CY_SETUP] >> 10));
+      at_add(&q.ctrl[CTRL_T_SWEEP], (uint32_t)(cyc[CY_SWEEP] >> 10));
+      at_add(&q.ctrl[CTRL_T_STORE], (uint32_t)(cyc[CY_STORE] >> 10));
+      at_add(&q.ctrl[CTRL_T_REL], (uint32_t)(cyc[CY_REL] >> 10));
+      at_add(&q.ctrl[CTRL_T_ACQ], (uint32_t)(cyc[CY_ACQ] >> 10));
     }
     report_error(q);
   }

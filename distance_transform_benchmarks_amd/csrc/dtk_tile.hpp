@@ -147,6 +147,26 @@ namespace dtk
       return gload<uint32_t>(row, 4ll * x);
   }
 
+  // Correctly rounded square root of an integer 0 <= n < 2^24 (here the
+  // exact squared colour distance, at most 3 * 255^2 = 195075), equal to
+  // sqrtf((float)n).  v_sqrt_f32 is accurate to 1 ulp, so the correctly
+  // rounded root is s or one of its float neighbours; the exact fma
+  // residuals n - s_dn * s and n - s_up * s against the neighbours' products
+  // pick it (the standard correction step of a 1-ulp square root).  n is 0
+  // or at least 1, so no denormal scaling and no special-value case is
+  // needed: for n = 0, s = 0 and both tests keep it.  dtk_selftest checks
+  // every n up to 195075 against the host's sqrtf.
+  __device__ __forceinline__ float sqrt_int_rn(uint32_t n)
+  {
+    const float x    = (float)n;
+    const float s    = __builtin_amdgcn_sqrtf(x);
+    const float s_dn = __uint_as_float(__float_as_uint(s) - 1u);
+    const float s_up = __uint_as_float(__float_as_uint(s) + 1u);
+    float       r    = __builtin_fmaf(-s_dn, s, x) <= 0.f ? s_dn : s;
+    r                = __builtin_fmaf(-s_up, s, x) > 0.f ? s_up : r;
+    return r;
+  }
+
   // |I(p) - I(q)|: uint8 abs difference (dt/src/utils.cuh:37-41); RGB8:
   // correctly rounded sqrtf of the exact integer squared distance; float32:
   // fabsf of the rounded difference.
@@ -163,7 +183,7 @@ namespace dtk
       const int dr = (int)(a & 255u) - (int)(b & 255u);
       const int dg = (int)((a >> 8) & 255u) - (int)((b >> 8) & 255u);
       const int db = (int)((a >> 16) & 255u) - (int)((b >> 16) & 255u);
-      return __builtin_sqrtf((float)(dr * dr + dg * dg + db * db));
+      return sqrt_int_rn((uint32_t)(dr * dr + dg * dg + db * db));
     }
     else
       return fabsf(__uint_as_float(a) - __uint_as_float(b));
