@@ -10,20 +10,27 @@
 //   G(p) = W(p) + sum over children c of G(c)         (subtree sums)
 //   edge q -> p : dL/dI(p) += G(p) * l_grad * sign(I(p) - I(q)), dL/dI(q) -= same
 //   root r      : dL/ds(r) = -nu * G(r)               (D0 = nu * (1 - s))
-// Subtree sums without a sort: every pixel counts its children; one thread
-// starts at each leaf and climbs while it is the last child to arrive at
-// the parent, so each pixel is finished exactly once, after all of its
-// children.  A pixel's count of children still to arrive and the float sum
-// of their subtree sums share one 64-bit word, updated by a relaxed
-// compare-and-swap: the last child reads the complete sum from the value it
-// wrote itself, so no fence orders two addresses (an acquire-release
-// counter beside a separate sum costs an L2 write-back and invalidate per
-// step).  Most pixels have a single child, and a single child is always
-// the last to arrive: k_child_count also writes an immutable byte per pixel
-// (child count << 4 | predecessor code), and a climb through a pixel with
-// one child reads only that byte (a plain load) and moves on.  The kernels
-// take a table of images (blockIdx.z), so the chains of a whole batch climb
-// in one launch.
+// Subtree sums in three phases over 32 x 32 tiles, so that the pointer
+// chasing runs in LDS (a climb through global memory spends a memory
+// latency, and a 64-byte transaction, on every 1-byte step):
+//   A  per tile: local subtree sums of W over the tile's own edges, to G; a
+//      pixel whose parent lies in another tile is an exit, and its local
+//      sum starts its coarse word in cw; for every tile-border pixel, the
+//      exit (or root) that its in-tile path ends at, to xp;
+//   B  over the exits only: the same climb on the coarse forest in which an
+//      exit's parent is the exit that its real parent's in-tile path ends
+//      at; an exit's coarse word ends as its full subtree sum C;
+//   C  per tile: each border pixel's inflow (C summed over its children in
+//      other tiles) is added along its in-tile path, which completes G.
+// The climbs of A and B are leaf-started: a pixel's count of children
+// still to arrive and the sum of their subtree sums (a double) share one
+// 64-bit word, updated by compare-and-swap, and the last child to arrive
+// continues with the complete sum (read from the value it wrote itself).  Sums and the
+// gradients built from them are kept in double; a subtree can hold a
+// million pixels, and a gradient is a difference of such sums.  The coarse
+// chains are about 64 times shorter than the pixel chains, and the tile
+// climbs run on LDS latencies.  The kernels take a table of images
+// (blockIdx.z), so a whole batch runs in one launch of each.
 #include <hip/hip_runtime.h>
 
 #include "dtk_launch.hpp"
@@ -31,8 +38,16 @@
 
 namespace dtk
 {
-  __device__ constexpr int kOffX[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
-  __device__ constexpr int kOffY[8] = {0, -1, -1, -1, 0, 1, 1, 1};
+  // dt::c8 offsets of predecessor code k, packed like the tile neighbours'.
+  constexpr uint32_t kOffPackX = 0x1A90u, kOffPackY = 0xA901u;
+  __host__ __device__ constexpr int off_x(int k) { return unpack_off(kOffPackX, k); }
+  __host__ __device__ constexpr int off_y(int k) { return unpack_off(kOffPackY, k); }
+  static_assert(off_x(0) == -1 && off_x(1) == -1 && off_x(2) == 0 && off_x(3) == 1 && off_x(4) == 1 &&
+                  off_x(5) == 1 && off_x(6) == 0 && off_x(7) == -1,
+                "dt::c8 x offsets");
+  static_assert(off_y(0) == 0 && off_y(1) == -1 && off_y(2) == -1 && off_y(3) == -1 && off_y(4) == 0 &&
+                  off_y(5) == 1 && off_y(6) == 1 && off_y(7) == 1,
+                "dt::c8 y offsets");
 
   __device__ __forceinline__ float rd_f(const unsigned char* base, long long pitch, int x, int y)
   {
@@ -64,15 +79,15 @@ namespace dtk
 #pragma unroll
       for (int k = 0; k < 8; ++k)
       {
-        const int nx = min(max(x + kOffX[k], 0), d.W - 1), ny = min(max(y + kOffY[k], 0), d.H - 1);
+        const int nx = min(max(x + off_x(k), 0), d.W - 1), ny = min(max(y + off_y(k), 0), d.H - 1);
         Dq[k]        = rd_f(Db, d.D_pitch, nx, ny);
         iq[k]        = load_pix<KIND>(d.img + (long long)ny * d.img_pitch, nx);
       }
 #pragma unroll
       for (int k = 7; k >= 0; --k)
       {
-        const bool diag = kOffX[k] != 0 && kOffY[k] != 0;
-        const int  nx = x + kOffX[k], ny = y + kOffY[k];
+        const bool diag = off_x(k) != 0 && off_y(k) != 0;
+        const int  nx = x + off_x(k), ny = y + off_y(k);
         const bool in = (!diag || c8) && nx >= 0 && ny >= 0 && nx < d.W && ny < d.H;
         const long long qid = (long long)ny * d.W + nx;
         // lambda = 0 has no image term (the forward kernels do not read the
@@ -102,106 +117,405 @@ namespace dtk
     pred_pixel<KIND>(tab[blockIdx.z], p, c8, (int)(blockIdx.x * blockDim.x + threadIdx.x), (int)blockIdx.y);
   }
 
-  // Children count of every pixel (gather, no atomics).
-  __global__ void k_child_count(const BackJob* jobs)
+  // ---- tiled subtree sums --------------------------------------------------
+#ifndef DTK_BACK_TILE
+#define DTK_BACK_TILE 32
+#endif
+  constexpr int      kBT      = DTK_BACK_TILE; // tile side of the subtree sums (32 or 64)
+  constexpr int      kBTPix   = kBT * kBT;
+#ifndef DTK_BACK_THREADS
+#define DTK_BACK_THREADS 256
+#endif
+  constexpr int      kBThreads = DTK_BACK_THREADS;  // threads of a tile block (a multiple of kBT)
+  constexpr int      kBTRows   = kBThreads / kBT;   // tile rows one pass of the block covers
+  static_assert(kBThreads % kBT == 0 && kBTPix % kBThreads == 0, "tile block shape");
+  static_assert(4 * kBT - 4 <= 256 && 4 * kBT - 4 <= kBThreads, "one thread per border position");
+  constexpr int      kBorder  = 4 * kBT - 4; // border positions of a tile
+  constexpr uint32_t kNoExit  = 0xFFFFFFFFu;
+  // Sum words: a double whose lowest mantissa bits are replaced by a count
+  // (the sum keeps 52 - bits of its 52 fraction bits: relative error below
+  // 2^-40, far under the float32 outputs).  Tile words keep the count of
+  // children still to arrive (at most 8) in 4 bits; coarse words keep it in
+  // 11 bits (an exit has fewer than 4 * 64 * 3 coarse children) and bit 11
+  // flags that the exit has any.
+  constexpr unsigned long long kTileCount   = 0xFull;
+  constexpr unsigned long long kCoarseCount = 0x7FFull;
+  constexpr unsigned long long kHasKids     = 0x800ull;
+  constexpr unsigned long long kCoarseLow   = 0xFFFull;
+  __device__ __forceinline__ double word_sum(unsigned long long w, unsigned long long low)
+  {
+    return __longlong_as_double((long long)(w & ~low));
+  }
+  __device__ __forceinline__ unsigned long long sum_bits(double v, unsigned long long low)
+  {
+    return (unsigned long long)__double_as_longlong(v) & ~low;
+  }
+
+  // Border position b (0 .. kBorder-1) of a tile: top row, bottom row, then
+  // the left and right columns without their corners.
+  __device__ __forceinline__ void border_pos(int b, int& lx, int& ly)
+  {
+    if (b < kBT)
+    {
+      lx = b;
+      ly = 0;
+    }
+    else if (b < 2 * kBT)
+    {
+      lx = b - kBT;
+      ly = kBT - 1;
+    }
+    else if (b < 3 * kBT - 2)
+    {
+      lx = 0;
+      ly = b - 2 * kBT + 1;
+    }
+    else
+    {
+      lx = kBT - 1;
+      ly = b - (3 * kBT - 2) + 1;
+    }
+  }
+
+  // Predecessor code of (x, y) in image d: 0..7, or 8 for a root (codes
+  // above 7 and codes that point outside the image are roots).
+  __device__ __forceinline__ int pred_code(const ImgDesc& d, int x, int y)
+  {
+    int c = d.pred[(long long)y * d.pred_pitch + x];
+    if (c >= 8)
+      return 8;
+    const int px = x + off_x(c), py = y + off_y(c);
+    return (px < 0 || py < 0 || px >= d.W || py >= d.H) ? 8 : c;
+  }
+
+  // The tile's pixels: in-image codes (9 outside) and per-pixel values.
+  // Thread t owns the tile pixels t, t + kBThreads, ... (column t % kBT).
+  struct TileGeom
+  {
+    int x0, y0, tw, th, lx, ly0;
+  };
+  __device__ __forceinline__ bool tile_geom(const ImgDesc& d, TileGeom& t)
+  {
+    t.x0 = (int)blockIdx.x * kBT;
+    t.y0 = (int)blockIdx.y * kBT;
+    if (t.x0 >= d.W || t.y0 >= d.H)
+      return false; // the tile lies past this image (the grid covers the largest)
+    t.tw  = min(kBT, d.W - t.x0);
+    t.th  = min(kBT, d.H - t.y0);
+    t.lx  = (int)threadIdx.x & (kBT - 1);
+    t.ly0 = (int)threadIdx.x / kBT;
+    return true;
+  }
+
+  // Phase A for one tile: local subtree sums of W over the tile's own edges
+  // (to G, every pixel; an exit's also to its coarse word in cw) and, for
+  // every border pixel, the exit (or root) that its in-tile path ends at
+  // (to xp).
+  __global__ void __launch_bounds__(kBThreads) k_tile_sums(const BackJob* jobs)
+  {
+    __shared__ unsigned char      s_code[kBTPix]; // 0..7, 8 = root, 9 = outside the image
+    __shared__ double             s_val[kBTPix];  // weight, then the pixel's subtree sum
+    __shared__ unsigned long long s_word[kBTPix]; // sum of the arrived children | count still to arrive
+    __shared__ unsigned char      s_kids[kBTPix]; // children in the tile (fixed: s_word's count runs down)
+    const BackJob& j = jobs[blockIdx.z];
+    const ImgDesc& d = j.d;
+    TileGeom       tg;
+    if (!tile_geom(d, tg))
+      return;
+    const int x0 = tg.x0, y0 = tg.y0, lx = tg.lx, ly0 = tg.ly0;
+    for (int i = 0; i < kBTPix / kBThreads; ++i)
+    {
+      const int ly = ly0 + kBTRows * i, li = ly * kBT + lx;
+      int       c  = 9;
+      double    w  = 0.0;
+      if (lx < tg.tw && ly < tg.th)
+      {
+        c = pred_code(d, x0 + lx, y0 + ly);
+        w = rd_f(j.gd, j.gd_pitch, x0 + lx, y0 + ly);
+      }
+      s_code[li] = (unsigned char)c;
+      s_val[li]  = w;
+    }
+    __syncthreads();
+    // Children inside the tile: pixel c is a child when its code k points
+    // here, c = p - off[k].
+    for (int i = 0; i < kBTPix / kBThreads; ++i)
+    {
+      const int ly = ly0 + kBTRows * i, li = ly * kBT + lx;
+      if (s_code[li] == 9)
+        continue;
+      uint32_t n = 0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+      {
+        const int cx = lx - off_x(k), cy = ly - off_y(k);
+        if (cx >= 0 && cy >= 0 && cx < kBT && cy < kBT)
+          n += s_code[cy * kBT + cx] == k ? 1u : 0u;
+      }
+      s_word[li] = n; // sum +0.0
+      s_kids[li] = (unsigned char)n;
+    }
+    __syncthreads();
+    // Leaf-started climbs, one flat loop per thread: a thread whose climb
+    // ends takes its next leaf at once, so a wave runs about as long as its
+    // longest climb rather than the sum of each pixel slot's longest.
+    {
+      int    slot = 0;  // next of the thread's own pixels to look at
+      int    p    = -1; // the climb's current pixel
+      int    c    = 8;
+      double g    = 0.0;
+      for (;;)
+      {
+        if (p < 0)
+        {
+          while (slot < kBTPix / kBThreads)
+          {
+            const int li = (ly0 + kBTRows * slot) * kBT + lx;
+            ++slot;
+            if (s_code[li] != 9 && s_kids[li] == 0)
+            {
+              p = li;
+              c = s_code[li];
+              g = s_val[li];
+              break;
+            }
+          }
+          if (p < 0)
+            break; // no leaf left
+        }
+        s_val[p] = g; // the pixel's local subtree sum
+        if (c >= 8)
+        {
+          p = -1; // a root
+          continue;
+        }
+        const int px = (p & (kBT - 1)) + off_x(c), py = p / kBT + off_y(c);
+        if (px < 0 || py < 0 || px >= kBT || py >= kBT)
+        {
+          // An exit: its parent lies in another tile.  Its coarse word
+          // starts as the local sum with no coarse child.
+          j.cw[(long long)(y0 + p / kBT) * d.W + x0 + (p & (kBT - 1))] = sum_bits(g, kCoarseLow);
+          p = -1;
+          continue;
+        }
+        const int q = py * kBT + px;
+        // The parent's child count, code and weight do not change until it
+        // is finished (by the thread that arrives last): loaded together.
+        const int    kq = s_kids[q];
+        const int    cq = s_code[q];
+        const double wq = s_val[q];
+        if (kq == 1)
+          g = wq + g; // the only child is the last to arrive: no exchange
+        else
+        {
+          // First guess: no sibling has arrived yet (count kq, sum +0.0).
+          unsigned long long old = (unsigned long long)kq, nw;
+          for (;;)
+          {
+            nw = sum_bits(word_sum(old, kTileCount) + g, kTileCount) | ((old & kTileCount) - 1ull);
+            if (__hip_atomic_compare_exchange_strong(&s_word[q], &old, nw, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP))
+              break;
+          }
+          if ((nw & kTileCount) != 0ull)
+          {
+            p = -1; // a sibling is still climbing: the last one continues
+            continue;
+          }
+          g = wq + word_sum(nw, kTileCount);
+        }
+        p = q;
+        c = cq;
+      }
+    }
+    __syncthreads();
+    for (int i = 0; i < kBTPix / kBThreads; ++i)
+    {
+      const int ly = ly0 + kBTRows * i;
+      if (lx < tg.tw && ly < tg.th)
+        j.G[(long long)(y0 + ly) * d.W + x0 + lx] = s_val[ly * kBT + lx];
+    }
+    if ((int)threadIdx.x < kBorder)
+    {
+      // The end of each border pixel's in-tile path: the exit's image index,
+      // or kNoExit when the path ends at a root (or runs in a cycle, which
+      // no forest from dtk_ggdt_pred has).
+      int bx, by;
+      border_pos((int)threadIdx.x, bx, by);
+      if (bx < tg.tw && by < tg.th)
+      {
+        int      p   = by * kBT + bx;
+        uint32_t out = kNoExit;
+        for (int step = 0; step < kBTPix; ++step)
+        {
+          const int c = s_code[p];
+          if (c >= 8)
+            break;
+          const int px = (p & (kBT - 1)) + off_x(c), py = (p / kBT) + off_y(c);
+          if (px < 0 || py < 0 || px >= kBT || py >= kBT)
+          {
+            out = (uint32_t)(y0 + p / kBT) * (uint32_t)d.W + (uint32_t)(x0 + (p & (kBT - 1)));
+            break;
+          }
+          p = py * kBT + px;
+        }
+        j.xp[(long long)(y0 + by) * d.W + x0 + bx] = out;
+      }
+    }
+  }
+
+  // Phase C for one tile: G = local sum + every inflow on the pixel's
+  // in-tile path from the border.  A border pixel's inflow In is the sum of
+  // C over its children in other tiles (exits, final after phase B); it is
+  // added to each pixel from there up to the path's end.
+  __global__ void __launch_bounds__(kBThreads) k_tile_inflow(const BackJob* jobs)
+  {
+    __shared__ unsigned char s_code[kBTPix];
+    __shared__ double        s_val[kBTPix];
+    const BackJob& j = jobs[blockIdx.z];
+    const ImgDesc& d = j.d;
+    TileGeom       tg;
+    if (!tile_geom(d, tg))
+      return;
+    const int x0 = tg.x0, y0 = tg.y0, lx = tg.lx, ly0 = tg.ly0;
+    for (int i = 0; i < kBTPix / kBThreads; ++i)
+    {
+      const int ly = ly0 + kBTRows * i, li = ly * kBT + lx;
+      int       c  = 9;
+      double    v  = 0.0;
+      if (lx < tg.tw && ly < tg.th)
+      {
+        c = pred_code(d, x0 + lx, y0 + ly);
+        v = j.G[(long long)(y0 + ly) * d.W + x0 + lx];
+      }
+      s_code[li] = (unsigned char)c;
+      s_val[li]  = v;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < kBorder)
+    {
+      int bx, by;
+      border_pos((int)threadIdx.x, bx, by);
+      double in = 0.0;
+      if (bx < tg.tw && by < tg.th)
+      {
+        // Children outside the tile, in dt::c8 order; their codes first.
+        const int x = x0 + bx, y = y0 + by;
+        int       cc[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+        {
+          const int cx = bx - off_x(k), cy = by - off_y(k), gx = x - off_x(k), gy = y - off_y(k);
+          const bool outside = cx < 0 || cy < 0 || cx >= kBT || cy >= kBT;
+          cc[k] = (outside && gx >= 0 && gy >= 0 && gx < d.W && gy < d.H) ? pred_code(d, gx, gy) : 9;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (cc[k] == k)
+            in += word_sum(j.cw[(long long)(y - off_y(k)) * d.W + x - off_x(k)], kCoarseLow);
+      }
+      if (in != 0.0)
+      {
+        int p = by * kBT + bx;
+        for (int step = 0; step < kBTPix; ++step)
+        {
+          __hip_atomic_fetch_add(&s_val[p], in, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          const int c = s_code[p];
+          if (c >= 8)
+            break;
+          const int px = (p & (kBT - 1)) + off_x(c), py = (p / kBT) + off_y(c);
+          if (px < 0 || py < 0 || px >= kBT || py >= kBT)
+            break; // the exit: its parent is in another tile
+          p = py * kBT + px;
+        }
+      }
+    }
+    __syncthreads();
+    for (int i = 0; i < kBTPix / kBThreads; ++i)
+    {
+      const int ly = ly0 + kBTRows * i;
+      if (lx < tg.tw && ly < tg.th)
+        j.G[(long long)(y0 + ly) * d.W + x0 + lx] = s_val[ly * kBT + lx];
+    }
+  }
+
+  // The exit at border position threadIdx.x of this tile, if any: its image
+  // index (*e) and the exit its parent's in-tile path ends at (*t, kNoExit
+  // when that path ends at a root).
+  __device__ __forceinline__ bool exit_at(const BackJob& j, int b, uint32_t* e, uint32_t* t)
+  {
+    const ImgDesc& d  = j.d;
+    const int      x0 = (int)blockIdx.x * kBT, y0 = (int)blockIdx.y * kBT;
+    if (b >= kBorder || x0 >= d.W || y0 >= d.H)
+      return false;
+    int bx, by;
+    border_pos(b, bx, by);
+    const int x = x0 + bx, y = y0 + by;
+    if (x >= d.W || y >= d.H)
+      return false;
+    const int c = pred_code(d, x, y);
+    if (c >= 8)
+      return false;
+    const int px = bx + off_x(c), py = by + off_y(c);
+    if (px >= 0 && py >= 0 && px < kBT && py < kBT)
+      return false; // the parent is in this tile
+    *e = (uint32_t)y * (uint32_t)d.W + (uint32_t)x;
+    *t = j.xp[(long long)(y + off_y(c)) * d.W + x + off_x(c)];
+    if (*t != kNoExit && *t >= (uint32_t)d.W * (uint32_t)d.H)
+      *t = kNoExit; // phase A writes only image indices here; anything else is not followed
+    return true;
+  }
+
+  // Phase B, first step: coarse children per exit (count and flag).
+  __global__ void __launch_bounds__(256) k_coarse_count(const BackJob* jobs)
+  {
+    const BackJob& j = jobs[blockIdx.z];
+    uint32_t       e, t;
+    if (!exit_at(j, (int)threadIdx.x, &e, &t) || t == kNoExit)
+      return;
+    __hip_atomic_fetch_or(&j.cw[t], kHasKids, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(&j.cw[t], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+
+  // Phase B, second step: climbs from the coarse leaves.  An exit's word is
+  // final once its count is 0: its low half is then C, the exit's full
+  // subtree sum.
+  __global__ void __launch_bounds__(256) k_coarse_climb(const BackJob* jobs)
   {
     const BackJob& j = jobs[blockIdx.z];
     const ImgDesc& d = j.d;
-    const int      x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int      y = blockIdx.y;
-    if (x >= d.W || y >= d.H)
+    uint32_t       e, t;
+    if (!exit_at(j, (int)threadIdx.x, &e, &t))
       return;
-    // Child c of p has c + off[k] == p and pred(c) == k; the eight codes
-    // are loaded first, from clamped addresses.
-    unsigned char pc[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-    {
-      const int cx = min(max(x - kOffX[k], 0), d.W - 1), cy = min(max(y - kOffY[k], 0), d.H - 1);
-      pc[k]        = d.pred[(long long)cy * d.pred_pitch + cx];
-    }
-    uint32_t n = 0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-    {
-      const int cx = x - kOffX[k], cy = y - kOffY[k];
-      n += (cx >= 0 && cy >= 0 && cx < d.W && cy < d.H && pc[k] == k) ? 1u : 0u;
-    }
-    j.cw[(long long)y * d.W + x] = (unsigned long long)n << 32; // sum +0.0f
-    int code = d.pred[(long long)y * d.pred_pitch + x];
-    if (code < 8 && (x + kOffX[code] < 0 || y + kOffY[code] < 0 || x + kOffX[code] >= d.W || y + kOffY[code] >= d.H))
-      code = 8; // invalid code: treated as a root
-    j.meta[(long long)y * d.W + x] = (unsigned char)(n << 4 | (uint32_t)code);
-  }
-
-  // Leaf-started climbs computing G.
-  // gx > 0: a one-dimensional grid in which image z runs on XCD z % 8
-  // (workgroups go to the XCDs in turn), gx x gy blocks per image.  The
-  // climbs of one image converge on shared trunks, so keeping an image on
-  // one XCD keeps those lines in one L2 instead of fetching them into all
-  // eight (diff32: 6.4 -> 5.1 ms per backward).
-  __global__ void k_subtree(const BackJob* jobs, int gx, int gy, int n)
-  {
-    int bx = (int)blockIdx.x, by = (int)blockIdx.y, bz = (int)blockIdx.z;
-    if (gx > 0)
-    {
-      const unsigned b = blockIdx.x, xcd = b & 7u, local = b >> 3;
-      const unsigned per = (unsigned)gx * (unsigned)gy, slot = local / per, rem = local - slot * per;
-      bz = (int)(xcd + 8u * slot);
-      if (bz >= n)
-        return;
-      bx = (int)(rem % (unsigned)gx);
-      by = (int)(rem / (unsigned)gx);
-    }
-    const BackJob&       j        = jobs[bz];
-    const ImgDesc&       d        = j.d;
-    const unsigned char* gd       = j.gd;
-    const long long      gd_pitch = j.gd_pitch;
-    unsigned long long*  cw       = j.cw;
-    float*               G        = j.G;
-    int                  x        = bx * blockDim.x + threadIdx.x;
-    int                  y        = by;
-    if (x >= d.W || y >= d.H)
-      return;
-    long long            id   = (long long)y * d.W + x;
-    const unsigned char* meta = j.meta; // immutable here: the words in cw change
-    uint32_t             m    = meta[id];
-    if ((m >> 4) != 0u)
-      return; // not a leaf
-    float g = rd_f(gd, gd_pitch, x, y); // a leaf: G = W
+    const unsigned long long w0 = __hip_atomic_load(&j.cw[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (w0 & kHasKids)
+      return; // not a coarse leaf (the flag does not change here; the count does)
+    double g = word_sum(w0, kCoarseLow);
     for (;;)
     {
-      G[id]          = g;
-      const int code = (int)(m & 15u);
-      if (code >= 8)
-        return; // root (k_child_count marked invalid codes as roots)
-      x += kOffX[code];
-      y += kOffY[code];
-      id = (long long)y * d.W + x;
-      m  = meta[id];
-      if ((m >> 4) == 1u)
-      {
-        // The only child: the last to arrive, and the sum is g alone.
-        g = rd_f(gd, gd_pitch, x, y) + g;
-        continue;
-      }
-      // One fewer child to wait for, g added to the sum.  Every failed
-      // exchange means a sibling's succeeded, and a pixel has at most eight
-      // children, so the loop ends.
-      unsigned long long old = __hip_atomic_load(&cw[id], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (t == kNoExit)
+        return; // the flow ends at a root, inside the parent's tile (phase C adds it there)
+      unsigned long long old = __hip_atomic_load(&j.cw[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       unsigned long long nw;
       for (;;)
       {
-        const uint32_t left = (uint32_t)(old >> 32) - 1u;
-        const float    sum  = __uint_as_float((uint32_t)old) + g;
-        nw                  = ((unsigned long long)left << 32) | __float_as_uint(sum);
-        if (__hip_atomic_compare_exchange_strong(&cw[id], &old, nw, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+        nw = sum_bits(word_sum(old, kCoarseLow) + g, kCoarseLow) | (old & kHasKids) |
+             (((old & kCoarseCount) - 1ull) & kCoarseCount);
+        if (__hip_atomic_compare_exchange_strong(&j.cw[t], &old, nw, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT))
           break;
       }
-      if ((uint32_t)(nw >> 32) != 0u)
-        return; // a sibling is still climbing: the last one continues
-      g = rd_f(gd, gd_pitch, x, y) + __uint_as_float((uint32_t)nw);
+      if ((nw & kCoarseCount) != 0ull)
+        return; // a coarse sibling is still climbing
+      g = word_sum(nw, kCoarseLow);
+      // Continue from exit t: its parent, then the exit that parent's path ends at.
+      const int x = (int)(t % (uint32_t)d.W), y = (int)(t / (uint32_t)d.W);
+      const int c = pred_code(d, x, y); // an exit: c < 8, the parent in another tile
+      if (c >= 8)
+        return;
+      t = j.xp[(long long)(y + off_y(c)) * d.W + x + off_x(c)];
+      if (t >= (uint32_t)d.W * (uint32_t)d.H)
+        return; // kNoExit, or not an image index (phase A writes only those)
     }
   }
 
@@ -217,7 +531,7 @@ namespace dtk
   {
     const BackJob&  j        = jobs[blockIdx.z];
     const ImgDesc&  d        = j.d;
-    const float*    G        = j.G;
+    const double*   G        = j.G;
     unsigned char*  gi       = j.gi;
     const long long gi_pitch = j.gi_pitch;
     unsigned char*  gm       = j.gm;
@@ -229,22 +543,26 @@ namespace dtk
     const long long id   = (long long)y * d.W + x;
     const float     ip   = img_val<IMG_F32>(d, x, y);
     int             code = d.pred[(long long)y * d.pred_pitch + x];
-    if (code < 8 && (x + kOffX[code] < 0 || y + kOffY[code] < 0 || x + kOffX[code] >= d.W || y + kOffY[code] >= d.H))
+    if (code < 8 && (x + off_x(code) < 0 || y + off_y(code) < 0 || x + off_x(code) >= d.W || y + off_y(code) >= d.H))
       code = 8; // invalid code: treated as a root
-    float acc = 0.f;
+    // In double, like the subtree sums: a gradient is a difference of sums
+    // that can be far larger than it; only the output is rounded to float.
+    const double lg  = (double)p.l_grad;
+    double       acc = 0.0;
     if (code < 8)
     {
-      const float iq = img_val<IMG_F32>(d, x + kOffX[code], y + kOffY[code]);
-      acc += G[id] * p.l_grad * sgnf(ip - iq);
+      const float iq = img_val<IMG_F32>(d, x + off_x(code), y + off_y(code));
+      acc += G[id] * lg * (double)sgnf(ip - iq);
     }
     // The eight possible children: codes, G and pixels loaded first from
     // clamped addresses, then summed in k order for the real children.
     unsigned char pc[8];
-    float         gc[8], ic[8];
+    double        gc[8];
+    float         ic[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k)
     {
-      const int cx = min(max(x - kOffX[k], 0), d.W - 1), cy = min(max(y - kOffY[k], 0), d.H - 1);
+      const int cx = min(max(x - off_x(k), 0), d.W - 1), cy = min(max(y - off_y(k), 0), d.H - 1);
       pc[k]        = d.pred[(long long)cy * d.pred_pitch + cx];
       gc[k]        = G[(long long)cy * d.W + cx];
       ic[k]        = img_val<IMG_F32>(d, cx, cy);
@@ -252,13 +570,13 @@ namespace dtk
 #pragma unroll
     for (int k = 0; k < 8; ++k)
     {
-      const int cx = x - kOffX[k], cy = y - kOffY[k];
+      const int cx = x - off_x(k), cy = y - off_y(k);
       if (cx >= 0 && cy >= 0 && cx < d.W && cy < d.H && pc[k] == k)
-        acc -= gc[k] * p.l_grad * sgnf(ic[k] - ip);
+        acc -= gc[k] * lg * (double)sgnf(ic[k] - ip);
     }
-    reinterpret_cast<float*>(gi + (long long)y * gi_pitch)[x] = acc;
+    reinterpret_cast<float*>(gi + (long long)y * gi_pitch)[x] = (float)acc;
     if (gm)
-      reinterpret_cast<float*>(gm + (long long)y * gm_pitch)[x] = (code >= 8) ? -p.v * G[id] : 0.f;
+      reinterpret_cast<float*>(gm + (long long)y * gm_pitch)[x] = (code >= 8) ? (float)(-(double)p.v * G[id]) : 0.f;
   }
 
   hipError_t launch_pred_batched(const ImgDesc* tab, int n, int maxW, int maxH, const KParams& p, bool c8,
@@ -309,15 +627,22 @@ namespace dtk
     if (n <= 0 || maxW <= 0 || maxH <= 0)
       return hipSuccess;
     const dim3 grid((maxW + 255) / 256, maxH, n), blk(256);
-    hipLaunchKernelGGL(k_child_count, grid, blk, 0, s, jobs);
+    // Each kernel reads what the one before wrote: stop at the first failed launch.
+    unsigned gx = 0, gy = 0;
+    if (!backward_tiles(n, maxW, maxH, kBT, &gx, &gy))
+      return hipErrorInvalidConfiguration;
+    const dim3 tiles(gx, gy, n);
+    hipLaunchKernelGGL(k_tile_sums, tiles, dim3(kBThreads), 0, s, jobs);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
       return e;
-    // Each kernel reads what the one before wrote: stop at the first failed launch.
-    if (const unsigned long long blocks = subtree_grid_1d(n, maxW, maxH))
-      hipLaunchKernelGGL(k_subtree, dim3((unsigned)blocks), blk, 0, s, jobs, (int)grid.x, (int)grid.y, n);
-    else
-      hipLaunchKernelGGL(k_subtree, grid, blk, 0, s, jobs, 0, 0, n);
+    hipLaunchKernelGGL(k_coarse_count, tiles, blk, 0, s, jobs);
+    if ((e = hipGetLastError()) != hipSuccess)
+      return e;
+    hipLaunchKernelGGL(k_coarse_climb, tiles, blk, 0, s, jobs);
+    if ((e = hipGetLastError()) != hipSuccess)
+      return e;
+    hipLaunchKernelGGL(k_tile_inflow, tiles, dim3(kBThreads), 0, s, jobs);
     if ((e = hipGetLastError()) != hipSuccess)
       return e;
     hipLaunchKernelGGL(k_grad, grid, blk, 0, s, jobs, p);

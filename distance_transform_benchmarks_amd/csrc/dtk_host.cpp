@@ -733,13 +733,15 @@ namespace
       if (preds[i].width != W || preds[i].height != H || gds[i].width != W || gds[i].height != H ||
           gis[i].width != W || gis[i].height != H || (have_gm && (gms[i].width != W || gms[i].height != H)))
         return fail(DTK_ERR_INVALID_ARGUMENT, "gradient image sizes differ");
-      if (H > 65535)
-        return fail(DTK_ERR_UNSUPPORTED, "the adjoint takes at most 65535 rows");
+      if ((unsigned long long)W * (unsigned long long)H >= 0xFFFFFFFFull)
+        return fail(DTK_ERR_UNSUPPORTED, "the adjoint takes images of fewer than 2^32 - 1 pixels");
+      if (H > 65535 * 64)
+        return fail(DTK_ERR_UNSUPPORTED, "the adjoint takes at most 4194240 rows");
       if (n > 65535)
         return fail(DTK_ERR_UNSUPPORTED, "the adjoint takes at most 65535 images per call");
       maxW = std::max(maxW, W);
       maxH = std::max(maxH, H);
-      dbytes += align_up((size_t)W * H * 4, 256) + align_up((size_t)W * H * 8, 256) + align_up((size_t)W * H, 256);
+      dbytes += 2 * align_up((size_t)W * H * 8, 256) + align_up((size_t)W * H * 4, 256);
     }
     if (n == 0 || maxW == 0 || maxH == 0)
       return DTK_OK;
@@ -774,9 +776,9 @@ namespace
       j.gi_pitch     = gis[i].pitch;
       j.gm           = have_gm && soft ? static_cast<unsigned char*>(gms[i].data) : nullptr;
       j.gm_pitch     = have_gm ? gms[i].pitch : 0;
-      j.G            = c.take<float>(npx);
+      j.G            = c.take<double>(npx);
       j.cw           = c.take<unsigned long long>(npx);
-      j.meta         = c.take<unsigned char>(npx);
+      j.xp           = c.take<uint32_t>(npx);
       hjobs[i]       = j;
     }
     DTK_HIP(hipMemcpyAsync(djobs, hjobs, (size_t)n * sizeof(BackJob), hipMemcpyHostToDevice, s), "upload jobs");

@@ -52,9 +52,9 @@ namespace dtk
 
   // One image of an adjoint launch (dtk_ggdt_backward[_batched]): its
   // inputs (d: image, mask, D, predecessor codes), dL/dD, the outputs and
-  // its scratch (subtree sums G; per pixel a word holding the count of
-  // children still to arrive (high 32 bits) and the float sum of their
-  // subtree sums so far (low 32 bits)), all device.
+  // its scratch (subtree sums G, the coarse words of the pixels whose
+  // predecessor lies in another tile, and the tile-border pixels' path
+  // ends), all device.
   struct BackJob
   {
     ImgDesc              d;
@@ -64,9 +64,9 @@ namespace dtk
     long long            gi_pitch;
     unsigned char*       gm; // NULL: no dL/dmask
     long long            gm_pitch;
-    float*               G;
-    unsigned long long*  cw;
-    unsigned char*       meta; // per pixel: child count << 4 | predecessor code
+    double*              G;  // subtree sums
+    unsigned long long*  cw; // per pixel; used at exits: their coarse words
+    uint32_t*            xp; // per pixel; used at tile borders: where the in-tile path ends
   };
 
   // A call: one image (d0) or a batch described by a device table.

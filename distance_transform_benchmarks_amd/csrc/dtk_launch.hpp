@@ -37,20 +37,21 @@ namespace dtk
                                  hipStream_t s);
   hipError_t launch_backward(const BackJob* jobs, int n, int maxW, int maxH, const KParams& p, hipStream_t s);
 
-  // Grid of the adjoint's subtree-sum kernel for n images of at most maxW x
-  // maxH pixels (256-thread blocks, gx = ceil(maxW / 256) by maxH blocks per
-  // image).  From eight images on it is one-dimensional, image z on XCD
-  // z % 8: 8 * ceil(n / 8) * gx * maxH blocks; HIP bounds a dimension's
-  // blocks times threads below 2^32, so beyond that (and below eight
-  // images) the grid is gx x maxH x n.  Returns the x extent of the 1-D grid,
-  // or 0 for the 3-D grid.
-  inline unsigned long long subtree_grid_1d(int n, int maxW, int maxH)
+  // Grid of the adjoint's tile kernels for n images of at most maxW x maxH
+  // pixels: one 256-thread block per tile x tile pixels, gx x gy tiles per
+  // image, images along z.  False when HIP cannot launch it (z and y at
+  // most 65535 blocks; blocks times threads along x below 2^32).
+  inline bool backward_tiles(int n, int maxW, int maxH, int tile, unsigned* gx, unsigned* gy)
   {
-    if (n < 8 || maxW <= 0 || maxH <= 0)
-      return 0ull;
-    const unsigned long long gx = (unsigned long long)(maxW + 255) / 256ull, slots = (unsigned long long)(n + 7) / 8ull;
-    const unsigned long long blocks = 8ull * slots * gx * (unsigned long long)maxH;
-    return blocks * 256ull <= 0xffffffffull ? blocks : 0ull;
+    if (n < 1 || maxW < 1 || maxH < 1 || n > 65535 || tile < 1)
+      return false;
+    const unsigned long long x = ((unsigned long long)maxW + (unsigned)tile - 1ull) / (unsigned)tile;
+    const unsigned long long y = ((unsigned long long)maxH + (unsigned)tile - 1ull) / (unsigned)tile;
+    if (y > 65535ull || x * 256ull > 0xffffffffull)
+      return false;
+    *gx = (unsigned)x;
+    *gy = (unsigned)y;
+    return true;
   }
 
   // dtk_selftest.hip

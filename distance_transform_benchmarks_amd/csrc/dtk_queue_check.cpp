@@ -403,27 +403,37 @@ namespace
   }
 } // namespace
 
-// Launch shapes that the host chooses without a device (dtk_launch.hpp).
+// Launch shapes that the host chooses without a device (dtk_launch.hpp):
+// the grid of the adjoint's tile kernels.
 int check_launch_shapes()
 {
   int bad = 0;
-  auto expect = [&](int n, int W, int H, unsigned long long want) {
-    const unsigned long long got = subtree_grid_1d(n, W, H);
-    if (got != want)
+  auto expect = [&](int n, int W, int H, bool ok, unsigned wx, unsigned wy) {
+    unsigned   gx = 0, gy = 0;
+    const bool got = backward_tiles(n, W, H, 64, &gx, &gy);
+    if (got != ok || (ok && (gx != wx || gy != wy)))
     {
-      std::printf("FAIL subtree grid n=%d %dx%d: %llu, expected %llu\n", n, W, H, got, want);
+      std::printf("FAIL backward tiles n=%d %dx%d: %d %ux%u, expected %d %ux%u\n", n, W, H, (int)got, gx, gy, (int)ok,
+                  wx, wy);
       ++bad;
     }
   };
-  expect(7, 1024, 1024, 0);                   // fewer than eight images: 3-D grid
-  expect(32, 1024, 1024, 8ull * 4 * 4 * 1024); // diff32: one dimension, 131072 blocks
-  expect(9, 300, 200, 8ull * 2 * 2 * 200);     // empty slots of the last XCD round
-  // 8 * slots * gx * H blocks of 256 threads must stay below 2^32 threads,
-  // i.e. at most 2^24 - 1 blocks.
-  expect(1016, 2048, 2048, 8ull * 127 * 8 * 2048); // 16 646 144 blocks: one dimension
-  expect(1024, 2048, 2048, 0);                     // 2^24 blocks, 2^32 threads: 3-D grid
-  expect(256, 4096, 4096, 0);                      // one 4096^2 image among 256: 2^24 blocks
-  expect(8, 256, 65535, 8ull * 1 * 1 * 65535);     // the tallest image the adjoint takes
+  expect(32, 1024, 1024, true, 16, 16);      // diff32
+  expect(1, 1000, 700, true, 16, 11);        // partial tiles round up
+  expect(1, 1, 65535 * 64, true, 1, 65535);  // the tallest image: 65535 tile rows
+  expect(1, 1, 65535 * 64 + 1, false, 0, 0); // one row more needs a 65536th
+  expect(65535, 64, 64, true, 1, 1);         // the most images per launch
+  expect(65536, 64, 64, false, 0, 0);
+  expect(0, 64, 64, false, 0, 0);
+  {
+    // the tile side sets the grid: 32-pixel tiles need twice the blocks per side
+    unsigned gx = 0, gy = 0;
+    if (!backward_tiles(2, 1000, 700, 32, &gx, &gy) || gx != 32 || gy != 22)
+    {
+      std::printf("FAIL backward tiles with 32-pixel tiles: %ux%u\n", gx, gy);
+      ++bad;
+    }
+  }
   std::printf(bad ? "launch shape checks FAILED\n" : "launch shape checks passed\n");
   return bad;
 }

@@ -214,7 +214,7 @@ namespace dtk
     float nmin = q.tmin[t];
     for (int k = 0; k < 8; ++k)
     {
-      const int nx = tx + kNbrDX[k], ny = ty + kNbrDY[k];
+      const int nx = tx + nbr_dx(k), ny = ty + nbr_dy(k);
       if (nx < 0 || ny < 0 || nx >= d.tiles_x || ny >= d.tiles_y)
         continue;
       nmin = fminf(nmin, q.tmin[d.tile0 + ny * d.tiles_x + nx]);
@@ -607,7 +607,7 @@ namespace dtk
     uint32_t  bucket = 0;
     if (lane < 8)
     {
-      const int nx = tx + kNbrDX[lane], ny = ty + kNbrDY[lane];
+      const int nx = tx + nbr_dx(lane), ny = ty + nbr_dy(lane);
       if (((border >> lane) & 1u) && nx >= 0 && ny >= 0 && nx < d.tiles_x && ny < d.tiles_y)
       {
         target      = d.tile0 + ny * d.tiles_x + nx;
@@ -792,7 +792,7 @@ namespace dtk
       {
         if (!((vr.border >> k) & 1u))
           continue;
-        const int nx = tx + kNbrDX[k], ny = ty + kNbrDY[k];
+        const int nx = tx + nbr_dx(k), ny = ty + nbr_dy(k);
         if (nx < 0 || ny < 0 || nx >= d.tiles_x || ny >= d.tiles_y)
           continue;
         next[d.tile0 + ny * d.tiles_x + nx] = 1;
@@ -815,7 +815,7 @@ namespace dtk
     float nmin = q.tmin[t];
     for (int k = 0; k < 8; ++k)
     {
-      const int nx = tx + kNbrDX[k], ny = ty + kNbrDY[k];
+      const int nx = tx + nbr_dx(k), ny = ty + nbr_dy(k);
       if (nx < 0 || ny < 0 || nx >= d.tiles_x || ny >= d.tiles_y)
         continue;
       nmin = fminf(nmin, q.tmin[d.tile0 + ny * d.tiles_x + nx]);

@@ -205,7 +205,7 @@ namespace dtk
 
   struct VisitResult
   {
-    uint32_t border;   // bit k: this visit improved a pixel of neighbour k (see kNbrDX/kNbrDY)
+    uint32_t border;   // bit k: this visit improved a pixel of neighbour k (see nbr_dx/nbr_dy)
     int      sweeps;   // sweeps run
     bool     converged;
     bool     changed;  // any owned pixel changed
@@ -215,8 +215,19 @@ namespace dtk
   };
 
   // Neighbour tiles in border-bit order: L, R, T, B, TL, TR, BL, BR.
-  __device__ constexpr int kNbrDX[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
-  __device__ constexpr int kNbrDY[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+  // Offsets of a direction k (0..7) from a table packed two bits per entry
+  // (offset + 1): looking one up by a per-lane k is a shift, where an
+  // indexed __device__ array would be a load from memory.
+  __host__ __device__ constexpr int unpack_off(uint32_t packed, int k) { return (int)((packed >> (2 * k)) & 3u) - 1; }
+  constexpr uint32_t kNbrPackX = 0x8858u, kNbrPackY = 0xA085u;
+  __host__ __device__ constexpr int nbr_dx(int k) { return unpack_off(kNbrPackX, k); }
+  __host__ __device__ constexpr int nbr_dy(int k) { return unpack_off(kNbrPackY, k); }
+  static_assert(nbr_dx(0) == -1 && nbr_dx(1) == 1 && nbr_dx(2) == 0 && nbr_dx(3) == 0 && nbr_dx(4) == -1 &&
+                  nbr_dx(5) == 1 && nbr_dx(6) == -1 && nbr_dx(7) == 1,
+                "neighbour x offsets: L, R, T, B, TL, TR, BL, BR");
+  static_assert(nbr_dy(0) == 0 && nbr_dy(1) == 0 && nbr_dy(2) == -1 && nbr_dy(3) == 1 && nbr_dy(4) == -1 &&
+                  nbr_dy(5) == -1 && nbr_dy(6) == 1 && nbr_dy(7) == 1,
+                "neighbour y offsets: L, R, T, B, TL, TR, BL, BR");
 
   template <int RH>
   __device__ __forceinline__ const float* drow(const ImgDesc& d, int y)

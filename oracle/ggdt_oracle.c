@@ -517,6 +517,52 @@ int ora_raster_f64(const double* img, const double* smask, double* D, int W, int
   return iters;
 }
 
+/* L = sum(Wt * D_f64) at the unperturbed inputs and, for each pixel in
+ * idx[0..m), with its image value moved by +eps and -eps and its soft-mask
+ * value by +eps_mask and -eps_mask: out[5k + 0..3] = L(I+), L(I-), L(s+),
+ * L(s-); out[5k + 4] = L (the same for every k).  The one-sided slopes tell
+ * whether a step straddles a kink of the piecewise-linear map. */
+void ora_fd_sides(const float* img, const float* smask, const float* Wt, int W, int H, double lambda, double nu,
+                  int conn8, double eps, double eps_mask, const long* idx, int m, double* out)
+{
+  const long n = (long)W * H;
+  if (n <= 0)
+    return;
+  double* I = (double*)calloc((size_t)n, sizeof(double));
+  double* S = (double*)calloc((size_t)n, sizeof(double));
+  double* D = (double*)calloc((size_t)n, sizeof(double));
+  for (long i = 0; i < n; ++i)
+  {
+    I[i] = img[i];
+    S[i] = smask[i];
+  }
+  double l0 = 0;
+  ora_raster_f64(I, S, D, W, H, lambda, nu, conn8);
+  for (long i = 0; i < n; ++i)
+    l0 += (double)Wt[i] * D[i];
+  for (int k = 0; k < m; ++k)
+  {
+    const long p = idx[k];
+    for (int side = 0; side < 4; ++side)
+    {
+      double*      v  = side < 2 ? &I[p] : &S[p];
+      const double v0 = *v;
+      const double st = side < 2 ? eps : eps_mask;
+      *v              = (side % 2 == 0) ? v0 + st : v0 - st;
+      ora_raster_f64(I, S, D, W, H, lambda, nu, conn8);
+      double l = 0;
+      for (long i = 0; i < n; ++i)
+        l += (double)Wt[i] * D[i];
+      out[5 * k + side] = l;
+      *v                = v0;
+    }
+    out[5 * k + 4] = l0;
+  }
+  free(I);
+  free(S);
+  free(D);
+}
+
 /* Central differences of L = sum(Wt * D_f64) w.r.t. image pixels and mask
  * pixels listed in idx[0..m).  out_img[k], out_mask[k] receive dL/dI, dL/ds. */
 void ora_fd_grad(const float* img, const float* smask, const float* Wt, int W, int H, double lambda, double nu,

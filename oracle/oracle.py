@@ -51,6 +51,7 @@ def _load():
         lib.ora_raster_f64.argtypes = [vp, vp, vp, i, i, dp, dp, i]
         lib.ora_raster_f64.restype = i
         lib.ora_fd_grad.argtypes = [fp, fp, fp, i, i, dp, dp, i, dp, dp, vp, i, vp, vp]
+        lib.ora_fd_sides.argtypes = [fp, fp, fp, i, i, dp, dp, i, dp, dp, vp, i, vp]
         _lib = lib
     return _lib
 
@@ -171,6 +172,34 @@ def fd_grad(img, smask, Wt, lam: float, nu: float, idx, eps: float = 1e-4, conn8
     _load().ora_fd_grad(_ptr(img), _ptr(smask), _ptr(Wt), W, H, lam, nu, int(conn8), eps, eps_mask, _ptr(idx), len(idx),
                         _ptr(gi), _ptr(gm))
     return gi, gm
+
+
+def fd_kinkfree(img, smask, Wt, lam: float, nu: float, idx, eps: float = 1e-4, conn8: bool = True,
+                eps_mask: float = None, slope_rtol: float = 1e-3):
+    """Central differences of sum(Wt*D) at steps eps (image) and eps_mask
+    (soft mask), with the pixels where each step stays on one linear piece.
+    D is piecewise linear in every input, so when the one-sided slopes
+    (L(+) - L) / eps and (L - L(-)) / eps agree to slope_rtol, the central
+    difference is the derivative there.  Returns (fd_img, ok_img, fd_mask,
+    ok_mask)."""
+    if eps_mask is None:
+        eps_mask = eps
+    img = np.ascontiguousarray(img, np.float32)
+    smask = np.ascontiguousarray(smask, np.float32)
+    Wt = np.ascontiguousarray(Wt, np.float32)
+    idx = np.ascontiguousarray(idx, np.int64)
+    H, W = img.shape
+    out = np.empty((len(idx), 5), np.float64)
+    _load().ora_fd_sides(_ptr(img), _ptr(smask), _ptr(Wt), W, H, lam, nu, int(conn8), eps, eps_mask, _ptr(idx),
+                         len(idx), _ptr(out))
+    l0 = out[:, 4]
+    res = []
+    for plus, minus, st in ((out[:, 0], out[:, 1], eps), (out[:, 2], out[:, 3], eps_mask)):
+        up, down = (plus - l0) / st, (l0 - minus) / st
+        fd = (plus - minus) / (2 * st)
+        ok = np.abs(up - down) <= slope_rtol * np.maximum(np.abs(fd), 1e-3 * max(1.0, np.abs(fd).max()))
+        res += [fd, ok]
+    return tuple(res)
 
 
 # ---- display of a distance map (numpy restatements) --------------------------

@@ -8,7 +8,8 @@ Bars:
     adjoint (same predecessor forest; only float accumulation order differs);
   - within 1e-3 relative of FP64 central differences on 64x64 crops
     (steps 1e-7 on the image and 1e-10 on the mask, below the inputs' tie
-    margins, see test_oracle.py).
+    margins, see test_oracle.py), and at the workload's step 1e-4 on the
+    pixels where that step stays on one linear piece.
 """
 from __future__ import annotations
 
@@ -69,6 +70,11 @@ def test_adjoint_vs_fd_on_crops(dt, dev, oracle, k):
     fi, fm = oracle.fd_grad(I, M, Wt, 1.0, 1e3, idx, eps=FD_EPS_IMG, eps_mask=FD_EPS_MASK)
     assert_rel(gi.ravel()[idx], fi, 1e-3, "dL/dimage vs FD")
     assert_rel(gm.ravel()[idx], fm, 1e-3, "dL/dmask vs FD")
+    # the workload's step, 1e-4, on the pixels where it stays on one linear piece
+    fi4, oki, fm4, okm = oracle.fd_kinkfree(I, M, Wt, 1.0, 1e3, idx, eps=1e-4)
+    assert oki.sum() >= 0.3 * len(idx) and okm.sum() >= 0.3 * len(idx)
+    assert_rel(gi.ravel()[idx][oki], fi4[oki], 1e-3, "dL/dimage vs FD, eps 1e-4, kink-free pixels")
+    assert_rel(gm.ravel()[idx][okm], fm4[okm], 1e-3, "dL/dmask vs FD, eps 1e-4, kink-free pixels")
 
 
 def test_adjoint_full_size_batch(dt, dev, oracle):
@@ -123,10 +129,40 @@ def test_autograd_equals_direct_adjoint(dt, dev):
     torch.cuda.synchronize()
 
 
+@pytest.mark.parametrize("lam", [1.0, 0.6])
+def test_adjoint_chains_across_tiles(dt, dev, oracle, lam):
+    """Subtree sums run per 64 x 64 tile and then over the pixels whose
+    predecessor lies in another tile.  A spiral maze (389 x 389, not a
+    multiple of 64) with one seed at the centre makes every chain wind
+    through many tiles, crossing their edges and corners; the gradients
+    match the CPU reference adjoint to 1e-4 of the largest one."""
+    import torch
+
+    from distance_transform_benchmarks_amd import synth
+
+    maze, (sx, sy) = synth.spiral_maze(389, wall=3, pitch=12)
+    rng = np.random.default_rng(5)
+    I = maze.astype(np.float32) / 255.0 + 1e-3 * rng.normal(size=maze.shape).astype(np.float32)
+    M = np.zeros(maze.shape, np.float32)
+    M[sy, sx] = 1.0
+    Wt = rng.random(maze.shape).astype(np.float32)
+    _say("adjoint spiral maze", lam)
+    D, P, gi, gm = _gpu_adjoint(dt, dev, I, M, Wt, lam=lam)
+    Dr, _ = oracle.raster(I, M, lam, 1e3)
+    assert_bit_exact(D, Dr, "forward")
+    Pr = oracle.pred(I, M, Dr, lam, 1e3)
+    assert np.array_equal(P, Pr)
+    gir, gmr, G = oracle.backward(I, Dr, Pr, Wt, lam, 1e3)
+    # the forest really has long chains: the seed's subtree holds most pixels
+    # (at lambda = 1 the chains follow the corridor, up to ~7900 pixels long)
+    assert G.max() >= 0.5 * Wt.astype(np.float64).sum()
+    for got, want, what in ((gi, gir, "dL/dimage"), (gm, gmr, "dL/dmask")):
+        err = np.abs(got.astype(np.float64) - want).max()
+        assert err <= 1e-4 * max(1.0, np.abs(want).max()), (what, err)
+
+
 SHAPES_4 = ((80, 64), (50, 120), (33, 33), (1, 17))
-# Nine images: from eight on, the subtree kernel runs each image's blocks on
-# one XCD (a one-dimensional grid), and the ninth leaves slots of its round
-# of eight empty.
+# Nine uneven images, several of them one 64 x 64 tile or less.
 SHAPES_9 = SHAPES_4 + ((64, 64), (17, 90), (120, 40), (2, 2), (70, 71))
 
 

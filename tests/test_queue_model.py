@@ -6,8 +6,8 @@ threads over a random wake graph, for the FIFO, KEY and SEED orders (with
 deferral windows of 0 and 3 buckets).  It checks that tile ids and buckets read back
 from the rings stay in range, that no tile is visited by two workers at
 once, that no wake is lost and that the workers terminate.  It also checks,
-without launching anything, that the adjoint's one-dimensional subtree grid
-is used only while its block count times 256 threads stays below 2^32.  The queue's
+without launching anything, that the grid of the adjoint's tile kernels is
+accepted only within HIP's launch limits.  The queue's
 replacement of the reference's task queue is dt/src/task_queue.cuh:20-150.
 """
 from __future__ import annotations
@@ -27,7 +27,7 @@ def test_queue_protocol_model_check():
     r = subprocess.run([CHECK, "3"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-3000:]
     assert "queue model check passed" in r.stdout
-    # the adjoint's subtree grid stays within HIP's 2^32 threads per dimension
+    # the adjoint's tile grid stays within HIP's launch limits
     assert "launch shape checks passed" in r.stdout
     # every order, both claim modes and the multi-thread cases ran
     for order in (0, 1, 2):
