@@ -55,7 +55,7 @@ class DtkStats(ctypes.Structure):
                 ("cycles_setup", ctypes.c_uint32), ("cycles_sweep", ctypes.c_uint32), ("cycles_store", ctypes.c_uint32),
                 ("cycles_release", ctypes.c_uint32), ("cycles_acquire", ctypes.c_uint32),
                 ("drains", ctypes.c_uint32), ("moved", ctypes.c_uint32), ("deferred", ctypes.c_uint32),
-                ("cycles_drain", ctypes.c_uint32)]
+                ("cycles_drain", ctypes.c_uint32), ("kept", ctypes.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -601,6 +601,7 @@ namespace
     st->moved          = ctrl_host ? ctrl_host[CTRL_MOVED] : 0;
     st->deferred       = ctrl_host ? ctrl_host[CTRL_DEFER] : 0;
     st->cycles_drain   = ctrl_host ? ctrl_host[CTRL_T_DRAIN] : 0;
+    st->kept           = ctrl_host ? ctrl_host[CTRL_KEPT] : 0;
     return DTK_OK;
   }
 

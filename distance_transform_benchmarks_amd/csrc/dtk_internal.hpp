@@ -117,6 +117,7 @@ namespace dtk
     CTRL_VISITS  = 112,
     CTRL_SWEEPS  = 113,
     CTRL_CHANGED = 114,
+    CTRL_KEPT    = 115, // task: visits that followed a visit of the same tile at once (end_visit's keep)
     CTRL_T_WAIT  = 116, // summed s_memtime cycles / 1024 of the task workers:
     CTRL_T_VISIT = 117, //   waiting for a tile, solving tiles,
     CTRL_T_PUB   = 118, //   publishing (release, wake-ups, re-queue)
@@ -152,7 +153,8 @@ namespace dtk
   //   0                  idle
   //   TILE_DIRTY         queued (in the ring, not yet popped)
   //   TILE_RUNNING       being solved by one wave
-  //   RUNNING|DIRTY      being solved, woken again meanwhile (re-queued at the end)
+  //   RUNNING|DIRTY      being solved, woken again meanwhile (re-queued at the end,
+  //                      or visited again at once by the same worker)
   // A waker does fetch_or(TILE_DIRTY); the one that sees 0 inserts the tile.
   enum TileState : uint32_t
   {
@@ -211,6 +213,18 @@ namespace dtk
 #define DTK_STATE_STRIDE 1
 #endif
   constexpr int kStateStride = DTK_STATE_STRIDE;
+
+// Visits in a row that one worker may add to a tile it keeps (a tile woken
+// during its own visit is visited again at once by the same worker, with its
+// edge costs still in registers).  The bound keeps the ticket the worker
+// claimed meanwhile from waiting on a long run of such visits.  4096^2 RGB8,
+// ms at lambda 0 / 0.5 / 1: 0.57 / 1.34 / 1.41 with 4, 0.59 / 1.36 / 1.45
+// with 16, 0.59 / 1.38 / 1.42 with 64 (1.55 / 1.61 at lambda > 0 without
+// keeping).
+#ifndef DTK_MAX_KEEPS
+#define DTK_MAX_KEEPS 4
+#endif
+  constexpr uint32_t kMaxKeeps = DTK_MAX_KEEPS;
 
 #ifndef DTK_POLL_NAPS
 #define DTK_POLL_NAPS 2

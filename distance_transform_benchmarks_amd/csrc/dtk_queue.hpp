@@ -14,7 +14,8 @@
 //   0                  idle
 //   TILE_DIRTY         queued (in a ring, not yet popped)
 //   TILE_RUNNING       being solved by one worker
-//   RUNNING|DIRTY      being solved, woken again meanwhile (re-queued at the end)
+//   RUNNING|DIRTY      being solved, woken again meanwhile (re-queued at the end,
+//                      or visited again at once by the same worker: end_visit's keep)
 // A waker does fetch_or(TILE_DIRTY); the one that sees 0 inserts the tile,
 // so a tile is in the queue at most once and never solved by two workers.
 //
@@ -273,16 +274,36 @@ namespace dtk
   // wake arrived meanwhile (DIRTY stays set) or when the visit stopped at
   // its sweep cap (`requeue`).  own_key: the key the visit was popped with.
   // Returns true when the caller must insert t.
-  DTK_HD bool end_visit(const Queue& q, int t, bool requeue, uint32_t own_key, uint32_t* bucket)
+  //
+  // Keeping (keep_ok): a wake that arrived during the visit is absorbed by
+  // visiting the tile again at once instead of queueing it.  The and above
+  // left the state DIRTY alone: no ring holds the tile, and wakers see a
+  // non-zero state and insert nothing, so only this owner can change it; the
+  // exchange makes it RUNNING again (*keep = true, nothing to insert, the
+  // tile's pending item stays).  ORDER_KEY: *own_key receives the key of the
+  // absorbed wakes.  The caller needs an acquire fence before the next
+  // visit's loads, as after a pop (the and read the wakers' RMWs).
+  DTK_HD bool end_visit(const Queue& q, int t, bool requeue, uint32_t own_key, uint32_t* bucket, bool keep_ok = false,
+                        bool* keep = nullptr, uint32_t* new_key = nullptr)
   {
     uint32_t*      st  = &q.state[(size_t)t * kStateStride];
     const uint32_t old = at_and(st, ~(uint32_t)TILE_RUNNING);
     bool           ins = false;
+    *bucket            = 0;
+    if (keep)
+      *keep = false;
+    if ((old & TILE_DIRTY) && keep_ok)
+    {
+      at_xchg(st, (uint32_t)TILE_RUNNING);
+      *keep = true;
+      if (new_key)
+        *new_key = q.order == ORDER_KEY ? at_xchg(&q.tkey[t], kInfBits) : kInfBits;
+      return false;
+    }
     if (old & TILE_DIRTY)
       ins = true;
     else if (requeue)
       ins = at_or(st, (uint32_t)TILE_DIRTY) == 0u;
-    *bucket = 0;
     if (!ins)
       return false;
     if (q.order == ORDER_KEY)
@@ -304,6 +325,12 @@ namespace dtk
   DTK_HD unsigned long long publish_count(const Queue& q, uint32_t n_main, uint32_t n_all)
   {
     return at_add(tp_word(q), (unsigned long long)n_main + ((unsigned long long)n_all << 32) + kMinusOnePending);
+  }
+  // The same for a visit whose tile is kept (end_visit's keep): its item stays
+  // pending, so pending cannot reach 0 here.
+  DTK_HD unsigned long long publish_count_keep(const Queue& q, uint32_t n_main, uint32_t n_all)
+  {
+    return at_add(tp_word(q), (unsigned long long)n_main + ((unsigned long long)n_all << 32));
   }
 
   DTK_HD void raise_done(const Queue& q) { at_store(&q.ctrl[CTRL_DONE], 1u); }

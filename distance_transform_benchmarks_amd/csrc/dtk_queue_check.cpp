@@ -11,7 +11,9 @@
 //     queue flags ERR_QUEUE otherwise), and no wait hits its spin limit;
 //   - no tile is visited by two workers at once;
 //   - no wake is lost: every tile's last visit starts after its last wake;
-//   - the workers terminate: DONE is raised exactly when pending reaches 0.
+//   - the workers terminate: DONE is raised exactly when pending reaches 0;
+// each case runs with and without workers keeping a tile that was woken
+// during its visit (end_visit's keep).
 //
 // Usage: dtk_queue_check [iterations]   (exit status 0 = all checks passed)
 #include <hip/hip_runtime.h>
@@ -72,7 +74,8 @@ namespace
     std::vector<std::atomic<uint32_t>> wakes;      // wakes issued to a tile
     std::vector<std::atomic<uint32_t>> covered;    // wakes seen at the start of its latest visit
     std::vector<std::atomic<int>>    budget;       // visits that may still wake neighbours
-    std::atomic<uint32_t>            visits{0}, faults{0};
+    std::atomic<uint32_t>            visits{0}, faults{0}, kept{0};
+    bool                             keeps = false; // workers may keep a tile woken during its visit
   };
 
   uint32_t next_pow2(uint32_t v)
@@ -83,8 +86,9 @@ namespace
     return p;
   }
 
-  void setup(Model& m, int tx, int ty, int workers, int order, int window, uint32_t seed)
+  void setup(Model& m, int tx, int ty, int workers, int order, int window, bool keeps, uint32_t seed)
   {
+    m.keeps = keeps;
     m.tx = tx;
     m.ty = ty;
     m.ntiles  = tx * ty;
@@ -130,6 +134,7 @@ namespace
       m.budget[t] = 1 + (int)(rng() % 6);
     m.visits = 0;
     m.faults = 0;
+    m.kept   = 0;
   }
 
   // Initial dirty tiles (k_seed): every third tile, with a random key.
@@ -295,9 +300,11 @@ namespace
       }
       if (t < 0)
         break;
-      const uint32_t own_key = begin_visit(q, t);
-      ticket                 = fifo_claim(q);
+      uint32_t own_key = begin_visit(q, t);
+      ticket           = fifo_claim(q);
       const uint32_t front   = at_load(&q.ctrl[CTRL_FRONT]);
+      int            kept    = 0;
+    revisit:
       // ---- visit
       if (m.running[t].fetch_add(1) != 0)
         m.faults.fetch_add(1);
@@ -337,7 +344,12 @@ namespace
         insert[i] = wake_tile(q, targets[i], keys[i], &bucket[i]);
       }
       targets[nt] = t;
-      insert[nt]  = end_visit(q, t, requeue, own_key, &bucket[nt]);
+      // The kernel keeps a tile woken during its visit (up to kMaxKeeps
+      // times in a row); the check keeps at random so both ends are taken.
+      bool           keep    = false;
+      uint32_t       new_key = kInfBits;
+      const bool     keep_ok = m.keeps && kept < (int)kMaxKeeps && (rng() & 1u);
+      insert[nt]             = end_visit(q, t, requeue, own_key, &bucket[nt], keep_ok, &keep, &new_key);
       uint32_t n_all = 0, n_main = 0;
       for (int i = 0; i <= nt; ++i)
       {
@@ -345,9 +357,15 @@ namespace
         n_all += insert[i] ? 1u : 0u;
         n_main += (insert[i] && !later[i]) ? 1u : 0u;
       }
-      const unsigned long long old = publish_count(q, n_main, n_all);
-      if (n_all == 0u && (uint32_t)(old >> 32) == 1u)
-        raise_done(q);
+      unsigned long long old = 0;
+      if (!keep)
+      {
+        old = publish_count(q, n_main, n_all);
+        if (n_all == 0u && (uint32_t)(old >> 32) == 1u)
+          raise_done(q);
+      }
+      else if (n_all)
+        old = publish_count_keep(q, n_main, n_all);
       uint32_t tk = (uint32_t)old;
       for (int i = 0; i <= nt; ++i)
       {
@@ -359,13 +377,21 @@ namespace
         if (later[i])
           later_push(q, bucket[i], (uint32_t)targets[i], HostNap{});
       }
+      if (keep)
+      {
+        // The same tile again, with the absorbed wakes' key.
+        own_key = new_key;
+        ++kept;
+        m.kept.fetch_add(1);
+        goto revisit;
+      }
     }
   }
 
-  int run_case(int tx, int ty, int threads, int order, int window, uint32_t seed)
+  int run_case(int tx, int ty, int threads, int order, int window, bool keeps, uint32_t seed)
   {
     Model m;
-    setup(m, tx, ty, threads, order, window, seed);
+    setup(m, tx, ty, threads, order, window, keeps, seed);
     seed_queue(m, seed * 7u + 1u);
     std::vector<std::thread> th;
     for (int i = 0; i < threads; ++i)
@@ -397,8 +423,8 @@ namespace
         std::printf("  tile %d left in state %u\n", t, m.state[(size_t)t * kStateStride]);
         break;
       }
-    std::printf("%s order=%d window=%d tiles=%dx%d threads=%d seed=%u visits=%u\n", bad ? "FAIL" : "ok  ", order,
-                window, tx, ty, threads, seed, m.visits.load());
+    std::printf("%s order=%d window=%d keep=%d tiles=%dx%d threads=%d seed=%u visits=%u kept=%u\n", bad ? "FAIL" : "ok  ",
+                order, window, (int)keeps, tx, ty, threads, seed, m.visits.load(), m.kept.load());
     return bad;
   }
 } // namespace
@@ -450,10 +476,12 @@ int main(int argc, char** argv)
       for (const auto& s : shapes)
         for (int threads : {1, 4, 24})
           for (int window : {0, 3})
+            for (bool keeps : {false, true})
           {
             if (order == ORDER_FIFO && window != 0)
               continue;
-            bad += run_case(s[0], s[1], threads, order, window, 1000u * (uint32_t)it + (uint32_t)(s[0] * 31 + threads));
+            bad += run_case(s[0], s[1], threads, order, window, keeps,
+                            1000u * (uint32_t)it + (uint32_t)(s[0] * 31 + threads));
             if (bad)
             {
               std::printf("queue model check FAILED\n");

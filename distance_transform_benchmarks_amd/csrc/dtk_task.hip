@@ -120,7 +120,7 @@ namespace dtk
     __shared__ float s_mn[kInitWaves], s_mx[kInitWaves];
     const int        t    = blockIdx.x;
     const int        lane = threadIdx.x & 63;
-    const int        wv   = threadIdx.x >> 6;
+    const int        wv   = wave_index();
     ImgDesc          d;
     int              tx, ty;
     resolve_tile(b, t, d, tx, ty);
@@ -597,13 +597,21 @@ namespace dtk
   // raises DONE.  Kw: the visit's wake keys per wave.
   // claim: the ticket this worker claimed when the visit started (lane 0);
   // pre: see fill_main.
+  // keep_ok: a tile woken during this visit may be kept (end_visit's keep):
+  // then nothing is inserted for it, its item stays pending, and the return
+  // value is true (the same in every lane) after an acquire fence that orders
+  // the next visit's loads after the wakers' stores; own_key then holds the
+  // absorbed wakes' key.
   template <int NW>
-  __device__ void publish(const Queue& q, const ImgDesc& d, int t, int tx, int ty, uint32_t border, bool requeue,
-                          uint32_t own_key, uint32_t front, const uint32_t* Kw, uint32_t claim, int& pre)
+  __device__ bool publish(const Queue& q, const ImgDesc& d, int t, int tx, int ty, uint32_t border, bool requeue,
+                          uint32_t& own_key, uint32_t front, const uint32_t* Kw, uint32_t claim, int& pre,
+                          bool keep_ok)
   {
     const int lane   = threadIdx.x & 63;
     int       target = -1;
     bool      insert = false;
+    bool      kept   = false;
+    uint32_t  nkey   = kInfBits;
     uint32_t  bucket = 0;
     if (lane < 8)
     {
@@ -621,18 +629,24 @@ namespace dtk
     else if (lane == 8)
     {
       target = t;
-      insert = end_visit(q, t, requeue, own_key, &bucket);
+      insert = end_visit(q, t, requeue, own_key, &bucket, keep_ok, &kept, &nkey);
     }
+    const bool     keep  = __shfl((int)kept, 8) != 0;
     const bool     later = insert && deferred(q, bucket, front);
     const uint64_t all   = __ballot(insert);
     const uint64_t mains = __ballot(insert && !later);
     uint32_t       base  = 0;
     if (lane == 0)
     {
-      const unsigned long long old = publish_count(q, (uint32_t)__popcll(mains), (uint32_t)__popcll(all));
-      base                         = (uint32_t)old;
-      if (all == 0u && (uint32_t)(old >> 32) == 1u)
-        raise_done(q);
+      if (!keep)
+      {
+        const unsigned long long old = publish_count(q, (uint32_t)__popcll(mains), (uint32_t)__popcll(all));
+        base                         = (uint32_t)old;
+        if (all == 0u && (uint32_t)(old >> 32) == 1u)
+          raise_done(q);
+      }
+      else if (all != 0u)
+        base = (uint32_t)publish_count_keep(q, (uint32_t)__popcll(mains), (uint32_t)__popcll(all));
     }
     base = (uint32_t)__shfl((int)base, 0);
     // No insertion is issued before that count has returned: the new items
@@ -645,10 +659,22 @@ namespace dtk
     const uint64_t lat = __ballot(later);
     if (lane == 0 && lat)
       at_add(&q.ctrl[CTRL_DEFER], (uint32_t)__popcll(lat));
+    if (keep)
+    {
+      own_key = (uint32_t)__shfl((int)nkey, 8);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    return keep;
   }
 
 #ifndef DTK_TASK_WAVES_PER_EU_16
 #define DTK_TASK_WAVES_PER_EU_16 2
+#endif
+// Whether lambda-0 tiles (constant edge costs, CM == 2) are kept for a visit
+// at once when woken during their own visit; their setup is nearly free, so
+// keeping saves only the queue round trip.
+#ifndef DTK_KEEP_CONST
+#define DTK_KEEP_CONST 1
 #endif
   // One workgroup of NW waves per worker; NW stacked waves solve one tile.
   // Wave 0 pops and publishes for the tile.
@@ -662,9 +688,10 @@ namespace dtk
     __shared__ uint32_t F[4 * NW];
     __shared__ uint32_t Kw[8 * NW];
     __shared__ int      s_tile;
-    const int           w    = NW > 1 ? (int)(threadIdx.x >> 6) : 0;
+    __shared__ int      s_keep;
+    const int           w    = NW > 1 ? wave_index() : 0;
     const int           lane = threadIdx.x & 63;
-    uint32_t         visits = 0, sweeps = 0;
+    uint32_t         visits = 0, sweeps = 0, kept = 0;
     // Cycles per phase, kept in LDS by thread 0 (64-bit registers here
     // would add to the spills of the visit's peak register pressure).
     __shared__ uint64_t cyc[CY_COUNT];
@@ -707,39 +734,59 @@ namespace dtk
       ImgDesc d;
       int     tx, ty;
       resolve_tile(b, t, d, tx, ty);
-      const VisitResult vr = visit_tile<S, C8, CM, NW, true>(d, p, tx, ty, T[w], Sv[w], X, F, Kw);
-      ++visits;
-      if (q.tile_visits && threadIdx.x == 0)
-        at_add(&q.tile_visits[t], 1);
-      sweeps += (uint32_t)vr.sweeps;
-      if (threadIdx.x == 0)
-      {
-        cyc[CY_SETUP] += vr.cyc_setup;
-        cyc[CY_SWEEP] += vr.cyc_sweep;
-        cyc[CY_STORE] += vr.cyc_store;
-      }
-      // Every wave's D stores are ordered (workgroup barrier) before the
-      // release below; wave 0 publishes for the whole tile.
-      __syncthreads();
-      const uint64_t c2 = __builtin_amdgcn_s_memtime();
-      if (threadIdx.x == 0)
-        cyc[CY_VISIT] += c2 - c1;
-      if (w == 0)
-      {
-        if (vr.changed)
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      // Runs after each visit of tile t: accounting, then wave 0 publishes.
+      // True when the tile was woken during the visit and is kept for a
+      // visit at once (same in every thread).
+      uint64_t cv    = c1;
+      uint32_t keeps = 0;
+      auto     after = [&](const VisitResult& vr) -> bool {
+        ++visits;
+        if (q.tile_visits && threadIdx.x == 0)
+          at_add(&q.tile_visits[t], 1);
+        sweeps += (uint32_t)vr.sweeps;
         if (threadIdx.x == 0)
-          cyc[CY_REL] += __builtin_amdgcn_s_memtime() - c2;
-        publish<NW>(q, d, t, tx, ty, vr.border, !vr.converged, own_key, (uint32_t)__shfl((int)front, 0), Kw, ticket,
-                    pre);
-      }
-      if (threadIdx.x == 0)
-        cyc[CY_PUB] += __builtin_amdgcn_s_memtime() - c2;
+        {
+          cyc[CY_SETUP] += vr.cyc_setup;
+          cyc[CY_SWEEP] += vr.cyc_sweep;
+          cyc[CY_STORE] += vr.cyc_store;
+        }
+        // Every wave's D stores are ordered (workgroup barrier) before the
+        // release below; wave 0 publishes for the whole tile.
+        __syncthreads();
+        const uint64_t c2 = __builtin_amdgcn_s_memtime();
+        if (threadIdx.x == 0)
+          cyc[CY_VISIT] += c2 - cv;
+        if (w == 0)
+        {
+          if (vr.changed)
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+          if (threadIdx.x == 0)
+            cyc[CY_REL] += __builtin_amdgcn_s_memtime() - c2;
+          const bool k = publish<NW>(q, d, t, tx, ty, vr.border, !vr.converged, own_key,
+                                     (uint32_t)__shfl((int)front, 0), Kw, ticket, pre,
+                                     keeps < kMaxKeeps && (CM != 2 || DTK_KEEP_CONST));
+          if (lane == 0)
+            s_keep = k ? 1 : 0;
+        }
+        // The keep decision, and (wave 0's acquire fence) the wakers' stores,
+        // reach every wave through this barrier.
+        __syncthreads();
+        const bool k = s_keep != 0;
+        cv           = __builtin_amdgcn_s_memtime();
+        if (threadIdx.x == 0)
+          cyc[CY_PUB] += cv - c2;
+        keeps += k ? 1u : 0u;
+        return k;
+      };
+      visit_tile<S, C8, CM, NW, true>(d, p, tx, ty, T[w], Sv[w], X, F, Kw, after);
+      kept += keeps;
     }
     if (threadIdx.x == 0 && visits)
     {
       at_add(&q.ctrl[CTRL_VISITS], visits);
       at_add(&q.ctrl[CTRL_SWEEPS], sweeps);
+      if (kept)
+        at_add(&q.ctrl[CTRL_KEPT], kept);
     }
     if (threadIdx.x == 0)
     {
@@ -767,7 +814,7 @@ namespace dtk
     __shared__ float    Sv[NW][2 * (64 / S)];
     __shared__ float    X[NW > 1 ? 2 * NW * 4 * 64 : 1];
     __shared__ uint32_t F[4 * NW];
-    const int           w = NW > 1 ? (int)(threadIdx.x >> 6) : 0;
+    const int           w = NW > 1 ? wave_index() : 0;
     const int           t = blockIdx.x;
     if (!active[t])
       return;

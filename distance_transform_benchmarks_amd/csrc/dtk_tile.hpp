@@ -131,6 +131,11 @@ namespace dtk
 
   __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
+  // Index of this wave in its workgroup, as a wave-uniform (scalar) value:
+  // row addresses and bounds tests derived from it then use scalar
+  // instructions instead of being computed per lane.
+  __device__ __forceinline__ int wave_index() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
   // ---- pixels and edge costs -----------------------------------------------
   template <int KIND>
   __device__ __forceinline__ uint32_t load_pix(const unsigned char* row, int x)
@@ -874,21 +879,34 @@ namespace dtk
   // bits; +inf when none): Kw[8 * w + k] for wave w, written before the
   // visit's last workgroup barrier.  The bucket orders queue a woken
   // neighbour by it.
-  template <int S, bool C8, int CM, int NW, bool KEYS = false>
+  //
+  // AFTER: called with each visit's result once its stores are issued (every
+  // thread of the workgroup, the same result in each).  It returns true to
+  // visit the tile again at once (the same in every thread): the edge costs
+  // stay in registers, the owned pixels in registers equal the ones just
+  // stored, and only the one-pixel ring around the tile is loaded again
+  // (the caller has ordered those loads after the stores of the neighbours
+  // that woke the tile).  The default never visits again.
+  struct NoRevisit
+  {
+    __device__ bool operator()(const VisitResult&) const { return false; }
+  };
+
+  template <int S, bool C8, int CM, int NW, bool KEYS = false, class After = NoRevisit>
   __device__ __attribute__((always_inline)) VisitResult visit_tile(const ImgDesc& d, const KParams& p, int tx, int ty,
                                                                    float* T, float* Sv, float* X, uint32_t* F,
-                                                                   uint32_t* Kw = nullptr)
+                                                                   uint32_t* Kw = nullptr, After after = After{})
   {
     constexpr int RH   = 64 / S;
     constexpr int SW   = 64 / S;
     const int     lane = threadIdx.x & 63;
-    const int     w    = NW > 1 ? (int)(threadIdx.x >> 6) : 0;
+    const int     w    = NW > 1 ? wave_index() : 0;
     const int     X0   = tx * kTileW - 1;
     const int     Y0   = ty * (NW * (RH - 2)) - 1 + w * (RH - 2);
     const int     gx   = X0 + lane;
     const bool    cin  = gx >= 0 && gx < d.W;
     constexpr bool TWO  = CM == 1;
-    const uint64_t clk0 = __builtin_amdgcn_s_memtime();
+    uint64_t       clk0 = __builtin_amdgcn_s_memtime();
 
     using Costs = std::conditional_t<CM == 2, ConstCosts, TileCosts<S>>;
     Costs cc;
@@ -920,280 +938,303 @@ namespace dtk
         break;
       }
 
+    // D of the region, +inf outside the image.  Loaded at the start of every
+    // visit as a min into the registers: the first visit starts from +inf;
+    // a kept tile's next visit takes the new values of the ring around the
+    // tile (lanes 0 and 63, the top wave's row 0 and the bottom wave's last
+    // row).  Every other pixel in memory equals its register (this tile
+    // stored it, and its stacked waves merged their shared rows), so the min
+    // over the whole region needs no divergent loads.
     float     Dc[RH];
     const int cx = min(max(gx, 0), d.W - 1);
 #pragma unroll
     for (int r = 0; r < RH; ++r)
-    {
-      const int   gy = Y0 + r;
-      const int   cy = min(max(gy, 0), d.H - 1);
-      const float v  = gload<float>(d.D, (long long)cy * d.D_pitch + 4ll * cx);
-      Dc[r]          = (cin && gy >= 0 && gy < d.H) ? v : DTK_INF;
-    }
-
-    // Halo as loaded: a neighbour needs a new visit exactly when one of its
-    // pixels in this tile's halo ends up strictly smaller (this visit then
-    // found a shorter path into that neighbour).
-    const float init_top = Dc[0];
-    const float init_bot = Dc[RH - 1];
-    if (lane == 0 || lane == 63)
-    {
-      const int o = (lane == 63) ? RH : 0;
-#pragma unroll
-      for (int r = 0; r < RH; ++r)
-        Sv[o + r] = Dc[r];
-    }
+      Dc[r] = DTK_INF;
 
     const float ea = p.e_axis, ed = p.e_diag;
-    const uint64_t clk1 = __builtin_amdgcn_s_memtime();
-    // A visit needs four quiet sweeps to certify its region, so a cap below
-    // four could never converge (the tile would re-queue itself forever).
-    const int   cap = p.max_sweeps > 0 ? (p.max_sweeps < 4 ? 4 : p.max_sweeps) : 0x7fffffff;
-    int         quiet = 0, sweeps = 0;
-    bool        conv = false, any = false;
-    float       R[SW + 2];
-    if constexpr (NW == 1)
-    {
-      for (;;)
-      {
-        uint64_t edge_unused = 0;
-        uint64_t ch          = sweep_col_fwd<RH, C8, TWO>(Dc, cc, ea, ed, edge_unused);
-        ++sweeps;
-        any |= ch != 0;
-        quiet = ch ? 0 : quiet + 1;
-        if (quiet >= 4)
-        {
-          conv = true;
-          break;
-        }
-        if (sweeps >= cap)
-          break;
-        ch = sweep_col_bwd<RH, C8, TWO>(Dc, cc, ea, ed);
-        ++sweeps;
-        any |= ch != 0;
-        quiet = ch ? 0 : quiet + 1;
-        if (quiet >= 4)
-        {
-          conv = true;
-          break;
-        }
-        if (sweeps >= cap)
-          break;
-
-        col_to_row<S>(Dc, R, T, lane);
-        const uint64_t cf = sweep_row_fwd<SW, C8, TWO>(R, cc, ea, ed, edge_unused);
-        ++sweeps;
-        any |= cf != 0;
-        quiet = cf ? 0 : quiet + 1;
-        if (quiet >= 4 || sweeps >= cap)
-        {
-          conv = quiet >= 4;
-          row_to_col<S>(R, Dc, T, lane);
-          break;
-        }
-        const uint64_t cb = sweep_row_bwd<SW, C8, TWO>(R, cc, ea, ed);
-        ++sweeps;
-        any |= cb != 0;
-        quiet = (cb | cf) ? 0 : quiet + 1;
-        row_to_col<S>(R, Dc, T, lane);
-        if (quiet >= 4)
-        {
-          conv = true;
-          break;
-        }
-        if (sweeps >= cap)
-          break;
-      }
-    }
-    else
-    {
-      // Sweep pairs (column down+up, row right+left) in every wave at once;
-      // the shared rows are merged after each pair.  A forward sweep leaves
-      // its direction's edges satisfied except where it read a copy that it
-      // then changed: rows a neighbour wave also holds, and (row layout) a
-      // segment's first and last columns.  So a pair whose backward sweep
-      // changed nothing and whose forward sweep did not touch those leaves
-      // the tile stable in its direction ("clean"), and one pair of the other
-      // direction without any change then certifies the tile.  Before the
-      // first pair the tile counts as not clean, which makes an untouched
-      // tile take two quiet pairs.  After a clean column pair the W / E check
-      // (row_we_violated, voted over the waves) can certify the tile in place
-      // of the row pair.
-      int  par        = 0;
-      bool prev_clean = false;
-      for (;;)
-      {
-        uint64_t       edge = 0;
-        const uint64_t cf   = sweep_col_fwd<RH, C8, TWO>(Dc, cc, ea, ed, edge);
-        const uint64_t cb   = sweep_col_bwd<RH, C8, TWO>(Dc, cc, ea, ed);
-        sweeps += 2;
-        any |= (cf | cb) != 0;
-        uint32_t all = exchange_rows<RH, NW>(
-          Dc, X, F, w, lane, par, ((cf | cb) != 0 ? 1u : 0u) | ((cb | edge) != 0 ? 2u : 0u), any);
-        par ^= 1;
-        if (!(all & 1u) && prev_clean)
-        {
-          conv = true;
-          break;
-        }
-        prev_clean = !(all & 2u);
-        if (sweeps >= cap)
-          break;
-        col_to_row<S>(Dc, R, T, lane);
-        // A clean column pair leaves only the W and E edges to check; the
-        // column layout is still current, so a pass needs no transpose back.
-        if (prev_clean && !any_wave<NW>(row_we_violated<SW, TWO>(R, cc, ea), F, w, lane))
-        {
-          conv = true;
-          break;
-        }
-        edge              = 0;
-        const uint64_t rf = sweep_row_fwd<SW, C8, TWO>(R, cc, ea, ed, edge);
-        const uint64_t rb = sweep_row_bwd<SW, C8, TWO>(R, cc, ea, ed);
-        row_to_col<S>(R, Dc, T, lane);
-        sweeps += 2;
-        any |= (rf | rb) != 0;
-        edge |= rf & shared_row_lanes<S>();
-        all = exchange_rows<RH, NW>(Dc, X, F, w, lane, par,
-                                    ((rf | rb) != 0 ? 1u : 0u) | ((rb | edge) != 0 ? 2u : 0u), any);
-        par ^= 1;
-        if (!(all & 1u) && prev_clean)
-        {
-          conv = true;
-          break;
-        }
-        prev_clean = !(all & 2u);
-        if (sweeps >= cap)
-          break;
-      }
-    }
-
-    const uint64_t clk2 = __builtin_amdgcn_s_memtime();
     VisitResult res;
-    res.sweeps    = sweeps;
-    res.converged = conv;
-    res.changed   = any;
-    res.border    = 0;
-    uint32_t b    = 0;
-    if (any)
+    for (;;)
     {
-      // Only pixels inside the image count: with constant costs the pixels
-      // outside it start at +inf on every visit and always decrease, which
-      // would wake the neighbours (and through them this tile) forever.
-      bool     colimp = false;
-      uint32_t colkey = 0x7f800000u; // smallest improved halo-column value (lanes 0 and 63)
-      if ((lane == 0 || lane == 63) && cin)
+      // The column passes through an empty asm at every visit, so the row
+      // addresses of the loads and stores are formed where they are used
+      // instead of being kept live through the sweeps.
+      int             vx     = cx;
+      const long long dpitch = d.D_pitch;
+      asm volatile("" : "+v"(vx));
+#pragma unroll
+      for (int r = 0; r < RH; ++r)
+      {
+        const int   gy = Y0 + r;
+        const int   cy = min(max(gy, 0), d.H - 1);
+        const float v  = gload<float>(d.D, (long long)cy * dpitch + 4ll * vx);
+        Dc[r]          = minf_nn(Dc[r], (cin && gy >= 0 && gy < d.H) ? v : DTK_INF);
+      }
+
+      // Halo as loaded: a neighbour needs a new visit exactly when one of its
+      // pixels in this tile's halo ends up strictly smaller (this visit then
+      // found a shorter path into that neighbour).
+      const float init_top = Dc[0];
+      const float init_bot = Dc[RH - 1];
+      if (lane == 0 || lane == 63)
       {
         const int o = (lane == 63) ? RH : 0;
 #pragma unroll
-        for (int r = 1; r < RH - 1; ++r)
-        {
-          const bool imp = Y0 + r < d.H && lt_nn(Dc[r], Sv[o + r]);
-          colimp |= imp;
-          if constexpr (KEYS)
-            colkey = imp ? min(colkey, __float_as_uint(Dc[r])) : colkey;
-        }
-      }
-      const bool     timp = cin && Y0 >= 0 && lt_nn(Dc[0], init_top);
-      const bool     bimp = cin && Y0 + RH - 1 < d.H && lt_nn(Dc[RH - 1], init_bot);
-      const uint64_t top  = ballot(timp);
-      const uint64_t bot  = ballot(bimp);
-      const uint64_t cols = ballot(colimp);
-      if constexpr (KEYS)
-      {
-        // Per wave: L, R from the halo columns; T, TL, TR from row 0 of
-        // the top wave; B, BL, BR from the last row of the bottom wave.
-        const bool     mid_lane = lane >= 1 && lane <= 62;
-        uint32_t       tk = (timp && mid_lane) ? __float_as_uint(Dc[0]) : 0x7f800000u;
-        uint32_t       bk = (bimp && mid_lane) ? __float_as_uint(Dc[RH - 1]) : 0x7f800000u;
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1)
-        {
-          tk = min(tk, (uint32_t)__shfl_xor((int)tk, o));
-          bk = min(bk, (uint32_t)__shfl_xor((int)bk, o));
-        }
-        const uint32_t c0  = (uint32_t)__shfl((int)colkey, 0), c63 = (uint32_t)__shfl((int)colkey, 63);
-        const uint32_t t0  = (uint32_t)__shfl((int)(timp ? __float_as_uint(Dc[0]) : 0x7f800000u), 0);
-        const uint32_t t63 = (uint32_t)__shfl((int)(timp ? __float_as_uint(Dc[0]) : 0x7f800000u), 63);
-        const uint32_t b0  = (uint32_t)__shfl((int)(bimp ? __float_as_uint(Dc[RH - 1]) : 0x7f800000u), 0);
-        const uint32_t b63 = (uint32_t)__shfl((int)(bimp ? __float_as_uint(Dc[RH - 1]) : 0x7f800000u), 63);
-        if (lane < 8)
-        {
-          uint32_t k = 0x7f800000u;
-          k          = lane == 0 ? c0 : k;
-          k          = lane == 1 ? c63 : k;
-          if (w == 0)
-          {
-            k = lane == 2 ? tk : k;
-            k = lane == 4 ? t0 : k;
-            k = lane == 5 ? t63 : k;
-          }
-          if (w == NW - 1)
-          {
-            k = lane == 3 ? bk : k;
-            k = lane == 6 ? b0 : k;
-            k = lane == 7 ? b63 : k;
-          }
-          Kw[8 * w + lane] = k;
-        }
-      }
-      const uint64_t mid  = 0x7FFFFFFFFFFFFFFEull; // lanes 1..62
-      b |= (uint32_t)(cols & 1u);                 // L : left halo column
-      b |= (uint32_t)((cols >> 63) & 1u) << 1;    // R : right halo column
-      if (w == 0)
-      {
-        b |= ((top & mid) != 0 ? 1u : 0u) << 2;   // T : top halo row
-        b |= (uint32_t)(top & 1u) << 4;           // TL: top-left corner
-        b |= (uint32_t)((top >> 63) & 1u) << 5;   // TR
-      }
-      if (w == NW - 1)
-      {
-        b |= ((bot & mid) != 0 ? 1u : 0u) << 3;   // B : bottom halo row
-        b |= (uint32_t)(bot & 1u) << 6;           // BL
-        b |= (uint32_t)((bot >> 63) & 1u) << 7;   // BR
+        for (int r = 0; r < RH; ++r)
+          Sv[o + r] = Dc[r];
       }
 
-      // Write back the owned pixels.
-      if (lane >= 1 && lane <= 62 && cin)
+      const uint64_t clk1 = __builtin_amdgcn_s_memtime();
+      // A visit needs four quiet sweeps to certify its region, so a cap below
+      // four could never converge (the tile would re-queue itself forever).
+      const int   cap = p.max_sweeps > 0 ? (p.max_sweeps < 4 ? 4 : p.max_sweeps) : 0x7fffffff;
+      int         quiet = 0, sweeps = 0;
+      bool        conv = false, any = false;
+      float       R[SW + 2];
+      if constexpr (NW == 1)
       {
-#pragma unroll
-        for (int r = 1; r < RH - 1; ++r)
+        for (;;)
         {
-          const int gy = Y0 + r;
-          if (gy < d.H) // device-scope store: written through, so the
-                        // release fence has no dirty L2 lines of D to write back
-            __hip_atomic_store(const_cast<float*>(drow<RH>(d, gy)) + gx, Dc[r], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+          uint64_t edge_unused = 0;
+          uint64_t ch          = sweep_col_fwd<RH, C8, TWO>(Dc, cc, ea, ed, edge_unused);
+          ++sweeps;
+          any |= ch != 0;
+          quiet = ch ? 0 : quiet + 1;
+          if (quiet >= 4)
+          {
+            conv = true;
+            break;
+          }
+          if (sweeps >= cap)
+            break;
+          ch = sweep_col_bwd<RH, C8, TWO>(Dc, cc, ea, ed);
+          ++sweeps;
+          any |= ch != 0;
+          quiet = ch ? 0 : quiet + 1;
+          if (quiet >= 4)
+          {
+            conv = true;
+            break;
+          }
+          if (sweeps >= cap)
+            break;
+
+          col_to_row<S>(Dc, R, T, lane);
+          const uint64_t cf = sweep_row_fwd<SW, C8, TWO>(R, cc, ea, ed, edge_unused);
+          ++sweeps;
+          any |= cf != 0;
+          quiet = cf ? 0 : quiet + 1;
+          if (quiet >= 4 || sweeps >= cap)
+          {
+            conv = quiet >= 4;
+            row_to_col<S>(R, Dc, T, lane);
+            break;
+          }
+          const uint64_t cb = sweep_row_bwd<SW, C8, TWO>(R, cc, ea, ed);
+          ++sweeps;
+          any |= cb != 0;
+          quiet = (cb | cf) ? 0 : quiet + 1;
+          row_to_col<S>(R, Dc, T, lane);
+          if (quiet >= 4)
+          {
+            conv = true;
+            break;
+          }
+          if (sweeps >= cap)
+            break;
         }
       }
-    }
-    if constexpr (KEYS)
-    {
-      if (!any && lane < 8)
-        Kw[8 * w + lane] = 0x7f800000u;
-    }
-    if constexpr (NW == 1)
-    {
-      res.border = b;
-      if constexpr (KEYS)
-        lds_wave_sync();
-    }
-    else
-    {
-      // Border bits and change flags of all waves (F[2*NW ..]).
-      if (lane == 0)
-        F[2 * NW + w] = b | (any ? 0x100u : 0u);
-      __syncthreads();
-      uint32_t all = 0;
+      else
+      {
+        // Sweep pairs (column down+up, row right+left) in every wave at once;
+        // the shared rows are merged after each pair.  A forward sweep leaves
+        // its direction's edges satisfied except where it read a copy that it
+        // then changed: rows a neighbour wave also holds, and (row layout) a
+        // segment's first and last columns.  So a pair whose backward sweep
+        // changed nothing and whose forward sweep did not touch those leaves
+        // the tile stable in its direction ("clean"), and one pair of the other
+        // direction without any change then certifies the tile.  Before the
+        // first pair the tile counts as not clean, which makes an untouched
+        // tile take two quiet pairs.  After a clean column pair the W / E check
+        // (row_we_violated, voted over the waves) can certify the tile in place
+        // of the row pair.
+        int  par        = 0;
+        bool prev_clean = false;
+        for (;;)
+        {
+          uint64_t       edge = 0;
+          const uint64_t cf   = sweep_col_fwd<RH, C8, TWO>(Dc, cc, ea, ed, edge);
+          const uint64_t cb   = sweep_col_bwd<RH, C8, TWO>(Dc, cc, ea, ed);
+          sweeps += 2;
+          any |= (cf | cb) != 0;
+          uint32_t all = exchange_rows<RH, NW>(
+            Dc, X, F, w, lane, par, ((cf | cb) != 0 ? 1u : 0u) | ((cb | edge) != 0 ? 2u : 0u), any);
+          par ^= 1;
+          if (!(all & 1u) && prev_clean)
+          {
+            conv = true;
+            break;
+          }
+          prev_clean = !(all & 2u);
+          if (sweeps >= cap)
+            break;
+          col_to_row<S>(Dc, R, T, lane);
+          // A clean column pair leaves only the W and E edges to check; the
+          // column layout is still current, so a pass needs no transpose back.
+          if (prev_clean && !any_wave<NW>(row_we_violated<SW, TWO>(R, cc, ea), F, w, lane))
+          {
+            conv = true;
+            break;
+          }
+          edge              = 0;
+          const uint64_t rf = sweep_row_fwd<SW, C8, TWO>(R, cc, ea, ed, edge);
+          const uint64_t rb = sweep_row_bwd<SW, C8, TWO>(R, cc, ea, ed);
+          row_to_col<S>(R, Dc, T, lane);
+          sweeps += 2;
+          any |= (rf | rb) != 0;
+          edge |= rf & shared_row_lanes<S>();
+          all = exchange_rows<RH, NW>(Dc, X, F, w, lane, par,
+                                      ((rf | rb) != 0 ? 1u : 0u) | ((rb | edge) != 0 ? 2u : 0u), any);
+          par ^= 1;
+          if (!(all & 1u) && prev_clean)
+          {
+            conv = true;
+            break;
+          }
+          prev_clean = !(all & 2u);
+          if (sweeps >= cap)
+            break;
+        }
+      }
+
+      const uint64_t clk2 = __builtin_amdgcn_s_memtime();
+      res.sweeps    = sweeps;
+      res.converged = conv;
+      res.changed   = any;
+      res.border    = 0;
+      uint32_t b    = 0;
+      if (any)
+      {
+        // Only pixels inside the image count: with constant costs the pixels
+        // outside it start at +inf on every visit and always decrease, which
+        // would wake the neighbours (and through them this tile) forever.
+        bool     colimp = false;
+        uint32_t colkey = 0x7f800000u; // smallest improved halo-column value (lanes 0 and 63)
+        if ((lane == 0 || lane == 63) && cin)
+        {
+          const int o = (lane == 63) ? RH : 0;
 #pragma unroll
-      for (int k = 0; k < NW; ++k)
-        all |= F[2 * NW + k];
-      res.border  = all & 0xFFu;
-      res.changed = (all & 0x100u) != 0;
+          for (int r = 1; r < RH - 1; ++r)
+          {
+            const bool imp = Y0 + r < d.H && lt_nn(Dc[r], Sv[o + r]);
+            colimp |= imp;
+            if constexpr (KEYS)
+              colkey = imp ? min(colkey, __float_as_uint(Dc[r])) : colkey;
+          }
+        }
+        const bool     timp = cin && Y0 >= 0 && lt_nn(Dc[0], init_top);
+        const bool     bimp = cin && Y0 + RH - 1 < d.H && lt_nn(Dc[RH - 1], init_bot);
+        const uint64_t top  = ballot(timp);
+        const uint64_t bot  = ballot(bimp);
+        const uint64_t cols = ballot(colimp);
+        if constexpr (KEYS)
+        {
+          // Per wave: L, R from the halo columns; T, TL, TR from row 0 of
+          // the top wave; B, BL, BR from the last row of the bottom wave.
+          const bool     mid_lane = lane >= 1 && lane <= 62;
+          uint32_t       tk = (timp && mid_lane) ? __float_as_uint(Dc[0]) : 0x7f800000u;
+          uint32_t       bk = (bimp && mid_lane) ? __float_as_uint(Dc[RH - 1]) : 0x7f800000u;
+#pragma unroll
+          for (int o = 32; o >= 1; o >>= 1)
+          {
+            tk = min(tk, (uint32_t)__shfl_xor((int)tk, o));
+            bk = min(bk, (uint32_t)__shfl_xor((int)bk, o));
+          }
+          const uint32_t c0  = (uint32_t)__shfl((int)colkey, 0), c63 = (uint32_t)__shfl((int)colkey, 63);
+          const uint32_t t0  = (uint32_t)__shfl((int)(timp ? __float_as_uint(Dc[0]) : 0x7f800000u), 0);
+          const uint32_t t63 = (uint32_t)__shfl((int)(timp ? __float_as_uint(Dc[0]) : 0x7f800000u), 63);
+          const uint32_t b0  = (uint32_t)__shfl((int)(bimp ? __float_as_uint(Dc[RH - 1]) : 0x7f800000u), 0);
+          const uint32_t b63 = (uint32_t)__shfl((int)(bimp ? __float_as_uint(Dc[RH - 1]) : 0x7f800000u), 63);
+          if (lane < 8)
+          {
+            uint32_t k = 0x7f800000u;
+            k          = lane == 0 ? c0 : k;
+            k          = lane == 1 ? c63 : k;
+            if (w == 0)
+            {
+              k = lane == 2 ? tk : k;
+              k = lane == 4 ? t0 : k;
+              k = lane == 5 ? t63 : k;
+            }
+            if (w == NW - 1)
+            {
+              k = lane == 3 ? bk : k;
+              k = lane == 6 ? b0 : k;
+              k = lane == 7 ? b63 : k;
+            }
+            Kw[8 * w + lane] = k;
+          }
+        }
+        const uint64_t mid  = 0x7FFFFFFFFFFFFFFEull; // lanes 1..62
+        b |= (uint32_t)(cols & 1u);                 // L : left halo column
+        b |= (uint32_t)((cols >> 63) & 1u) << 1;    // R : right halo column
+        if (w == 0)
+        {
+          b |= ((top & mid) != 0 ? 1u : 0u) << 2;   // T : top halo row
+          b |= (uint32_t)(top & 1u) << 4;           // TL: top-left corner
+          b |= (uint32_t)((top >> 63) & 1u) << 5;   // TR
+        }
+        if (w == NW - 1)
+        {
+          b |= ((bot & mid) != 0 ? 1u : 0u) << 3;   // B : bottom halo row
+          b |= (uint32_t)(bot & 1u) << 6;           // BL
+          b |= (uint32_t)((bot >> 63) & 1u) << 7;   // BR
+        }
+
+        // Write back the owned pixels.
+        if (lane >= 1 && lane <= 62 && cin)
+        {
+#pragma unroll
+          for (int r = 1; r < RH - 1; ++r)
+          {
+            const int gy = Y0 + r;
+            if (gy < d.H) // device-scope store: written through, so the
+                          // release fence has no dirty L2 lines of D to write back
+              __hip_atomic_store(reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(d.D) + (long long)gy * dpitch) + vx,
+                                 Dc[r], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+          }
+        }
+      }
+      if constexpr (KEYS)
+      {
+        if (!any && lane < 8)
+          Kw[8 * w + lane] = 0x7f800000u;
+      }
+      if constexpr (NW == 1)
+      {
+        res.border = b;
+        if constexpr (KEYS)
+          lds_wave_sync();
+      }
+      else
+      {
+        // Border bits and change flags of all waves (F[2*NW ..]).
+        if (lane == 0)
+          F[2 * NW + w] = b | (any ? 0x100u : 0u);
+        __syncthreads();
+        uint32_t all = 0;
+#pragma unroll
+        for (int k = 0; k < NW; ++k)
+          all |= F[2 * NW + k];
+        res.border  = all & 0xFFu;
+        res.changed = (all & 0x100u) != 0;
+      }
+      res.cyc_setup = (uint32_t)(clk1 - clk0);
+      res.cyc_sweep = (uint32_t)(clk2 - clk1);
+      res.cyc_store = (uint32_t)(__builtin_amdgcn_s_memtime() - clk2);
+      if (!after(res))
+        return res;
+      clk0 = __builtin_amdgcn_s_memtime();
     }
-    res.cyc_setup = (uint32_t)(clk1 - clk0);
-    res.cyc_sweep = (uint32_t)(clk2 - clk1);
-    res.cyc_store = (uint32_t)(__builtin_amdgcn_s_memtime() - clk2);
-    return res;
   }
 } // namespace dtk

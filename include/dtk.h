@@ -138,6 +138,8 @@ typedef struct
   uint32_t moved;        /*   tiles those drains moved into the main ring */
   uint32_t deferred;     /*   wakes deferred into a later ring (seeds excluded) */
   uint32_t cycles_drain; /*   summed cycles / 1024 of the drains (inside waiting) */
+  uint32_t kept;         /* task: visits that followed a visit of the same tile at once, by the same
+                            worker, because the tile was woken during that visit (counted in visits) */
 } dtk_stats;
 
 void dtk_params_default(dtk_params* p);

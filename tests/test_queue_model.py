@@ -3,7 +3,8 @@
 lib/dtk_queue_check runs the queue functions of csrc/dtk_queue.hpp (the
 same __host__ __device__ code the persistent kernel runs) on many CPU
 threads over a random wake graph, for the FIFO, KEY and SEED orders (with
-deferral windows of 0 and 3 buckets).  It checks that tile ids and buckets read back
+deferral windows of 0 and 3 buckets), each with and without workers that keep
+a tile woken during its own visit and visit it again at once.  It checks that tile ids and buckets read back
 from the rings stay in range, that no tile is visited by two workers at
 once, that no wake is lost and that the workers terminate.  It also checks,
 without launching anything, that the grid of the adjoint's tile kernels is
@@ -31,5 +32,6 @@ def test_queue_protocol_model_check():
     assert "launch shape checks passed" in r.stdout
     # every order, both claim modes and the multi-thread cases ran
     for order in (0, 1, 2):
-        assert f"order={order} window=0 tiles=40x25 threads=24" in r.stdout
-    assert "order=1 window=3 tiles=40x25 threads=24" in r.stdout
+        for keep in (0, 1):
+            assert f"order={order} window=0 keep={keep} tiles=40x25 threads=24" in r.stdout
+    assert "order=1 window=3 keep=1 tiles=40x25 threads=24" in r.stdout
