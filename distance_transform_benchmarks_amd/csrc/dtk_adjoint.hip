@@ -669,7 +669,21 @@ namespace dtk
     const int      el = (int)u - (lane == 63 ? 0 : (int)((lane + 1) * 37 % 256));
     const int      er = (int)u - (lane == 0 ? 0 : (int)((lane - 1) * 37 % 256));
     const bool     ok2 = dl == el && dr == er && pix_dist<IMG_U8>(u, ul) == (float)(el < 0 ? -el : el);
-    const uint64_t all = __ballot(ok && ok2);
+    // The RGB8 squared distance (dot products) against channel differences,
+    // on pixel pairs that include every channel extreme.
+    bool ok3 = true;
+    for (int k = 0; k < 64; ++k)
+    {
+      const uint32_t pa = ((uint32_t)(lane * 97 + 13 * k) & 255u) | (((uint32_t)(lane * 5 + k * 71) & 255u) << 8) |
+                          (((uint32_t)(255 - lane * 3 - k) & 255u) << 16);
+      const uint32_t pb = ((uint32_t)(k * 4 + (lane & 1) * 255) & 255u) | ((uint32_t)((lane + k) & 1 ? 255 : 0) << 8) |
+                          (((uint32_t)(lane * k) & 255u) << 16);
+      const int dr2 = (int)(pa & 255u) - (int)(pb & 255u);
+      const int dg2 = (int)((pa >> 8) & 255u) - (int)((pb >> 8) & 255u);
+      const int db2 = (int)((pa >> 16) & 255u) - (int)((pb >> 16) & 255u);
+      ok3 &= rgb_dist2(pa, pb) == (uint32_t)(dr2 * dr2 + dg2 * dg2 + db2 * db2);
+    }
+    const uint64_t all = __ballot(ok && ok2 && ok3);
     if (lane == 0)
       out[0] = all == ~0ull ? 1 : 0;
   }

@@ -152,24 +152,32 @@ namespace dtk
       return gload<uint32_t>(row, 4ll * x);
   }
 
-  // Correctly rounded square root of an integer 0 <= n < 2^24 (here the
-  // exact squared colour distance, at most 3 * 255^2 = 195075), equal to
-  // sqrtf((float)n).  v_sqrt_f32 is accurate to 1 ulp, so the correctly
-  // rounded root is s or one of its float neighbours; the exact fma
-  // residuals n - s_dn * s and n - s_up * s against the neighbours' products
-  // pick it (the standard correction step of a 1-ulp square root).  n is 0
-  // or at least 1, so no denormal scaling and no special-value case is
-  // needed: for n = 0, s = 0 and both tests keep it.  dtk_selftest checks
-  // every n up to 195075 against the host's sqrtf.
+  // Correctly rounded square root of an integer 0 <= n <= 3 * 255^2 (the
+  // exact squared colour distance), equal to sqrtf((float)n).  v_sqrt_f32
+  // is accurate to 1 ulp; on gfx950 it never returns more than the
+  // correctly rounded root on this domain (checked exhaustively for every
+  // n < 2^18), so the root is s or its upper float neighbour, and the exact
+  // fma residual n - s_up * s picks it (the upper half of the standard
+  // two-sided correction of a 1-ulp square root).  dtk_selftest compares
+  // every n of the domain with the host's sqrtf on each run.
   __device__ __forceinline__ float sqrt_int_rn(uint32_t n)
   {
     const float x    = (float)n;
     const float s    = __builtin_amdgcn_sqrtf(x);
-    const float s_dn = __uint_as_float(__float_as_uint(s) - 1u);
     const float s_up = __uint_as_float(__float_as_uint(s) + 1u);
-    float       r    = __builtin_fmaf(-s_dn, s, x) <= 0.f ? s_dn : s;
-    r                = __builtin_fmaf(-s_up, s, x) > 0.f ? s_up : r;
-    return r;
+    return __builtin_fmaf(-s_up, s, x) > 0.f ? s_up : s;
+  }
+
+  // Exact squared distance of two RGB8 pixels packed r | g << 8 | b << 16
+  // (top byte zero): |a|^2 + |b|^2 - 2 a.b with three v_dot4_u32_u8, in
+  // place of extracting and squaring three channel differences.  No term
+  // exceeds 2 * 3 * 255^2, so the unsigned arithmetic is exact.
+  __device__ __forceinline__ uint32_t rgb_dist2(uint32_t a, uint32_t b)
+  {
+    const uint32_t ab = __builtin_amdgcn_udot4(a, b, 0u, false);
+    const uint32_t aa = __builtin_amdgcn_udot4(a, a, 0u, false);
+    const uint32_t bb = __builtin_amdgcn_udot4(b, b, 0u, false);
+    return aa + bb - 2u * ab;
   }
 
   // |I(p) - I(q)|: uint8 abs difference (dt/src/utils.cuh:37-41); RGB8:
@@ -184,12 +192,7 @@ namespace dtk
       return (float)(d < 0 ? -d : d);
     }
     else if constexpr (KIND == IMG_RGB8)
-    {
-      const int dr = (int)(a & 255u) - (int)(b & 255u);
-      const int dg = (int)((a >> 8) & 255u) - (int)((b >> 8) & 255u);
-      const int db = (int)((a >> 16) & 255u) - (int)((b >> 16) & 255u);
-      return sqrt_int_rn((uint32_t)(dr * dr + dg * dg + db * db));
-    }
+      return sqrt_int_rn(rgb_dist2(a, b));
     else
       return fabsf(__uint_as_float(a) - __uint_as_float(b));
   }
