@@ -131,9 +131,9 @@ def test_autograd_equals_direct_adjoint(dt, dev):
 
 @pytest.mark.parametrize("lam", [1.0, 0.6])
 def test_adjoint_chains_across_tiles(dt, dev, oracle, lam):
-    """Subtree sums run per 64 x 64 tile and then over the pixels whose
+    """Subtree sums run per 32 x 32 tile and then over the pixels whose
     predecessor lies in another tile.  A spiral maze (389 x 389, not a
-    multiple of 64) with one seed at the centre makes every chain wind
+    multiple of 32) with one seed at the centre makes every chain wind
     through many tiles, crossing their edges and corners; the gradients
     match the CPU reference adjoint to 1e-4 of the largest one."""
     import torch
@@ -162,7 +162,7 @@ def test_adjoint_chains_across_tiles(dt, dev, oracle, lam):
 
 
 SHAPES_4 = ((80, 64), (50, 120), (33, 33), (1, 17))
-# Nine uneven images, several of them one 64 x 64 tile or less.
+# Nine uneven images, several of them one 32 x 32 tile of the subtree sums or less.
 SHAPES_9 = SHAPES_4 + ((64, 64), (17, 90), (120, 40), (2, 2), (70, 71))
 
 
