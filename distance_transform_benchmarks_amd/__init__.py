@@ -37,5 +37,6 @@ from .api import (  # noqa: F401
     visits_buffer,
 )
 from .autograd import GGDTFunction, ggdt_differentiable  # noqa: F401
+from .imageio import image_info, imread, imsave  # noqa: F401
 
 __version__ = "0.1.0"

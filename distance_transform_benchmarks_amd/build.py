@@ -37,7 +37,7 @@ COMMON = [
 ]
 
 DTK_SOURCES = ["dtk_task.hip", "dtk_raster.hip", "dtk_adjoint.hip", "dtk_color.hip", "dtk_priority.hip", "dtk_host.cpp"]
-DT_SOURCES = ["dt_api.cpp", "dt_color.cpp", "dt_png.cpp"]
+DT_SOURCES = ["dt_api.cpp", "dt_color.cpp", "dt_png.cpp", "dt_jpeg.cpp", "dt_image_formats.cpp"]
 # Command-line tools (the reference's bin/distance_transform.cpp,
 # bin/empiric_check.cpp, bin/rounds_per_blocks.cu and bin/priority_block.cu),
 # linked against libdt.

@@ -4,19 +4,22 @@
 // dt/src/imsave.cpp:9-30); this is an independent codec on zlib that keeps
 // the reference's observable behaviour:
 //   - imread<uint8_t> accepts an image with exactly one 8-bit component, as
-//     stb reports components: a grey PNG (any bit depth; 16-bit samples keep
-//     their high byte, 1/2/4-bit samples are scaled to 0..255), interlaced
-//     or not.  Grey with a tRNS chunk counts as two components; RGB, RGBA,
-//     grey+alpha and palette images are rejected with the reference's
-//     message.  Unreadable files raise "Error while loading <file>".
+//     stb reports components.  PNG: grey of any bit depth (16-bit samples
+//     keep their high byte, 1/2/4-bit samples are scaled to 0..255),
+//     interlaced or not; grey with a tRNS chunk counts as two components;
+//     RGB, RGBA, grey+alpha and palette images are rejected with the
+//     reference's message.  JPEG, PGM and TGA are decoded by dt_jpeg.cpp and
+//     dt_image_formats.cpp, which also recognise the formats that only ever
+//     have colour.  Unreadable files raise "Error while loading <file>".
 //   - imsave writes 8-bit grey or RGB, refuses a name that does not end in
 //     ".png", and reads rows at the view's pitch.
-// Only PNG is decoded (stb also reads JPEG, BMP, ...; those inputs raise the
-// loading error here).  Chunk CRCs are not checked on input, as stb does
-// not check them; they are written on output.
+// Chunk CRCs are not checked on input, as stb does not check them; they are
+// written on output.
 #include <dt/imread.hpp>
 #include <dt/imsave.hpp>
 #include <dt_io.h>
+
+#include "dt_image_decode.hpp"
 
 #include <zlib.h>
 
@@ -46,13 +49,6 @@ namespace
     v.push_back((unsigned char)(x >> 8));
     v.push_back((unsigned char)x);
   }
-
-  struct Decoded
-  {
-    int                        width = 0, height = 0;
-    int                        components = 0; // as stb_image counts them
-    std::vector<unsigned char> grey;           // width * height, filled when components == 1
-  };
 
   struct Header
   {
@@ -143,121 +139,126 @@ namespace
     }
   }
 
-  // Decodes the file's header and, for a one-component image, its samples.
-  // Returns an empty string on success, else the reason.
-  std::string decode(const std::vector<unsigned char>& f, Decoded& out)
+  using dtio::Decoded;
+} // namespace
+
+// Decodes the file's header and, for a one-component image, its samples.
+// Returns an empty string on success, else the reason.
+std::string dtio::decode_png(const std::vector<unsigned char>& f, Decoded& out)
+{
+  if (f.size() < 8 || std::memcmp(f.data(), kSignature, 8) != 0)
+    return "not a PNG file";
+  size_t                     pos = 8;
+  Header                     h;
+  bool                       have_header = false, have_trns = false, have_plte = false;
+  std::vector<unsigned char> idat;
+  while (pos + 12 <= f.size())
   {
-    if (f.size() < 8 || std::memcmp(f.data(), kSignature, 8) != 0)
-      return "not a PNG file";
-    size_t                     pos = 8;
-    Header                     h;
-    bool                       have_header = false, have_trns = false, have_plte = false;
-    std::vector<unsigned char> idat;
-    while (pos + 12 <= f.size())
+    const uint32_t len = be32(&f[pos]);
+    if (len > f.size() - pos - 12)
+      return "truncated chunk";
+    const char*          type = reinterpret_cast<const char*>(&f[pos + 4]);
+    const unsigned char* data = &f[pos + 8];
+    if (std::memcmp(type, "IHDR", 4) == 0)
     {
-      const uint32_t len = be32(&f[pos]);
-      if (len > f.size() - pos - 12)
-        return "truncated chunk";
-      const char*          type = reinterpret_cast<const char*>(&f[pos + 4]);
-      const unsigned char* data = &f[pos + 8];
-      if (std::memcmp(type, "IHDR", 4) == 0)
-      {
-        if (len != 13)
-          return "bad IHDR";
-        h.width     = be32(data);
-        h.height    = be32(data + 4);
-        h.depth     = data[8];
-        h.color     = data[9];
-        h.interlace = data[12];
-        if (data[10] != 0 || data[11] != 0 || h.interlace > 1 || !valid_depth(h.color, h.depth))
-          return "unsupported IHDR";
-        if (h.width == 0 || h.height == 0 || h.width > (1u << 24) || h.height > (1u << 24) ||
-            (unsigned long long)h.width * h.height > (1ull << 31))
-          return "bad image size";
-        have_header = true;
-      }
-      else if (std::memcmp(type, "PLTE", 4) == 0)
-        have_plte = true;
-      else if (std::memcmp(type, "tRNS", 4) == 0)
-        have_trns = true;
-      else if (std::memcmp(type, "IDAT", 4) == 0)
-        idat.insert(idat.end(), data, data + len);
-      else if (std::memcmp(type, "IEND", 4) == 0)
-        break;
-      pos += 12 + (size_t)len;
+      if (len != 13)
+        return "bad IHDR";
+      h.width     = be32(data);
+      h.height    = be32(data + 4);
+      h.depth     = data[8];
+      h.color     = data[9];
+      h.interlace = data[12];
+      if (data[10] != 0 || data[11] != 0 || h.interlace > 1 || !valid_depth(h.color, h.depth))
+        return "unsupported IHDR";
+      if (h.width == 0 || h.height == 0 || h.width > (1u << 24) || h.height > (1u << 24) ||
+          (unsigned long long)h.width * h.height > (1ull << 31))
+        return "bad image size";
+      have_header = true;
     }
-    if (!have_header)
-      return "missing IHDR";
-    if (h.color == 3 && !have_plte)
-      return "missing PLTE";
-    out.width  = (int)h.width;
-    out.height = (int)h.height;
-    // Components as stb_image reports them: palette images expand to RGB(A),
-    // a tRNS chunk adds an alpha component to grey and RGB images.
-    if (h.color == 3)
-      out.components = have_trns ? 4 : 3;
-    else
-      out.components = channels_of(h.color) + ((have_trns && (h.color == 0 || h.color == 2)) ? 1 : 0);
-    if (out.components != 1)
-      return "";
-
-    // Grey without tRNS: inflate, unfilter and convert to 8-bit samples.
-    const size_t W = h.width, H = h.height;
-    struct Pass
-    {
-      size_t x0, y0, dx, dy;
-    };
-    const Pass   adam7[7] = {{0, 0, 8, 8}, {4, 0, 8, 8}, {0, 4, 4, 8}, {2, 0, 4, 4},
-                             {0, 2, 2, 4}, {1, 0, 2, 2}, {0, 1, 1, 2}};
-    const Pass   whole    = {0, 0, 1, 1};
-    const int    npass    = h.interlace ? 7 : 1;
-    const size_t bpp      = std::max<size_t>(1, (size_t)h.depth / 8);
-    size_t       need     = 0;
-    for (int k = 0; k < npass; ++k)
-    {
-      const Pass&  p  = h.interlace ? adam7[k] : whole;
-      const size_t pw = W > p.x0 ? (W - p.x0 + p.dx - 1) / p.dx : 0;
-      const size_t ph = H > p.y0 ? (H - p.y0 + p.dy - 1) / p.dy : 0;
-      if (pw && ph)
-        need += ph * (1 + (pw * (size_t)h.depth + 7) / 8);
-    }
-    std::vector<unsigned char> raw(need);
-    z_stream                   zs{};
-    if (inflateInit(&zs) != Z_OK)
-      return "zlib init failed";
-    zs.next_in   = idat.data();
-    zs.avail_in  = (uInt)idat.size();
-    zs.next_out  = raw.data();
-    zs.avail_out = (uInt)raw.size();
-    const int zr = inflate(&zs, Z_FINISH);
-    const size_t got = raw.size() - zs.avail_out;
-    inflateEnd(&zs);
-    if ((zr != Z_STREAM_END && zr != Z_BUF_ERROR && zr != Z_OK) || got < need)
-      return "corrupt image data";
-
-    out.grey.assign(W * H, 0);
-    size_t off = 0;
-    for (int k = 0; k < npass; ++k)
-    {
-      const Pass&  p  = h.interlace ? adam7[k] : whole;
-      const size_t pw = W > p.x0 ? (W - p.x0 + p.dx - 1) / p.dx : 0;
-      const size_t ph = H > p.y0 ? (H - p.y0 + p.dy - 1) / p.dy : 0;
-      if (!pw || !ph)
-        continue;
-      const size_t rowbytes = (pw * (size_t)h.depth + 7) / 8;
-      if (!unfilter(raw.data() + off, ph, rowbytes, bpp))
-        return "bad filter type";
-      for (size_t y = 0; y < ph; ++y)
-      {
-        const unsigned char* row = raw.data() + off + y * (rowbytes + 1) + 1;
-        for (size_t x = 0; x < pw; ++x)
-          out.grey[(p.y0 + y * p.dy) * W + p.x0 + x * p.dx] = grey_sample(row, x, h.depth);
-      }
-      off += ph * (rowbytes + 1);
-    }
-    return "";
+    else if (std::memcmp(type, "PLTE", 4) == 0)
+      have_plte = true;
+    else if (std::memcmp(type, "tRNS", 4) == 0)
+      have_trns = true;
+    else if (std::memcmp(type, "IDAT", 4) == 0)
+      idat.insert(idat.end(), data, data + len);
+    else if (std::memcmp(type, "IEND", 4) == 0)
+      break;
+    pos += 12 + (size_t)len;
   }
+  if (!have_header)
+    return "missing IHDR";
+  if (h.color == 3 && !have_plte)
+    return "missing PLTE";
+  out.width  = (int)h.width;
+  out.height = (int)h.height;
+  // Components as stb_image reports them: palette images expand to RGB(A),
+  // a tRNS chunk adds an alpha component to grey and RGB images.
+  if (h.color == 3)
+    out.components = have_trns ? 4 : 3;
+  else
+    out.components = channels_of(h.color) + ((have_trns && (h.color == 0 || h.color == 2)) ? 1 : 0);
+  if (out.components != 1 || out.header_only)
+    return "";
 
+  // Grey without tRNS: inflate, unfilter and convert to 8-bit samples.
+  const size_t W = h.width, H = h.height;
+  struct Pass
+  {
+    size_t x0, y0, dx, dy;
+  };
+  const Pass   adam7[7] = {{0, 0, 8, 8}, {4, 0, 8, 8}, {0, 4, 4, 8}, {2, 0, 4, 4},
+                           {0, 2, 2, 4}, {1, 0, 2, 2}, {0, 1, 1, 2}};
+  const Pass   whole    = {0, 0, 1, 1};
+  const int    npass    = h.interlace ? 7 : 1;
+  const size_t bpp      = std::max<size_t>(1, (size_t)h.depth / 8);
+  size_t       need     = 0;
+  for (int k = 0; k < npass; ++k)
+  {
+    const Pass&  p  = h.interlace ? adam7[k] : whole;
+    const size_t pw = W > p.x0 ? (W - p.x0 + p.dx - 1) / p.dx : 0;
+    const size_t ph = H > p.y0 ? (H - p.y0 + p.dy - 1) / p.dy : 0;
+    if (pw && ph)
+      need += ph * (1 + (pw * (size_t)h.depth + 7) / 8);
+  }
+  std::vector<unsigned char> raw(need);
+  z_stream                   zs{};
+  if (inflateInit(&zs) != Z_OK)
+    return "zlib init failed";
+  zs.next_in   = idat.data();
+  zs.avail_in  = (uInt)idat.size();
+  zs.next_out  = raw.data();
+  zs.avail_out = (uInt)raw.size();
+  const int zr = inflate(&zs, Z_FINISH);
+  const size_t got = raw.size() - zs.avail_out;
+  inflateEnd(&zs);
+  if ((zr != Z_STREAM_END && zr != Z_BUF_ERROR && zr != Z_OK) || got < need)
+    return "corrupt image data";
+
+  out.grey.assign(W * H, 0);
+  size_t off = 0;
+  for (int k = 0; k < npass; ++k)
+  {
+    const Pass&  p  = h.interlace ? adam7[k] : whole;
+    const size_t pw = W > p.x0 ? (W - p.x0 + p.dx - 1) / p.dx : 0;
+    const size_t ph = H > p.y0 ? (H - p.y0 + p.dy - 1) / p.dy : 0;
+    if (!pw || !ph)
+      continue;
+    const size_t rowbytes = (pw * (size_t)h.depth + 7) / 8;
+    if (!unfilter(raw.data() + off, ph, rowbytes, bpp))
+      return "bad filter type";
+    for (size_t y = 0; y < ph; ++y)
+    {
+      const unsigned char* row = raw.data() + off + y * (rowbytes + 1) + 1;
+      for (size_t x = 0; x < pw; ++x)
+        out.grey[(p.y0 + y * p.dy) * W + p.x0 + x * p.dx] = grey_sample(row, x, h.depth);
+    }
+    off += ph * (rowbytes + 1);
+  }
+  return "";
+}
+
+namespace
+{
   bool read_file(const char* path, std::vector<unsigned char>& bytes)
   {
     std::ifstream in(path, std::ios::binary);
@@ -369,7 +370,7 @@ namespace dt
     bool                       ok = false;
     try
     {
-      ok = read_file(filename, bytes) && decode(bytes, d).empty();
+      ok = read_file(filename, bytes) && dtio::decode_image(bytes, d).empty();
     }
     catch (const std::bad_alloc&)
     {
@@ -417,7 +418,10 @@ extern "C"
       if (!path || !read_file(path, bytes))
         why = "cannot read the file";
       else
-        why = decode(bytes, d);
+      {
+        d.header_only = true;
+        why           = dtio::decode_image(bytes, d);
+      }
     }
     catch (const std::exception& e)
     {

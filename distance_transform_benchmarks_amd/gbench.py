@@ -10,14 +10,20 @@ ms, ``bytes_per_second`` = pixels per second (the reference sets
 counters.  The result file has the reference's ``context`` / ``benchmarks``
 layout (results/benchmark_results.json), so both can be compared row by row.
 
-The corpus is synthetic: the reference's 639 greyscale JPEGs are not
-available.  Image i is a seeded blob field (SplitMix64, seed 2000 + i) whose
+The default corpus is synthetic: the reference's 639 greyscale JPEGs
+(/data/dt_images, benchmarks/distance_transform.cpp:17) are not available
+here.  Image i is a seeded blob field (SplitMix64, seed 2000 + i) whose
 pixel count equals the i-th distinct ``size`` of the reference's published
-results (8.5-11.5 Mpx, 4:3), with a single seed pixel at the centre.  The
+results (8.5-11.5 Mpx, 4:3), with a single seed pixel at the centre.  With
+``--corpus DIR`` the images are instead the files of DIR in sorted path
+order, read with the library's imread (any file the reference's imread
+reads: grey JPEG, PNG, PGM, TGA), as the reference lists and loads its
+directory (benchmarks/distance_transform.cpp:19-37, :57-59).  The
 reference's mask is uninitialised heap memory apart from one pixel
 (benchmarks/distance_transform.cpp:66-67); this harness defines it instead.
 
   python -m distance_transform_benchmarks_amd.gbench --images 4 --out gbench.json
+  python -m distance_transform_benchmarks_amd.gbench --corpus /data/dt_images --images 639
   python -m distance_transform_benchmarks_amd.gbench --compare gbench.json REF.json
 """
 from __future__ import annotations
@@ -63,18 +69,38 @@ def corpus_image(i: int):
     return img, mask
 
 
-def run(n_images: int, iterations: int, families, device="cuda:0"):
+def corpus_files(directory: str):
+    """The files of a corpus directory in sorted path order."""
+    return sorted(os.path.join(directory, f) for f in os.listdir(directory)
+                  if os.path.isfile(os.path.join(directory, f)))
+
+
+def file_image(path: str):
+    """A corpus file read with the library's imread, with one seed pixel at
+    the centre."""
+    from .imageio import imread
+
+    img = imread(path)
+    mask = np.zeros(img.shape, np.uint8)
+    mask[img.shape[0] // 2, img.shape[1] // 2] = 1
+    return img, mask
+
+
+def run(n_images: int, iterations: int, families, device="cuda:0", corpus: str = ""):
     import torch
 
     from . import api
 
     dev = torch.device(device)
+    files = corpus_files(corpus) if corpus else []
+    if corpus:
+        n_images = min(n_images, len(files))
     rows = []
     for fi, (fam, sched) in enumerate(FAMILIES):
         if sched not in families:
             continue
         for i in range(n_images):
-            img_np, mask_np = corpus_image(i)
+            img_np, mask_np = file_image(files[i]) if corpus else corpus_image(i)
             img = torch.from_numpy(img_np).to(dev)
             mask = torch.from_numpy(mask_np).to(dev)
             size = int(img_np.size)
@@ -94,7 +120,8 @@ def run(n_images: int, iterations: int, families, device="cuda:0"):
                              "run_name": name, "run_type": "iteration", "repetitions": 1, "repetition_index": 0,
                              "threads": 1, "iterations": iterations, "real_time": real, "cpu_time": real,
                              "time_unit": "ms", "bytes_per_second": size / (real * 1e-3), "lambda": lam,
-                             "size": float(size), "label": f"synthetic:blob_{i}_{img_np.shape[1]}x{img_np.shape[0]}"})
+                             "size": float(size),
+                             "label": files[i] if corpus else f"synthetic:blob_{i}_{img_np.shape[1]}x{img_np.shape[0]}"})
                 print(f"{name}: {real:.3f} ms ({size / real / 1e3:.1f} Mpix/s)", file=sys.stderr, flush=True)
             del img, mask
     ctx = {"date": datetime.datetime.now(datetime.timezone.utc).isoformat(timespec="seconds"),
@@ -115,8 +142,9 @@ def summarize(doc) -> dict:
 
 
 def main(argv=None) -> int:
-    ap = argparse.ArgumentParser(description="google-benchmark style GGDT benchmark over a synthetic corpus")
+    ap = argparse.ArgumentParser(description="google-benchmark style GGDT benchmark over a synthetic or file corpus")
     ap.add_argument("--images", type=int, default=4)
+    ap.add_argument("--corpus", default="", help="directory of images (sorted by path) instead of the synthetic corpus")
     ap.add_argument("--iterations", type=int, default=3)
     ap.add_argument("--families", default="raster,blocks,task")
     ap.add_argument("--out", default="")
@@ -128,7 +156,7 @@ def main(argv=None) -> int:
         for k in sorted(set(ours) | set(ref)):
             print(f"{k:<40} ours {ours.get(k, float('nan')):>10} Mpix/s   reference {ref.get(k, float('nan')):>10} Mpix/s")
         return 0
-    doc = run(a.images, a.iterations, set(a.families.split(",")))
+    doc = run(a.images, a.iterations, set(a.families.split(",")), corpus=a.corpus)
     txt = json.dumps(doc, indent=1)
     if a.out:
         with open(a.out, "w") as f:

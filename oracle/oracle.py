@@ -25,10 +25,50 @@ IMG_U8, IMG_RGB8, IMG_F32 = 0, 1, 2
 MASK_U8, MASK_F32 = 0, 1
 
 
+_REF_ROOT = "/root/reference"
+_STB_SRC = os.path.join(_REF_ROOT, "dt", "src", "stb_image_implementation.c")
+_STB_PATH = os.path.join(_HERE, "_ref", "libstb_image.so")
+_stb = None
+
+
 def build() -> str:
-    """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
+    """Compile the oracle with its Makefile (gcc, -ffp-contract=off), and the
+    reference's stb_image into _ref/ when the reference sources are present."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    if os.path.exists(_STB_SRC):
+        subprocess.run(["make", "-s", "-C", _HERE, "ref", f"REF={_REF_ROOT}"], check=True)
     return _LIB_PATH
+
+
+def stb_available() -> bool:
+    """True when the reference's stb_image has been built into oracle/_ref."""
+    return os.path.exists(_STB_PATH)
+
+
+def stb_load(path: str):
+    """The reference's imread decoding step (dt/src/imread.cpp:12,
+    stbi_load(filename, &w, &h, &n, 0)) through the reference's own
+    stb_image built in oracle/_ref.  Returns (pixels, n) with pixels of
+    shape (h, w, n), or raises RuntimeError with stb's failure reason."""
+    global _stb
+    if _stb is None:
+        lib = ctypes.CDLL(_STB_PATH)
+        P = ctypes.POINTER
+        lib.stbi_load.argtypes = [ctypes.c_char_p, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int), ctypes.c_int]
+        lib.stbi_load.restype = ctypes.c_void_p
+        lib.stbi_image_free.argtypes = [ctypes.c_void_p]
+        lib.stbi_failure_reason.restype = ctypes.c_char_p
+        _stb = lib
+    w, h, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    p = _stb.stbi_load(str(path).encode(), ctypes.byref(w), ctypes.byref(h), ctypes.byref(n), 0)
+    if not p:
+        raise RuntimeError(_stb.stbi_failure_reason().decode())
+    try:
+        size = w.value * h.value * n.value
+        px = np.frombuffer((ctypes.c_ubyte * size).from_address(p), np.uint8).copy()
+    finally:
+        _stb.stbi_image_free(p)
+    return px.reshape(h.value, w.value, n.value), n.value
 
 
 def _load():
