@@ -1048,64 +1048,114 @@ namespace dtk
       }
       else
       {
-        // Sweep pairs (column down+up, row right+left) in every wave at once;
-        // the shared rows are merged after each pair.  A forward sweep leaves
-        // its direction's edges satisfied except where it read a copy that it
-        // then changed: rows a neighbour wave also holds, and (row layout) a
-        // segment's first and last columns.  So a pair whose backward sweep
-        // changed nothing and whose forward sweep did not touch those leaves
-        // the tile stable in its direction ("clean"), and one pair of the other
-        // direction without any change then certifies the tile.  Before the
-        // first pair the tile counts as not clean, which makes an untouched
-        // tile take two quiet pairs.  After a clean column pair the W / E check
-        // (row_we_violated, voted over the waves) can certify the tile in place
-        // of the row pair.
-        int  par        = 0;
-        bool prev_clean = false;
-        for (;;)
+        int par = 0;
+        if constexpr (CM != 2)
         {
-          uint64_t       edge = 0;
-          const uint64_t cf   = sweep_col_fwd<RH, C8, TWO>(Dc, cc, ea, ed, edge);
-          const uint64_t cb   = sweep_col_bwd<RH, C8, TWO>(Dc, cc, ea, ed);
-          sweeps += 2;
-          any |= (cf | cb) != 0;
-          uint32_t all = exchange_rows<RH, NW>(
-            Dc, X, F, w, lane, par, ((cf | cb) != 0 ? 1u : 0u) | ((cb | edge) != 0 ? 2u : 0u), any);
-          par ^= 1;
-          if (!(all & 1u) && prev_clean)
+          // Per-pixel costs (lambda > 0): one exchange per column pair and
+          // row pair together.  A wave whose column pair is clean and whose
+          // W / E check passes is at its region's fixed point and skips the
+          // row pair; otherwise it runs the row pair and is at the fixed
+          // point only if neither pair changed anything.  Either way it
+          // changed no shared row then, so when every wave reports a fixed
+          // point the merge changes nothing and the tile is stable.  Half the
+          // workgroup barriers of the exchange after every pair (each costs
+          // the wait for the slowest of the four waves), and the W / E
+          // certification needs no vote of its own.
+          int iters = 0;
+          for (;;)
           {
-            conv = true;
-            break;
+            uint64_t       edge = 0;
+            const uint64_t cf   = sweep_col_fwd<RH, C8, TWO>(Dc, cc, ea, ed, edge);
+            const uint64_t cb   = sweep_col_bwd<RH, C8, TWO>(Dc, cc, ea, ed);
+            sweeps += 2;
+            ++iters;
+            any |= (cf | cb) != 0;
+            col_to_row<S>(Dc, R, T, lane);
+            bool fixed = (cb | edge) == 0 && !row_we_violated<SW, TWO>(R, cc, ea);
+            if (!fixed)
+            {
+              uint64_t       redge = 0;
+              const uint64_t rf    = sweep_row_fwd<SW, C8, TWO>(R, cc, ea, ed, redge);
+              const uint64_t rb    = sweep_row_bwd<SW, C8, TWO>(R, cc, ea, ed);
+              row_to_col<S>(R, Dc, T, lane);
+              sweeps += 2;
+              any |= (rf | rb) != 0;
+              fixed = (cf | cb | rf | rb) == 0;
+            }
+            const uint32_t all = exchange_rows<RH, NW>(Dc, X, F, w, lane, par, fixed ? 0u : 1u, any);
+            par ^= 1;
+            if (!(all & 1u))
+            {
+              conv = true;
+              break;
+            }
+            // The cap counts four sweeps per exchange in every wave, so the
+            // waves leave the loop together.
+            if (4 * iters >= cap)
+              break;
           }
-          prev_clean = !(all & 2u);
-          if (sweeps >= cap)
-            break;
-          col_to_row<S>(Dc, R, T, lane);
-          // A clean column pair leaves only the W and E edges to check; the
-          // column layout is still current, so a pass needs no transpose back.
-          if (prev_clean && !any_wave<NW>(row_we_violated<SW, TWO>(R, cc, ea), F, w, lane))
+        }
+        else
+        {
+          // Constant costs (lambda = 0, 32-row waves, which gain from merging
+          // more often): sweep pairs (column down+up, row right+left) in every
+          // wave at once, the shared rows merged after each pair.  A forward
+          // sweep leaves its direction's edges satisfied except where it read
+          // a copy that it then changed: rows a neighbour wave also holds, and
+          // (row layout) a segment's first and last columns.  So a pair whose
+          // backward sweep changed nothing and whose forward sweep did not
+          // touch those leaves the tile stable in its direction ("clean"), and
+          // one pair of the other direction without any change then certifies
+          // the tile.  Before the first pair the tile counts as not clean,
+          // which makes an untouched tile take two quiet pairs.  After a clean
+          // column pair the W / E check (row_we_violated, voted over the
+          // waves) can certify the tile in place of the row pair.
+          bool prev_clean = false;
+          for (;;)
           {
-            conv = true;
-            break;
+            uint64_t       edge = 0;
+            const uint64_t cf   = sweep_col_fwd<RH, C8, TWO>(Dc, cc, ea, ed, edge);
+            const uint64_t cb   = sweep_col_bwd<RH, C8, TWO>(Dc, cc, ea, ed);
+            sweeps += 2;
+            any |= (cf | cb) != 0;
+            uint32_t all = exchange_rows<RH, NW>(
+              Dc, X, F, w, lane, par, ((cf | cb) != 0 ? 1u : 0u) | ((cb | edge) != 0 ? 2u : 0u), any);
+            par ^= 1;
+            if (!(all & 1u) && prev_clean)
+            {
+              conv = true;
+              break;
+            }
+            prev_clean = !(all & 2u);
+            if (sweeps >= cap)
+              break;
+            col_to_row<S>(Dc, R, T, lane);
+            // A clean column pair leaves only the W and E edges to check; the
+            // column layout is still current, so a pass needs no transpose back.
+            if (prev_clean && !any_wave<NW>(row_we_violated<SW, TWO>(R, cc, ea), F, w, lane))
+            {
+              conv = true;
+              break;
+            }
+            edge              = 0;
+            const uint64_t rf = sweep_row_fwd<SW, C8, TWO>(R, cc, ea, ed, edge);
+            const uint64_t rb = sweep_row_bwd<SW, C8, TWO>(R, cc, ea, ed);
+            row_to_col<S>(R, Dc, T, lane);
+            sweeps += 2;
+            any |= (rf | rb) != 0;
+            edge |= rf & shared_row_lanes<S>();
+            all = exchange_rows<RH, NW>(Dc, X, F, w, lane, par,
+                                        ((rf | rb) != 0 ? 1u : 0u) | ((rb | edge) != 0 ? 2u : 0u), any);
+            par ^= 1;
+            if (!(all & 1u) && prev_clean)
+            {
+              conv = true;
+              break;
+            }
+            prev_clean = !(all & 2u);
+            if (sweeps >= cap)
+              break;
           }
-          edge              = 0;
-          const uint64_t rf = sweep_row_fwd<SW, C8, TWO>(R, cc, ea, ed, edge);
-          const uint64_t rb = sweep_row_bwd<SW, C8, TWO>(R, cc, ea, ed);
-          row_to_col<S>(R, Dc, T, lane);
-          sweeps += 2;
-          any |= (rf | rb) != 0;
-          edge |= rf & shared_row_lanes<S>();
-          all = exchange_rows<RH, NW>(Dc, X, F, w, lane, par,
-                                      ((rf | rb) != 0 ? 1u : 0u) | ((rb | edge) != 0 ? 2u : 0u), any);
-          par ^= 1;
-          if (!(all & 1u) && prev_clean)
-          {
-            conv = true;
-            break;
-          }
-          prev_clean = !(all & 2u);
-          if (sweeps >= cap)
-            break;
         }
       }
 
