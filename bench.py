@@ -110,6 +110,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the batched comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable (adjoint) leg")
     ap.add_argument("--no-maze", action="store_true", help="skip the maze leg")
+    ap.add_argument("--no-orders", action="store_true", help="skip the per-lambda queue-order comparison")
+    ap.add_argument("--no-blob512", action="store_true", help="skip the blob512 leg (and its single-thread CPU run)")
     ap.add_argument("--batch-count", type=int, default=256)
     a = ap.parse_args(argv)
 
@@ -154,6 +156,8 @@ def main(argv=None) -> int:
                                 "sweeps": st["sweeps"], "tiles": st["tiles"], "tile_rows": st["tile_rows"],
                                 "workers": st["workers"]}
         log(f"lambda={lam}: {ms:.3f} ms/call, {per_lambda[str(lam)]['mpix_s']} Mpix/s, stats {st}")
+        if a.no_orders:
+            continue
         orders = {}
         for order in ("fifo", "key", "seed"):
             okw = dict(kw, queue_order=order)
@@ -212,7 +216,8 @@ def main(argv=None) -> int:
     if not a.no_maze:
         result["maze"] = bench_maze(dt, synth, torch, dev)
 
-    result["blob512"] = bench_blob512(dt, synth, torch, dev, a.no_cpu)
+    if not a.no_blob512:
+        result["blob512"] = bench_blob512(dt, synth, torch, dev, a.no_cpu)
 
     if not a.no_cpu:
         from oracle import oracle as O
@@ -235,7 +240,7 @@ def main(argv=None) -> int:
             "threads_note": f"OpenMP over the affinity cores, at most {CPU_SHARE} (one GPU's share of the box)",
             "runs_s": [round(t, 3) for t in runs], "timer": "monotonic wall clock (time.perf_counter)",
             "gpu_bit_exact": exact,
-            "single_thread": result["blob512"].get("cpu_single_thread"),
+            "single_thread": result.get("blob512", {}).get("cpu_single_thread"),
         }
         log(f"cpu baseline: {cpu_s:.2f}s median ({res['iters']} iterations, {threads} of {aff} affinity cores, "
             f"{hw} hardware threads); GPU bit-exact: {exact}")
