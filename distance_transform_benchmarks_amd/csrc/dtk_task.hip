@@ -621,10 +621,14 @@ namespace dtk
       if (((border >> lane) & 1u) && nx >= 0 && ny >= 0 && nx < d.tiles_x && ny < d.tiles_y)
       {
         target      = d.tile0 + ny * d.tiles_x + nx;
-        uint32_t kb = Kw[lane];
+        uint32_t kb = kInfBits; // wake keys are formed for the KEY order only
+        if (q.order == ORDER_KEY)
+        {
+          kb = Kw[lane];
 #pragma unroll
-        for (int w = 1; w < NW; ++w)
-          kb = min(kb, Kw[8 * w + lane]);
+          for (int w = 1; w < NW; ++w)
+            kb = min(kb, Kw[8 * w + lane]);
+        }
         insert = wake_tile(q, target, kb, &bucket);
       }
     }
@@ -810,7 +814,7 @@ namespace dtk
 #endif
         return k;
       };
-      visit_tile<S, C8, CM, NW, true>(d, p, tx, ty, T[w], Sv[w], X, F, Kw, after);
+      visit_tile<S, C8, CM, NW, true>(d, p, tx, ty, T[w], Sv[w], X, F, q.order == ORDER_KEY ? Kw : nullptr, after);
       kept += keeps;
     }
     if (threadIdx.x == 0 && visits)

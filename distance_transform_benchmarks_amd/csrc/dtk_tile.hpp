@@ -880,8 +880,9 @@ namespace dtk
   // KEYS: also report, per neighbour k (border-bit order), the smallest value
   // of this visit's improved halo pixels that belong to neighbour k (float
   // bits; +inf when none): Kw[8 * w + k] for wave w, written before the
-  // visit's last workgroup barrier.  The bucket orders queue a woken
-  // neighbour by it.
+  // visit's last workgroup barrier, when Kw is not null (a launch-wide
+  // choice: the KEY order queues a woken neighbour by it, the other orders
+  // pass null and skip the reductions).
   //
   // AFTER: called with each visit's result once its stores are issued (every
   // thread of the workgroup, the same result in each).  It returns true to
@@ -1189,7 +1190,7 @@ namespace dtk
         const uint64_t top  = ballot(timp);
         const uint64_t bot  = ballot(bimp);
         const uint64_t cols = ballot(colimp);
-        if constexpr (KEYS)
+        if (KEYS && Kw != nullptr)
         {
           // Per wave: L, R from the halo columns; T, TL, TR from row 0 of
           // the top wave; B, BL, BR from the last row of the bottom wave.
@@ -1258,7 +1259,7 @@ namespace dtk
           }
         }
       }
-      if constexpr (KEYS)
+      if (KEYS && Kw != nullptr)
       {
         if (!any && lane < 8)
           Kw[8 * w + lane] = 0x7f800000u;
