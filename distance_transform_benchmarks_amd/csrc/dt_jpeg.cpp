@@ -257,6 +257,7 @@ namespace dtio
       bool build(const int count[16])
       {
         std::memset(fast, 0, sizeof(fast));
+        std::memset(value, 0, sizeof(value)); // symbols past the table's count read as 0
         uint32_t code = 0;
         int      k    = 0;
         for (int len = 1; len <= 16; ++len)
