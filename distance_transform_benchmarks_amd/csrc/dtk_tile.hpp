@@ -994,6 +994,33 @@ namespace dtk
       int         quiet = 0, sweeps = 0;
       bool        conv = false, any = false;
       float       R[SW + 2];
+      // The current stored values of the region's halo pixels, which other
+      // tiles own, for the wake test after the sweeps (constant costs only):
+      // a neighbour needs a visit when this visit found a value below the one
+      // it holds now.  Comparing with the values as loaded at the start of the
+      // visit instead also wakes a neighbour whose own concurrent visit has
+      // already stored as good a value, for a visit that changes nothing (a
+      // fifth of the visits at lambda 0).  Agent-scope loads, not cached in
+      // this L2, see the neighbours' latest stores; they are issued before
+      // this visit's own stores.  With per-pixel costs the reload is not
+      // worth its latency (the loads cannot be overlapped without spilling).
+      float       hv[RH];
+      auto        now = [&](float loaded, int r) {
+        if constexpr (CM == 2)
+          return minf_nn(loaded, hv[r]);
+        else
+          return loaded;
+      };
+      auto        reload_halo = [&]() {
+#pragma unroll
+        for (int r = 0; r < RH; ++r)
+        {
+          const int cy = min(max(Y0 + r, 0), d.H - 1);
+          hv[r]        = __hip_atomic_load(reinterpret_cast<const float*>(reinterpret_cast<const unsigned char*>(d.D) +
+                                                                          (long long)cy * dpitch) + vx,
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      };
       if constexpr (NW == 1)
       {
         for (;;)
@@ -1168,6 +1195,23 @@ namespace dtk
       uint32_t b    = 0;
       if (any)
       {
+        if constexpr (CM == 2)
+          reload_halo();
+        // Write back the owned pixels.
+        if (lane >= 1 && lane <= 62 && cin)
+        {
+#pragma unroll
+          for (int r = 1; r < RH - 1; ++r)
+          {
+            const int gy = Y0 + r;
+            if (gy < d.H) // device-scope store: written through, so the
+                          // release fence has no dirty L2 lines of D to write back
+              __hip_atomic_store(reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(d.D) + (long long)gy * dpitch) + vx,
+                                 Dc[r], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+          }
+        }
+
         // Only pixels inside the image count: with constant costs the pixels
         // outside it start at +inf on every visit and always decrease, which
         // would wake the neighbours (and through them this tile) forever.
@@ -1179,14 +1223,14 @@ namespace dtk
 #pragma unroll
           for (int r = 1; r < RH - 1; ++r)
           {
-            const bool imp = Y0 + r < d.H && lt_nn(Dc[r], Sv[o + r]);
+            const bool imp = Y0 + r < d.H && lt_nn(Dc[r], now(Sv[o + r], r));
             colimp |= imp;
             if constexpr (KEYS)
               colkey = imp ? min(colkey, __float_as_uint(Dc[r])) : colkey;
           }
         }
-        const bool     timp = cin && Y0 >= 0 && lt_nn(Dc[0], init_top);
-        const bool     bimp = cin && Y0 + RH - 1 < d.H && lt_nn(Dc[RH - 1], init_bot);
+        const bool     timp = cin && Y0 >= 0 && lt_nn(Dc[0], now(init_top, 0));
+        const bool     bimp = cin && Y0 + RH - 1 < d.H && lt_nn(Dc[RH - 1], now(init_bot, RH - 1));
         const uint64_t top  = ballot(timp);
         const uint64_t bot  = ballot(bimp);
         const uint64_t cols = ballot(colimp);
@@ -1244,20 +1288,6 @@ namespace dtk
           b |= (uint32_t)((bot >> 63) & 1u) << 7;   // BR
         }
 
-        // Write back the owned pixels.
-        if (lane >= 1 && lane <= 62 && cin)
-        {
-#pragma unroll
-          for (int r = 1; r < RH - 1; ++r)
-          {
-            const int gy = Y0 + r;
-            if (gy < d.H) // device-scope store: written through, so the
-                          // release fence has no dirty L2 lines of D to write back
-              __hip_atomic_store(reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(d.D) + (long long)gy * dpitch) + vx,
-                                 Dc[r], __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_AGENT);
-          }
-        }
       }
       if (KEYS && Kw != nullptr)
       {
