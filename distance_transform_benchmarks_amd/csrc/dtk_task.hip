@@ -814,7 +814,8 @@ namespace dtk
 #endif
         return k;
       };
-      visit_tile<S, C8, CM, NW, true>(d, p, tx, ty, T[w], Sv[w], X, F, q.order == ORDER_KEY ? Kw : nullptr, after);
+      visit_tile<S, C8, CM, NW, true, true>(d, p, tx, ty, T[w], Sv[w], X, F, q.order == ORDER_KEY ? Kw : nullptr,
+                                             after);
       kept += keeps;
     }
     if (threadIdx.x == 0 && visits)

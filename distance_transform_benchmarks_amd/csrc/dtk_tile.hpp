@@ -896,7 +896,10 @@ namespace dtk
     __device__ bool operator()(const VisitResult&) const { return false; }
   };
 
-  template <int S, bool C8, int CM, int NW, bool KEYS = false, class After = NoRevisit>
+  // FRESH: with constant costs, test the wakes against the neighbours'
+  // current values (below).  The task schedule gains from it; the blocks
+  // schedule, whose rounds wait for their slowest tile, does not.
+  template <int S, bool C8, int CM, int NW, bool KEYS = false, bool FRESH = false, class After = NoRevisit>
   __device__ __attribute__((always_inline)) VisitResult visit_tile(const ImgDesc& d, const KParams& p, int tx, int ty,
                                                                    float* T, float* Sv, float* X, uint32_t* F,
                                                                    uint32_t* Kw = nullptr, After after = After{})
@@ -1006,7 +1009,7 @@ namespace dtk
       // worth its latency (the loads cannot be overlapped without spilling).
       float       hv[RH];
       auto        now = [&](float loaded, int r) {
-        if constexpr (CM == 2)
+        if constexpr (FRESH && CM == 2)
           return minf_nn(loaded, hv[r]);
         else
           return loaded;
@@ -1195,7 +1198,7 @@ namespace dtk
       uint32_t b    = 0;
       if (any)
       {
-        if constexpr (CM == 2)
+        if constexpr (FRESH && CM == 2)
           reload_halo();
         // Write back the owned pixels.
         if (lane >= 1 && lane <= 62 && cin)
