@@ -124,6 +124,19 @@ def report(a) -> str:
         tot = sum(float(r["TotalDurationNs"]) for r in kt)
         n = sum(int(r["Calls"]) for r in kt)
         out += ["", f"{a.kernel}: {n} dispatches, average {tot / n / 1e6:.4f} ms"]
+    if a.split_us > 0:
+        # Two workloads in one run that launch the same kernel (a long solve
+        # and the short ones around it) are told apart by duration.
+        traces = sorted(glob.glob(os.path.join(a.trace, "**", "*kernel_trace.csv"), recursive=True))
+        if not traces:
+            raise FileNotFoundError(f"no kernel_trace.csv under {a.trace}")
+        durs = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+                for r in csv.DictReader(open(traces[0], newline="")) if a.kernel in r["Kernel_Name"]]
+        for label, sel in (("at least", [d for d in durs if d >= a.split_us]),
+                           ("under", [d for d in durs if d < a.split_us])):
+            if sel:
+                out.append(f"  {len(sel)} dispatches {label} {a.split_us:g} us: average {sum(sel) / len(sel) / 1e3:.4f} ms, "
+                           f"min {min(sel) / 1e3:.4f}, max {max(sel) / 1e3:.4f}")
     if bench:
         rl = bench["roofline"]
         out += [f"bench line of the same run: {a.kernel} average from HIP events on its stream {rl['kernel_avg_ms']} ms "
@@ -168,6 +181,8 @@ def main(argv=None) -> int:
     r.add_argument("--bench", default="")
     r.add_argument("--traffic", default="")
     r.add_argument("--title", default="kernel trace")
+    r.add_argument("--split-us", type=float, default=0.0,
+                   help="also report the kernel's dispatches at least / under this many us apart (kernel_trace.csv)")
     r.add_argument("--commands", nargs="*", default=[])
     r.add_argument("--out", default="")
     a = ap.parse_args(argv)
