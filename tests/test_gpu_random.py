@@ -1,14 +1,16 @@
-"""Seeded random sweep of the tile schedules against the CPU oracle.
+"""Seeded random sweep of the three schedules against the CPU oracle.
 
 The other parity files cover each option on a fixed matrix of cases.  This
 one draws the options together: image kind (random or smooth uint8, RGB8,
 float32), mask kind and seed count (none included), size (1 to 600 on each
 side, so ragged last tiles of every shape), lambda (the single-add values
 0 and 1, the two-add values in between, a random one), C4 / C8, schedule,
-tile shape and queue order; and batches of 1 to 8 such images sharing one
-queue.  Every result must equal the oracle's converged raster map bit for
-bit (the fixed point does not depend on the schedule).  Each case prints
-its parameters before it runs.
+tile shape and queue order; batches of 1 to 8 such images sharing one
+queue; and the raster schedule, converged or after a fixed number of
+iterations.  Every converged result must equal the oracle's converged
+raster map bit for bit (the fixed point does not depend on the schedule),
+and a fixed iteration count the oracle's map after as many iterations.
+Each case prints its parameters before it runs.
 """
 from __future__ import annotations
 
@@ -76,6 +78,30 @@ def test_random_single_images(dev, oracle, seed):
         D = dt.ggdt(_dev(img, dev), _dev(m, dev), lam, v, connectivity=8 if c8 else 4, schedule=sched, **kw)
         torch.cuda.synchronize()
         want, _ = oracle.raster(img, m, lam, v, conn8=c8, threads=8)
+        assert_bit_exact(D.cpu().numpy(), want, json.dumps(case))
+
+
+@pytest.mark.parametrize("seed", [404])
+def test_random_raster(dev, oracle, seed):
+    """The raster schedule, converged or after 1 to 3 full iterations (each
+    step reads the previous step's final values, as in the reference)."""
+    import torch
+
+    import distance_transform_benchmarks_amd as dt
+
+    rng = np.random.default_rng(seed)
+    for _ in range(30):
+        ik, mk, lam, c8, W, H = _draw(rng, 300)
+        iters = int(rng.choice([0, 0, 1, 2, 3]))
+        rows = int(rng.choice([16, 32]))
+        s, img, m = _inputs(rng, ik, mk, W, H)
+        v = 1e10 if mk == "u8" else 1e3
+        case = dict(img=ik, mask=mk, W=W, H=H, seed=s, lam=lam, c8=c8, v=v, max_iterations=iters, tile_rows=rows)
+        _say(json.dumps(case))
+        D = dt.ggdt(_dev(img, dev), _dev(m, dev), lam, v, connectivity=8 if c8 else 4, schedule="raster",
+                    max_iterations=iters, tile_rows=rows)
+        torch.cuda.synchronize()
+        want, _ = oracle.raster(img, m, lam, v, conn8=c8, max_iters=iters, threads=8)
         assert_bit_exact(D.cpu().numpy(), want, json.dumps(case))
 
 
