@@ -6,11 +6,12 @@ float32), mask kind and seed count (none included), size (1 to 600 on each
 side, so ragged last tiles of every shape), lambda (the single-add values
 0 and 1, the two-add values in between, a random one), C4 / C8, schedule,
 tile shape and queue order; batches of 1 to 8 such images sharing one
-queue; and the raster schedule, converged or after a fixed number of
-iterations.  Every converged result must equal the oracle's converged
-raster map bit for bit (the fixed point does not depend on the schedule),
-and a fixed iteration count the oracle's map after as many iterations.
-Each case prints its parameters before it runs.
+queue; the raster schedule, converged or after a fixed number of
+iterations; and predecessor codes and adjoint gradients.  Every converged
+result must equal the oracle's converged raster map bit for bit (the fixed
+point does not depend on the schedule), and a fixed iteration count the
+oracle's map after as many iterations.  Each case prints its parameters
+before it runs.
 """
 from __future__ import annotations
 
@@ -129,3 +130,43 @@ def test_random_batches(dev, oracle, seed):
         for i, (_, img, m) in enumerate(data):
             want, _ = oracle.raster(img, m, lam, v, conn8=c8, threads=8)
             assert_bit_exact(Ds[i].cpu().numpy(), want, f"image {i} of {json.dumps(case)}")
+
+
+@pytest.mark.parametrize("seed", [505])
+def test_random_predecessors_and_adjoint(dev, oracle, seed):
+    """Predecessor codes of every image kind equal the oracle's argmin rule;
+    for float32 images with soft masks the adjoint of sum(W * D) matches the
+    CPU reference gradient to 1e-4 of the largest one."""
+    import torch
+
+    import distance_transform_benchmarks_amd as dt
+
+    rng = np.random.default_rng(seed)
+    for _ in range(24):
+        ik, mk, lam, c8, W, H = _draw(rng, 200)
+        if ik == "u8s":
+            ik = "u8"
+        if rng.random() < 0.5:  # the differentiable kinds
+            ik, mk = "f32", "f32"
+        s, img, m = _inputs(rng, ik, mk, W, H)
+        v = 1e10 if mk == "u8" else 1e3
+        conn = 8 if c8 else 4
+        case = dict(img=ik, mask=mk, W=W, H=H, seed=s, lam=lam, c8=c8, v=v)
+        _say("adjoint", json.dumps(case))
+        di, dm = _dev(img, dev), _dev(m, dev)
+        D = dt.ggdt(di, dm, lam, v, connectivity=conn)
+        P = dt.predecessors(di, dm, D, lam, v, connectivity=conn)
+        torch.cuda.synchronize()
+        Dr, _ = oracle.raster(img, m, lam, v, conn8=c8, threads=8)
+        assert_bit_exact(D.cpu().numpy(), Dr, json.dumps(case))
+        Pr = oracle.pred(img, m, Dr, lam, v, conn8=c8)
+        assert np.array_equal(P.cpu().numpy(), Pr), json.dumps(case)
+        if ik != "f32" or mk != "f32":
+            continue
+        Wt = rng.random((H, W)).astype(np.float32)
+        gi, gm = dt.backward(di, dm, D, P, _dev(Wt, dev), lam, v, connectivity=conn)
+        torch.cuda.synchronize()
+        gir, gmr, _ = oracle.backward(img, Dr, Pr, Wt, lam, v)
+        for got, want, what in ((gi, gir, "dL/dimage"), (gm, gmr, "dL/dmask")):
+            err = np.abs(got.cpu().numpy().astype(np.float64) - want).max()
+            assert err <= 1e-4 * max(1.0, np.abs(want).max()), (what, err, json.dumps(case))
