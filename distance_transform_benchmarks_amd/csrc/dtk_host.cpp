@@ -862,12 +862,12 @@ dtk_status dtk_ggdt(const dtk_image* img, const dtk_image* mask, dtk_image* dist
   if (p->schedule == DTK_SCHED_RASTER)
   {
     // Strips of a pass wait on each other, so all of them must be resident.
-    if ((long long)raster_strips(d) > (long long)w->cu * raster_occupancy(kp, c8))
+    if ((long long)raster_strips(d, kp) > (long long)w->cu * raster_occupancy(kp, c8))
       return fail(DTK_ERR_UNSUPPORTED, "image too large for the resident raster pipeline");
-    // Chunk numbers (16 steps each) must stay below 2^16 in the stamps.
+    // Chunk numbers (16 or 20 steps each) must stay below 2^16 in the stamps.
     if (d.W >= (1 << 19) || d.H >= (1 << 19))
       return fail(DTK_ERR_UNSUPPORTED, "raster schedule: at most 2^19 - 1 pixels per side");
-    const size_t mail_bytes = raster_mail_words(d) * sizeof(unsigned long long);
+    const size_t mail_bytes = raster_mail_words(d, kp) * sizeof(unsigned long long);
     if ((st = ws_reserve(*w, 4096 + mail_bytes, 4096)) != DTK_OK)
       return st;
     uint32_t*           changed = static_cast<uint32_t*>(w->dbuf);

@@ -25,8 +25,8 @@ namespace dtk
                                      unsigned long long* mail, uint32_t pass0, hipStream_t s);
   // Strips of the longest pass (all resident at once) and the mailbox words
   // of their chunk handoff.
-  size_t raster_strips(const ImgDesc& d);
-  size_t raster_mail_words(const ImgDesc& d);
+  size_t raster_strips(const ImgDesc& d, const KParams& p);
+  size_t raster_mail_words(const ImgDesc& d, const KParams& p);
   // Strips (one 64-lane workgroup each) of one raster pass kernel that can be
   // resident on one CU at once; every strip of a pass must be resident.
   int        raster_occupancy(const KParams& p, bool c8);

@@ -7,12 +7,13 @@
 // oracle runs exactly this schedule, so after N iterations GPU and CPU
 // agree bit-for-bit (and at convergence for any N).
 //
-// MI355X mapping.  A pass is split into strips of R = 32 lines (rows for a
-// row pass, columns for a column pass).  One wave holds 64 lanes = its 32
-// lines plus G = 16 ghost lines on each side, and advances in chunks of
-// C = G steps: at step j of a chunk, lane l is exact when lanes l-1..l+1
-// were exact at step j-1, so after C steps the 32 owned lanes are exact even
-// though the outermost ghost lanes had no neighbours.  A chunk needs, for
+// MI355X mapping.  A pass is split into strips of R = 64 - 2G lines (rows
+// for a row pass, columns for a column pass; G = 16, R = 32 at lambda 0,
+// G = 20, R = 24 otherwise).  One wave holds 64 lanes = its R lines plus G
+// ghost lines on each side, and advances in chunks of C = G steps: at step
+// j of a chunk, lane l is exact when lanes l-1..l+1 were exact at step j-1,
+// so after C steps the R owned lanes are exact even though the outermost
+// ghost lanes had no neighbours.  A chunk needs, for
 // its ghost lanes, the neighbour strips' final values of the step before it
 // (their other values give the same result whenever they are read).  So at
 // the end of every chunk a strip publishes the final values of its first
@@ -35,14 +36,19 @@ namespace dtk
 {
   namespace
   {
-#ifndef DTK_RASTER_GHOST
-#define DTK_RASTER_GHOST 16
-#endif
-    constexpr int kGhost  = DTK_RASTER_GHOST; // ghost lines per side == steps per chunk
-    constexpr int kStripR = 64 - 2 * kGhost;  // owned lines per strip
+    // Ghost lines per side (== steps per chunk) of the passes of cost mode
+    // CM: 16 for the chamfer steps of lambda 0, 20 for image costs.  A chunk
+    // costs a fixed handshake plus a part that grows with its steps; with
+    // image costs the handshake weighs more, and the fewer chunks of G = 20
+    // pay (4-5% on the synthetic corpus against G = 16, 11% slower at
+    // lambda 0).  20 is the widest multiple of four (for the row vectors)
+    // whose ghost lines still lie in the two neighbour strips.
+    __host__ __device__ constexpr int raster_ghost(int cm) { return cm == 0 ? 16 : 20; }
+    __host__ __device__ constexpr int raster_owned(int cm) { return 64 - 2 * raster_ghost(cm); }
     // A strip waits only for its two neighbours, so its ghost lines must lie
     // within them.
-    static_assert(kGhost <= kStripR, "ghost lines must not reach beyond the neighbour strips");
+    static_assert(raster_ghost(0) <= raster_owned(0) && raster_ghost(2) <= raster_owned(2),
+                  "ghost lines must not reach beyond the neighbour strips");
   }
 
   __global__ void k_raster_init(ImgDesc d, KParams p, uint32_t* err)
@@ -77,9 +83,10 @@ namespace dtk
 
   // Mailbox word of strip s: side 0 = its first G owned lines, side 1 = its
   // last G; par = chunk parity; k = line within the side.
+  template <int G>
   __device__ __forceinline__ size_t mail_index(int strip, int side, int par, int k)
   {
-    return ((size_t)(strip * 2 + side) * 2 + par) * kGhost + k;
+    return ((size_t)(strip * 2 + side) * 2 + par) * G + k;
   }
 
   // Image value of one pixel as the raster passes hold it: uint8 as a float
@@ -145,20 +152,21 @@ namespace dtk
     return g;
   }
 
-  // Sixteen consecutive pixels of one row, for a whole chunk of a row pass
-  // (a lane is a row there, so every scalar access of a step touches its own
-  // cache line).  Float rows are only 4-byte aligned, and 16-byte loads and
-  // stores need only dword alignment.
+  // N consecutive pixels of one row (N = G, a multiple of four), for a
+  // whole chunk of a row pass (a lane is a row there, so every scalar access
+  // of a step touches its own cache line).  Float rows are only 4-byte
+  // aligned, and 16-byte loads and stores need only dword alignment.
   typedef float f4d __attribute__((ext_vector_type(4), aligned(4)));
   typedef const __attribute__((address_space(1))) f4d* gf4d_r;
   typedef __attribute__((address_space(1))) f4d*       gf4d_w;
 
-  __device__ __forceinline__ void load_f32_row16(const unsigned char* base, long long pitch, int y, int x0,
-                                                 float (&v)[16])
+  template <int N>
+  __device__ __forceinline__ void load_f32_row(const unsigned char* base, long long pitch, int y, int x0, float (&v)[N])
   {
+    static_assert(N % 4 == 0, "row vectors of whole dwordx4");
     const gf4d_r q = reinterpret_cast<gf4d_r>((gbytes)base + (long long)y * pitch + 4ll * x0);
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < N / 4; ++k)
     {
       const f4d t = q[k];
       v[4 * k]     = t.x;
@@ -168,32 +176,37 @@ namespace dtk
     }
   }
 
-  __device__ __forceinline__ void store_f32_row16(void* base, long long pitch, int y, int x0, const float (&v)[16])
+  template <int N>
+  __device__ __forceinline__ void store_f32_row(void* base, long long pitch, int y, int x0, const float (&v)[N])
   {
+    static_assert(N % 4 == 0, "row vectors of whole dwordx4");
     const gf4d_w q = reinterpret_cast<gf4d_w>((gbytes_w)base + (long long)y * pitch + 4ll * x0);
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < N / 4; ++k)
       q[k] = f4d{v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]};
   }
 
-  // Bytes x0 .. x0+15 of a uint8 row as floats: the four or five aligned
-  // dwords that hold them, realigned with v_alignbyte.  The fifth dword is
-  // read only when x0 is not 4-aligned, and then it holds byte x0+15, so no
+  // Bytes x0 .. x0+N-1 of a uint8 row as floats: the N/4 or N/4 + 1 aligned
+  // dwords that hold them, realigned with v_alignbyte.  The last dword is
+  // read only when x0 is not 4-aligned, and then it holds byte x0+N-1, so no
   // dword without a pixel of the row is read.
-  __device__ __forceinline__ void load_u8_row16(const ImgDesc& d, int y, int x0, float (&v)[16])
+  template <int N>
+  __device__ __forceinline__ void load_u8_row(const ImgDesc& d, int y, int x0, float (&v)[N])
   {
+    static_assert(N % 4 == 0, "whole dwords");
+    constexpr int   M  = N / 4;
     const uintptr_t a  = reinterpret_cast<uintptr_t>(d.img + (long long)y * d.img_pitch + x0);
     const uint32_t  sh = (uint32_t)(a & 3u);
     const auto*     q  = reinterpret_cast<const __attribute__((address_space(1))) uint32_t*>(a - sh);
-    uint32_t        w[5];
+    uint32_t        w[M + 1];
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < M; ++k)
       w[k] = q[k];
-    w[4] = 0u;
+    w[M] = 0u;
     if (sh != 0u)
-      w[4] = q[4];
+      w[M] = q[M];
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < M; ++k)
     {
       const uint32_t b = __builtin_amdgcn_alignbyte(w[k + 1], w[k], sh);
       v[4 * k]         = (float)(b & 255u);
@@ -221,7 +234,9 @@ namespace dtk
   __global__ void __launch_bounds__(64) k_raster_pass(ImgDesc d, KParams p, uint32_t* changed,
                                                       unsigned long long* mail, uint32_t epoch)
   {
-    using T            = typename RasterPix<KIND>::T;
+    using T                 = typename RasterPix<KIND>::T;
+    constexpr int kGhost    = raster_ghost(CM);
+    constexpr int kStripR   = raster_owned(CM);
     const int lane     = threadIdx.x;
     const int strip    = blockIdx.x;
     const int nlines   = ROWS ? d.H : d.W; // lines (lanes) dimension
@@ -264,7 +279,7 @@ namespace dtk
 
       T     Iv[kGhost];
       float Dv[kGhost];
-      // A whole chunk of a row pass covers pixels xlo .. xlo+15 of each
+      // A whole chunk of a row pass covers pixels xlo .. xlo+G-1 of each
       // lane's row, in step order for a forward pass and reversed for a
       // backward one; it is loaded (and stored) as row vectors.
       const bool rowvec = ROWS && kn == kGhost;
@@ -275,9 +290,9 @@ namespace dtk
         {
           float t[kGhost];
           if constexpr (KIND == IMG_U8)
-            load_u8_row16(d, cl, xlo, t);
+            load_u8_row<kGhost>(d, cl, xlo, t);
           else
-            load_f32_row16(d.img, d.img_pitch, cl, xlo, t);
+            load_f32_row<kGhost>(d.img, d.img_pitch, cl, xlo, t);
 #pragma unroll
           for (int j = 0; j < kGhost; ++j)
             Iv[j] = (T)t[FWD ? j : kGhost - 1 - j];
@@ -295,7 +310,7 @@ namespace dtk
       if (rowvec)
       {
         float t[kGhost];
-        load_f32_row16(reinterpret_cast<const unsigned char*>(d.D), d.D_pitch, cl, xlo, t);
+        load_f32_row<kGhost>(reinterpret_cast<const unsigned char*>(d.D), d.D_pitch, cl, xlo, t);
 #pragma unroll
         for (int j = 0; j < kGhost; ++j)
           Dv[j] = lin ? t[FWD ? j : kGhost - 1 - j] : DTK_INF;
@@ -321,8 +336,8 @@ namespace dtk
       const bool   lower = lane < kGhost && strip > 0;
       const bool   upper = lane >= kGhost + kStripR && strip + 1 < nstrips;
       const bool   need  = c > 0 && (lower || upper) && lin;
-      const size_t at    = lower ? mail_index(strip - 1, 1, c & 1, lane)
-                                 : mail_index(strip + 1, 0, c & 1, max(lane - kGhost - kStripR, 0));
+      const size_t at    = lower ? mail_index<kGhost>(strip - 1, 1, c & 1, lane)
+                                 : mail_index<kGhost>(strip + 1, 0, c & 1, max(lane - kGhost - kStripR, 0));
       const uint32_t     want = epoch | (uint32_t)c;
       unsigned long long w    = 0;
       if (c == 0)
@@ -422,7 +437,7 @@ namespace dtk
 #pragma unroll
         for (int j = 0; j < kGhost; ++j)
           t[FWD ? j : kGhost - 1 - j] = Dv[j];
-        store_f32_row16(d.D, d.D_pitch, line, xlo, t);
+        store_f32_row<kGhost>(d.D, d.D_pitch, line, xlo, t);
       }
       else if (own)
       {
@@ -445,13 +460,19 @@ namespace dtk
       // lines (Dprev holds the last step's value; every chunk but the last
       // has G steps).
       // First G owned lanes: [G, 2G) -> side 0; last G: [R, R + G) -> side 1.
+      // With 2G > R the two sets overlap (G = 20, R = 24: lanes 24..39), and
+      // a lane in both publishes to both sides.
       const bool first = lane >= kGhost && lane < 2 * kGhost;
       const bool last  = lane >= kStripR && lane < kStripR + kGhost;
-      if (c + 1 < nchunks && (first || last))
+      if (c + 1 < nchunks)
       {
         const unsigned long long wv = ((unsigned long long)(epoch | (uint32_t)(c + 1)) << 32) | __float_as_uint(Dprev);
-        __hip_atomic_store(&mail[mail_index(strip, first ? 0 : 1, (c + 1) & 1, first ? lane - kGhost : lane - kStripR)],
-                           wv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (first)
+          __hip_atomic_store(&mail[mail_index<kGhost>(strip, 0, (c + 1) & 1, lane - kGhost)], wv, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+        if (last)
+          __hip_atomic_store(&mail[mail_index<kGhost>(strip, 1, (c + 1) & 1, lane - kStripR)], wv, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
       }
     }
     if ((ch & own_bits) != 0 && lane == 0)
@@ -464,8 +485,9 @@ namespace dtk
   static hipError_t raster_iter_t(const ImgDesc& d, const KParams& p, uint32_t* changed, unsigned long long* mail,
                                   uint32_t pass0, hipStream_t s)
   {
-    const int rs = (d.H + kStripR - 1) / kStripR; // row-pass strips
-    const int cs = (d.W + kStripR - 1) / kStripR; // column-pass strips
+    constexpr int R  = raster_owned(CM);
+    const int     rs = (d.H + R - 1) / R; // row-pass strips
+    const int     cs = (d.W + R - 1) / R; // column-pass strips
     hipLaunchKernelGGL((k_raster_pass<KIND, true, true, C8, CM>), dim3(rs), dim3(64), 0, s, d, p, changed, mail,
                        (pass0 + 0u) << 16);
     hipLaunchKernelGGL((k_raster_pass<KIND, true, false, C8, CM>), dim3(rs), dim3(64), 0, s, d, p, changed, mail,
@@ -566,12 +588,16 @@ namespace dtk
     }
   }
 
-  size_t raster_strips(const ImgDesc& d)
+  size_t raster_strips(const ImgDesc& d, const KParams& p)
   {
-    const int rs = (d.H + kStripR - 1) / kStripR;
-    const int cs = (d.W + kStripR - 1) / kStripR;
+    const int R  = raster_owned(raster_cm(p));
+    const int rs = (d.H + R - 1) / R;
+    const int cs = (d.W + R - 1) / R;
     return (size_t)(rs > cs ? rs : cs);
   }
 
-  size_t raster_mail_words(const ImgDesc& d) { return raster_strips(d) * 2 * 2 * kGhost; }
+  size_t raster_mail_words(const ImgDesc& d, const KParams& p)
+  {
+    return raster_strips(d, p) * 2 * 2 * raster_ghost(raster_cm(p));
+  }
 } // namespace dtk

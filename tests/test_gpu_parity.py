@@ -275,9 +275,12 @@ def test_fixed_iterations(dt, dev, oracle, iters, conn):
 
 FIXED_CASES = [
     # (img kind, mask kind, W, H, conn, lambda): every cost mode of the raster
-    # passes (lambda 0, 1, other), each image kind, C4 and C8; 177 columns
-    # give row passes of 176 steps (whole 16-step chunks), 97 rows column
-    # passes of 96, the other sizes a partial last chunk.
+    # passes (lambda 0, 1, other), each image kind, C4 and C8.  Whole last
+    # chunks: 161 x 101 at lambda > 0 (row passes of 160 steps, column
+    # passes of 100, in 20-step chunks) and 177 x 97 at lambda 0 (176 and 96
+    # steps in 16-step chunks); the other sizes end with a partial chunk.
+    ("u8", "u8", 161, 101, 8, 0.5),
+    ("f32", "f32", 161, 101, 8, 1.0),
     ("u8", "u8", 177, 97, 8, 0.0),
     ("u8", "u8", 177, 97, 8, 1.0),
     ("u8", "u8", 177, 97, 4, 0.5),
@@ -306,7 +309,7 @@ def test_fixed_iterations_kinds(dt, dev, oracle, case, iters):
 @pytest.mark.parametrize("case", [("u8", 700, 300, 8, 0.5), ("u8", 300, 700, 8, 0.5), ("f32", 517, 261, 4, 1.0),
                                   ("rgb8", 333, 444, 8, 0.0)], ids=lambda c: "-".join(map(str, c)))
 def test_fixed_iterations_many_chunks(dt, dev, oracle, case):
-    """Two raster iterations on images whose passes span many 16-step chunks
+    """Two raster iterations on images whose passes span many chunks
     and many strips (the mailbox handoff, whole and partial chunks, the row
     vectors), in both orientations: bit-exact against the CPU schedule."""
     ik, W, H, conn, lam = case
@@ -320,7 +323,7 @@ def test_fixed_iterations_many_chunks(dt, dev, oracle, case):
 
 @pytest.mark.parametrize("shape", [(1, 1), (1, 9), (9, 1), (62, 14), (63, 15), (64, 16), (2, 70), (70, 2),
                                    (62, 60), (63, 61), (64, 62), (70, 121), (125, 31), (130, 241), (1100, 70),
-                                   (70, 1100)])
+                                   (70, 1100), (24, 25), (25, 24), (48, 49), (49, 48)])
 @pytest.mark.parametrize("sched", ["raster", "blocks", "task"])
 @pytest.mark.parametrize("lam", [0.5, 0.0])
 def test_edge_shapes(dt, dev, oracle, shape, sched, lam):
