@@ -8,8 +8,8 @@
 // agree bit-for-bit (and at convergence for any N).
 //
 // MI355X mapping.  A pass is split into strips of R = 64 - 2G lines (rows
-// for a row pass, columns for a column pass; G = 16, R = 32 at lambda 0,
-// G = 20, R = 24 otherwise).  One wave holds 64 lanes = its R lines plus G
+// for a row pass, columns for a column pass; G = 20, R = 24 for row passes
+// with image costs, G = 16, R = 32 otherwise).  One wave holds 64 lanes = its R lines plus G
 // ghost lines on each side, and advances in chunks of C = G steps: at step
 // j of a chunk, lane l is exact when lanes l-1..l+1 were exact at step j-1,
 // so after C steps the R owned lanes are exact even though the outermost
@@ -36,18 +36,21 @@ namespace dtk
 {
   namespace
   {
-    // Ghost lines per side (== steps per chunk) of the passes of cost mode
-    // CM: 16 for the chamfer steps of lambda 0, 20 for image costs.  A chunk
-    // costs a fixed handshake plus a part that grows with its steps; with
-    // image costs the handshake weighs more, and the fewer chunks of G = 20
-    // pay (4-5% on the synthetic corpus against G = 16, 11% slower at
-    // lambda 0).  20 is the widest multiple of four (for the row vectors)
-    // whose ghost lines still lie in the two neighbour strips.
-    __host__ __device__ constexpr int raster_ghost(int cm) { return cm == 0 ? 16 : 20; }
-    __host__ __device__ constexpr int raster_owned(int cm) { return 64 - 2 * raster_ghost(cm); }
+    // Ghost lines per side (== steps per chunk) of a pass: 20 for row passes
+    // with image costs, 16 otherwise.  A chunk costs a fixed handshake plus a
+    // part that grows with its steps.  Row passes with image costs gain from
+    // fewer chunks (7-10% per pass on the synthetic corpus); column passes
+    // load one row segment per strip and step, and the narrower strips of
+    // G = 20 (24 owned lines instead of 32) load a third more of them (2-3%
+    // slower per pass), and at lambda 0 G = 20 is 11% slower.  20 is the
+    // widest multiple of four (for the row vectors) whose ghost lines still
+    // lie in the two neighbour strips.
+    __host__ __device__ constexpr int raster_ghost(int cm, bool rows) { return cm != 0 && rows ? 20 : 16; }
+    __host__ __device__ constexpr int raster_owned(int cm, bool rows) { return 64 - 2 * raster_ghost(cm, rows); }
     // A strip waits only for its two neighbours, so its ghost lines must lie
     // within them.
-    static_assert(raster_ghost(0) <= raster_owned(0) && raster_ghost(2) <= raster_owned(2),
+    static_assert(raster_ghost(0, true) <= raster_owned(0, true) && raster_ghost(2, true) <= raster_owned(2, true) &&
+                    raster_ghost(2, false) <= raster_owned(2, false),
                   "ghost lines must not reach beyond the neighbour strips");
   }
 
@@ -235,8 +238,8 @@ namespace dtk
                                                       unsigned long long* mail, uint32_t epoch)
   {
     using T                 = typename RasterPix<KIND>::T;
-    constexpr int kGhost    = raster_ghost(CM);
-    constexpr int kStripR   = raster_owned(CM);
+    constexpr int kGhost    = raster_ghost(CM, ROWS);
+    constexpr int kStripR   = raster_owned(CM, ROWS);
     const int lane     = threadIdx.x;
     const int strip    = blockIdx.x;
     const int nlines   = ROWS ? d.H : d.W; // lines (lanes) dimension
@@ -485,9 +488,9 @@ namespace dtk
   static hipError_t raster_iter_t(const ImgDesc& d, const KParams& p, uint32_t* changed, unsigned long long* mail,
                                   uint32_t pass0, hipStream_t s)
   {
-    constexpr int R  = raster_owned(CM);
-    const int     rs = (d.H + R - 1) / R; // row-pass strips
-    const int     cs = (d.W + R - 1) / R; // column-pass strips
+    constexpr int RR = raster_owned(CM, true), RC = raster_owned(CM, false);
+    const int     rs = (d.H + RR - 1) / RR; // row-pass strips
+    const int     cs = (d.W + RC - 1) / RC; // column-pass strips
     hipLaunchKernelGGL((k_raster_pass<KIND, true, true, C8, CM>), dim3(rs), dim3(64), 0, s, d, p, changed, mail,
                        (pass0 + 0u) << 16);
     hipLaunchKernelGGL((k_raster_pass<KIND, true, false, C8, CM>), dim3(rs), dim3(64), 0, s, d, p, changed, mail,
@@ -590,14 +593,19 @@ namespace dtk
 
   size_t raster_strips(const ImgDesc& d, const KParams& p)
   {
-    const int R  = raster_owned(raster_cm(p));
-    const int rs = (d.H + R - 1) / R;
-    const int cs = (d.W + R - 1) / R;
+    const int cm = raster_cm(p);
+    const int RR = raster_owned(cm, true), RC = raster_owned(cm, false);
+    const int rs = (d.H + RR - 1) / RR;
+    const int cs = (d.W + RC - 1) / RC;
     return (size_t)(rs > cs ? rs : cs);
   }
 
   size_t raster_mail_words(const ImgDesc& d, const KParams& p)
   {
-    return raster_strips(d, p) * 2 * 2 * raster_ghost(raster_cm(p));
+    const int    cm = raster_cm(p);
+    const int    RR = raster_owned(cm, true), RC = raster_owned(cm, false);
+    const size_t wr = (size_t)((d.H + RR - 1) / RR) * 2 * 2 * raster_ghost(cm, true);
+    const size_t wc = (size_t)((d.W + RC - 1) / RC) * 2 * 2 * raster_ghost(cm, false);
+    return wr > wc ? wr : wc;
   }
 } // namespace dtk

@@ -275,10 +275,11 @@ def test_fixed_iterations(dt, dev, oracle, iters, conn):
 
 FIXED_CASES = [
     # (img kind, mask kind, W, H, conn, lambda): every cost mode of the raster
-    # passes (lambda 0, 1, other), each image kind, C4 and C8.  Whole last
-    # chunks: 161 x 101 at lambda > 0 (row passes of 160 steps, column
-    # passes of 100, in 20-step chunks) and 177 x 97 at lambda 0 (176 and 96
-    # steps in 16-step chunks); the other sizes end with a partial chunk.
+    # passes (lambda 0, 1, other), each image kind, C4 and C8.  Row passes
+    # with image costs run 20-step chunks, the other passes 16-step chunks.
+    # Whole last chunks: the row passes of 161 columns (160 steps) at
+    # lambda > 0, of 177 columns (176 steps) at lambda 0, and the column
+    # passes of 97 rows (96 steps); the other sizes end with a partial chunk.
     ("u8", "u8", 161, 101, 8, 0.5),
     ("f32", "f32", 161, 101, 8, 1.0),
     ("u8", "u8", 177, 97, 8, 0.0),
