@@ -155,7 +155,10 @@ const char* dtk_version(void);
  * dt/include/dt/generalised_distance_transform.hpp:7-8),
  * dt::generalised_distance_transform_blocks (BLOCKS, :13-15) and
  * dt::generalised_distance_transform_task (TASK, :21-23).  The converged
- * result is identical for every schedule. */
+ * result is identical for every schedule.  The RASTER schedule keeps every
+ * strip of a pass resident at once; on a 256-CU MI355X that allows at
+ * least about 73 000 rows and 98 000 columns (C8 with lambda > 0, the
+ * tightest case), and a larger image gives DTK_ERR_UNSUPPORTED. */
 dtk_status dtk_ggdt(const dtk_image* img, const dtk_image* mask, dtk_image* dist, const dtk_params* p, void* stream,
                     dtk_stats* stats);
 
