@@ -13,7 +13,10 @@ The JSON line also carries:
                 average duration, from HIP events recorded on its stream
                 during the timed steps; `traffic` comes from the rocprofv3
                 FETCH_SIZE / WRITE_SIZE summary committed under profiles/
-                (null when absent);
+                (null when absent); operator_frac is the same figure for
+                the whole transform (the setup kernels' time added; its
+                compulsory bytes are also 11 B/px: image 3 + mask 4 +
+                D write 4);
   cpu_baseline  the CPU oracle (a restatement of the reference's raster GGDT)
                 on the same 4096x4096 image at lambda=0.5, OpenMP over the
                 box's cores (at most 16), median of 5 runs after a warm-up,
@@ -191,6 +194,10 @@ def main(argv=None) -> int:
         "traffic": traffic.get("bytes_per_launch") if traffic else None,
         "algorithmic_bytes_per_launch": BYTES_PER_PX * npx, "kernel_avg_ms": round(solve_avg_ms, 4),
         "setup_avg_ms": round(setup_avg_ms, 4), "kernel_launches": kt.calls,
+        # The whole transform (reset, init, seed and solve): its compulsory
+        # bytes (image 3 + mask 4 + D write 4 = 11 B/px) over setup + solve.
+        "operator_achieved": round(BYTES_PER_PX * npx / ((solve_avg_ms + setup_avg_ms) * 1e-3) / 1e9, 2),
+        "operator_frac": round(BYTES_PER_PX * npx / ((solve_avg_ms + setup_avg_ms) * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
     }
     log(f"timed: {ms_per_step:.3f} ms/step, {value:.1f} Mpix/s; k_task avg {solve_avg_ms:.3f} ms "
         f"-> {achieved:.1f} GB/s algorithmic")
