@@ -37,13 +37,13 @@ namespace dtk
   namespace
   {
     // Mailbox polls kept in flight while a chunk waits for its neighbours:
-    // 3 with image costs (spaced by s_sleep 4), 1 at lambda 0.  A poll that
+    // 3 with image costs, 2 at lambda 0 (spaced by s_sleep 4).  A poll that
     // arrives before the neighbour's publish then costs one spacing instead
-    // of a round trip (3.5% faster at lambda 0.5, 2.4% at lambda 1 on the
-    // synthetic corpus; 2 or 4 polls gain less, 6 loses); at lambda 0 extra
-    // polls cost up to 16%, and issuing the first poll before the chunk's
-    // loads was slower at every lambda.
-    __host__ __device__ constexpr int raster_poll_train(int cm) { return cm != 0 ? 3 : 1; }
+    // of a round trip (on the synthetic corpus 3.5% faster at lambda 0.5,
+    // 2.4% at lambda 1 and 1.6% at lambda 0; with image costs 2 or 4 polls
+    // gain less and 6 lose, at lambda 0 4 polls lose 16%).  Issuing the
+    // first poll before the chunk's loads was slower at every lambda.
+    __host__ __device__ constexpr int raster_poll_train(int cm) { return cm != 0 ? 3 : 2; }
     constexpr int                     kPollNap = 4;
     // Ghost lines per side (== steps per chunk) of a pass: 20 for row passes
     // with image costs, 16 otherwise.  A chunk costs a fixed handshake plus a
@@ -407,9 +407,9 @@ namespace dtk
         float got  = __uint_as_float((uint32_t)w);
         if constexpr (raster_poll_train(CM) > 1)
         {
-          // A train of polls: two more loads of the word in flight, each
-          // checked in turn (the oldest first, so each check waits for one
-          // load only) and reissued.
+          // A train of polls: one or two more loads of the word in flight,
+          // each checked in turn (the oldest first, so each check waits for
+          // one load only) and reissued.
           if (__ballot(!have) != 0)
           {
             constexpr int      kTrain = raster_poll_train(CM) - 1;
