@@ -405,48 +405,21 @@ namespace dtk
       {
         bool  have = !need || (uint32_t)(w >> 32) == want;
         float got  = __uint_as_float((uint32_t)w);
-        if constexpr (raster_poll_train(CM) > 1)
+        // A train of polls: one or two more loads of the word in flight, each
+        // checked in turn (the oldest first, so each check waits for one load
+        // only) and reissued.
+        static_assert(raster_poll_train(CM) > 1, "the wait keeps at least one more poll in flight");
+        if (__ballot(!have) != 0)
         {
-          // A train of polls: one or two more loads of the word in flight,
-          // each checked in turn (the oldest first, so each check waits for
-          // one load only) and reissued.
-          if (__ballot(!have) != 0)
+          constexpr int      kTrain = raster_poll_train(CM) - 1;
+          unsigned long long q[kTrain];
+#pragma unroll
+          for (int k = 0; k < kTrain; ++k)
           {
-            constexpr int      kTrain = raster_poll_train(CM) - 1;
-            unsigned long long q[kTrain];
-#pragma unroll
-            for (int k = 0; k < kTrain; ++k)
-            {
-              __builtin_amdgcn_s_sleep(kPollNap);
-              q[k] = have ? 0ull : __hip_atomic_load(&mail[at], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            for (uint32_t spin = 0; __ballot(!have) != 0; spin += kTrain)
-            {
-              if (spin >= kSpinLimit)
-              {
-                // changed[1] is the error word of a raster call.
-                if (lane == 0)
-                  __hip_atomic_fetch_or(&changed[1], (uint32_t)ERR_WATCHDOG, __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT);
-                break;
-              }
-#pragma unroll
-              for (int k = 0; k < kTrain; ++k)
-              {
-                if (!have && (uint32_t)(q[k] >> 32) == want)
-                {
-                  have = true;
-                  got  = __uint_as_float((uint32_t)q[k]);
-                }
-                __builtin_amdgcn_s_sleep(kPollNap);
-                q[k] = have ? 0ull : __hip_atomic_load(&mail[at], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              }
-            }
+            __builtin_amdgcn_s_sleep(kPollNap);
+            q[k] = have ? 0ull : __hip_atomic_load(&mail[at], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           }
-        }
-        else
-        {
-          for (uint32_t spin = 0; __ballot(!have) != 0; ++spin)
+          for (uint32_t spin = 0; __ballot(!have) != 0; spin += kTrain)
           {
             if (spin >= kSpinLimit)
             {
@@ -456,15 +429,16 @@ namespace dtk
                                       __HIP_MEMORY_SCOPE_AGENT);
               break;
             }
-            __builtin_amdgcn_s_sleep(1);
-            if (!have)
+#pragma unroll
+            for (int k = 0; k < kTrain; ++k)
             {
-              w = __hip_atomic_load(&mail[at], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              if ((uint32_t)(w >> 32) == want)
+              if (!have && (uint32_t)(q[k] >> 32) == want)
               {
                 have = true;
-                got  = __uint_as_float((uint32_t)w);
+                got  = __uint_as_float((uint32_t)q[k]);
               }
+              __builtin_amdgcn_s_sleep(kPollNap);
+              q[k] = have ? 0ull : __hip_atomic_load(&mail[at], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
           }
         }
