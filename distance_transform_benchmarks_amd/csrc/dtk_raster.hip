@@ -37,14 +37,15 @@ namespace dtk
   namespace
   {
     // Mailbox polls kept in flight while a chunk waits for its neighbours:
-    // 3 with image costs, 2 at lambda 0 (spaced by s_sleep 4).  A poll that
-    // arrives before the neighbour's publish then costs one spacing instead
-    // of a round trip (on the synthetic corpus 3.5% faster at lambda 0.5,
-    // 2.4% at lambda 1 and 1.6% at lambda 0; with image costs 2 or 4 polls
-    // gain less and 6 lose, at lambda 0 4 polls lose 16%).  Issuing the
-    // first poll before the chunk's loads was slower at every lambda.
+    // 3 spaced by s_sleep 4 with image costs, 2 spaced by s_sleep 2 at
+    // lambda 0.  A poll that arrives before the neighbour's publish then
+    // costs one spacing instead of a round trip (on the synthetic corpus
+    // 3.5% faster at lambda 0.5, 2.4% at lambda 1 and 2.6% at lambda 0; with
+    // image costs 2 or 4 polls gain less and 6 lose, at lambda 0 4 polls
+    // lose 16% and a spacing of 8 11%).  Issuing the first poll before the
+    // chunk's loads was slower at every lambda.
     __host__ __device__ constexpr int raster_poll_train(int cm) { return cm != 0 ? 3 : 2; }
-    constexpr int                     kPollNap = 4;
+    __host__ __device__ constexpr int raster_poll_nap(int cm) { return cm != 0 ? 4 : 2; }
     // Ghost lines per side (== steps per chunk) of a pass: 20 for row passes
     // with image costs, 16 otherwise.  A chunk costs a fixed handshake plus a
     // part that grows with its steps.  Row passes with image costs gain from
@@ -416,7 +417,7 @@ namespace dtk
 #pragma unroll
           for (int k = 0; k < kTrain; ++k)
           {
-            __builtin_amdgcn_s_sleep(kPollNap);
+            __builtin_amdgcn_s_sleep(raster_poll_nap(CM));
             q[k] = have ? 0ull : __hip_atomic_load(&mail[at], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           }
           for (uint32_t spin = 0; __ballot(!have) != 0; spin += kTrain)
@@ -437,7 +438,7 @@ namespace dtk
                 have = true;
                 got  = __uint_as_float((uint32_t)q[k]);
               }
-              __builtin_amdgcn_s_sleep(kPollNap);
+              __builtin_amdgcn_s_sleep(raster_poll_nap(CM));
               q[k] = have ? 0ull : __hip_atomic_load(&mail[at], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
           }
